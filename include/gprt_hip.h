@@ -1,0 +1,279 @@
+/*
+ * gprt_hip.h -- C ABI of libgprt_hip.so, the MI355X (gfx950) spatial-search
+ * backend.  Plain pointers and sizes only.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The 13 symbols of the reference's native FFI, same names and argument
+ *     types, so the Rust wrapper crate can bind this library with its extern
+ *     block unchanged (gprt-optix/src/lib.rs:13-31, implemented in
+ *     gprt-optix/host/main.cpp:101-415).  They keep the reference's
+ *     no-return-value signatures; failures are reported through
+ *     gprt_hip_last_error() instead of exit().
+ *
+ *  2. gprt_hip_* entry points returning an int status (GPRT_OK == 0).  These
+ *     carry the hot path: LBVH build, exact k-NN (plain FP64 form and the
+ *     batched FP32 + FP64-recheck form), fixed-radius range query, Barnes-Hut
+ *     forces, the radius samplers, Morton codes and run statistics.
+ *
+ * Result contract shared by every k-NN / range entry point:
+ *   d64(q,p) = ((dx*dx + dy*dy) + dz*dz) in binary64 with dx = (double)q.x -
+ *   (double)p.x, each operation rounded to nearest.  k-NN returns the k points
+ *   with the smallest (d64, id) pairs in ascending order; self-matches are
+ *   included (gprt-macros/src/lib.rs:341 has no d > 0 filter).  Missing slots
+ *   (cloud smaller than k) hold 0xFFFFFFFF and +inf.  A range query returns all
+ *   points with d64 <= (double)r * (double)r, ordered the same way.
+ *
+ * Pointer flags: GPRT_PTR_DEVICE means every array argument of the call is a
+ * device pointer on the context's device (inputs already resident in HBM);
+ * otherwise they are host pointers and the call copies through.
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not
+ * overlap (guard it with a mutex, as the Rust wrapper does for its caches,
+ * gprt-optix/src/lib.rs:35-40).  Different contexts may run concurrently.
+ */
+#ifndef GPRT_HIP_H
+#define GPRT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPRT_HIP_API __attribute__((visibility("default")))
+
+/* ---------------- status codes ---------------- */
+#define GPRT_OK 0
+#define GPRT_ERR_ARG 1      /* invalid argument (null pointer, k <= 0, ...) */
+#define GPRT_ERR_INPUT 2    /* NaN / Inf / |coordinate| > 1e18 in the input */
+#define GPRT_ERR_HIP 3      /* HIP runtime failure */
+#define GPRT_ERR_ALLOC 4    /* device allocation failed */
+#define GPRT_ERR_CAPACITY 5 /* output capacity too small (range query) */
+#define GPRT_ERR_STATE 6    /* object in the wrong state */
+
+#define GPRT_PTR_HOST 0
+#define GPRT_PTR_DEVICE 1
+/* with GPRT_PTR_DEVICE: only enqueue on the context stream and return; the
+ * caller synchronises with gprt_hip_ctx_sync() (statistics are then partial) */
+#define GPRT_NO_SYNC 2
+/* gprt_hip_knn_plain: count node visits and point tests into the statistics */
+#define GPRT_COLLECT_COUNTERS 4
+
+#define GPRT_ID_NONE 0xFFFFFFFFu
+
+/* RadiusHeuristic of gprt-ir/src/lib.rs:33-37, plus INFINITE (one unbounded
+ * round, no sampler) */
+#define GPRT_RADIUS_MAX 0
+#define GPRT_RADIUS_PERCENTILE 1
+#define GPRT_RADIUS_FIXED 2
+#define GPRT_RADIUS_INFINITE 3
+
+/* initial-radius sampler */
+#define GPRT_SAMPLER_LCG 0     /* gprt-optix/src/dispatch.rs:65-103 */
+#define GPRT_SAMPLER_STRIDED 1 /* gprt-macros/src/lib.rs:417-450 */
+
+/* traversal mapping */
+#define GPRT_TRAV_THREAD 0 /* one query per lane */
+#define GPRT_TRAV_WAVE 1   /* one query per wavefront, lanes test a leaf together */
+
+typedef struct gprt_hip_ctx gprt_hip_ctx;
+typedef struct gprt_hip_tree gprt_hip_tree;
+
+/* Tunable knobs of the batched k-NN form.  Superset of the reference's
+ * Schedule (gprt-ir/src/lib.rs:58-89): radius_heuristic/radius_param map to
+ * RadiusHeuristic, radius_mult to radius_increment_mult, sort_queries to
+ * use_morton_lbv.  max_hits_per_query and memory_strategy have no role here:
+ * candidates live in registers and the output is k ids per query. */
+typedef struct {
+    int sampler;          /* GPRT_SAMPLER_*, default LCG */
+    int radius_heuristic; /* GPRT_RADIUS_*, default PERCENTILE */
+    float radius_param;   /* percentile p (default 0.95) or the fixed radius */
+    float radius_mult;    /* radius growth per round (default 3.0) */
+    int max_rounds;       /* bounded rounds before the final unbounded one (default 6) */
+    int traversal;        /* GPRT_TRAV_*, default THREAD */
+    int stack_depth;      /* LDS traversal-stack entries per query (default 16) */
+    int guard_slots;      /* spare candidate slots beyond k (default 3) */
+    int sort_queries;     /* process queries in Morton order (default 1) */
+    int use_graph;        /* replay the pipeline as a captured hipGraph (default 1) */
+    int grid_blocks;      /* persistent grid size, 0 = automatic */
+    int block_threads;    /* 0 = automatic (256) */
+    int collect_counters; /* count node visits / point tests (slower; for the roofline) */
+} gprt_hip_config;
+
+typedef struct {
+    int rounds_run;            /* bounded rounds that had work */
+    int64_t queries;           /* queries in the last call */
+    int64_t fallback_queries;  /* queries re-ranked by the exact FP64 kernel */
+    int64_t partial_queries;   /* queries whose cloud holds fewer than k points */
+    int64_t active_per_round[16];
+    float initial_radius;      /* sampled radius of cloud 0 */
+    double ms_total;           /* device time of the last call (hipEvents) */
+    double ms_query;           /* query phase only (resident -> resident) */
+    double ms_build;           /* last tree build */
+    int64_t node_visits;       /* only with counters enabled */
+    int64_t point_tests;
+} gprt_hip_run_stats;
+
+typedef struct {
+    int64_t n;
+    int nclouds;
+    int leaf_size;
+    float bbox[6];  /* cloud 0: min xyz, max xyz */
+    int key_bits;
+} gprt_hip_tree_desc;
+
+/* ---------------- library / context ---------------- */
+GPRT_HIP_API int gprt_hip_version(void);
+GPRT_HIP_API const char* gprt_hip_last_error(void);
+GPRT_HIP_API void gprt_hip_config_default(gprt_hip_config* cfg);
+GPRT_HIP_API int gprt_hip_ctx_create(int device, gprt_hip_ctx** out);
+GPRT_HIP_API int gprt_hip_ctx_destroy(gprt_hip_ctx* ctx);
+GPRT_HIP_API int gprt_hip_ctx_sync(gprt_hip_ctx* ctx);
+GPRT_HIP_API int gprt_hip_stats(gprt_hip_ctx* ctx, gprt_hip_run_stats* out);
+GPRT_HIP_API int gprt_hip_num_devices(int* out);
+
+/* Device buffers on the context's device, for callers that keep inputs and
+ * outputs resident in HBM (GPRT_PTR_DEVICE).  kind: 0 host->device,
+ * 1 device->host, 2 device->device.  Copies are ordered on the context stream
+ * and complete before the call returns. */
+GPRT_HIP_API int gprt_hip_device_alloc(gprt_hip_ctx* ctx, size_t bytes, void** out);
+GPRT_HIP_API int gprt_hip_device_free(gprt_hip_ctx* ctx, void* ptr);
+GPRT_HIP_API int gprt_hip_copy(gprt_hip_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* ---------------- LBVH ---------------- */
+/* Build a GPU LBVH over n points (xyz float32, 12 bytes each): 30-bit Morton
+ * codes normalised per axis as gprt-core/src/morton.rs:19-48, stable LSD radix
+ * sort, Karras hierarchy, atomic bottom-up refit.  Replaces the GAS build of
+ * gprt_bvh_build (gprt-optix/host/main.cpp:200-248) and Scene::build
+ * (gprt-core/src/lib.rs:135-150).  leaf_size in [1, 32]. */
+GPRT_HIP_API int gprt_hip_tree_build(gprt_hip_ctx* ctx, const float* pts, int64_t n, int leaf_size, int ptr_flags,
+                                     gprt_hip_tree** out);
+/* Many independent clouds in one tree: cloud c owns points
+ * [cloud_offsets[c], cloud_offsets[c+1]) of pts (host array of nclouds+1).
+ * Each cloud gets its own Morton normalisation and its own subtree. */
+GPRT_HIP_API int gprt_hip_tree_build_batched(gprt_hip_ctx* ctx, const float* pts, const int64_t* cloud_offsets,
+                                             int nclouds, int leaf_size, int ptr_flags, gprt_hip_tree** out);
+GPRT_HIP_API int gprt_hip_tree_destroy(gprt_hip_tree* tree);
+GPRT_HIP_API int gprt_hip_tree_info(const gprt_hip_tree* tree, gprt_hip_tree_desc* out);
+/* Host copies of the tree for invariant checks: sorted keys (n), permutation
+ * (n), nodes (16 floats per internal node, max(n-1,0) of them), roots (2 ints
+ * per cloud).  Any pointer may be NULL. */
+GPRT_HIP_API int gprt_hip_tree_export(gprt_hip_tree* tree, uint64_t* keys, uint32_t* perm, float* nodes,
+                                      int32_t* roots);
+
+/* ---------------- exact k-NN ---------------- */
+/* Batched FP32 + FP64-recheck form (the k_nn! driver of
+ * gprt-macros/src/lib.rs:408-604 with TrueKNN rounds on the device): sampled
+ * initial radius, Morton-ordered queries, FP32 traversal keeping a guard band
+ * of candidates, FP64 re-rank by (d64, id), on-device compaction of unresolved
+ * queries between rounds, queries whose band overflows re-ranked by the exact
+ * kernel.  ids_out: nq*k; d2_out (may be NULL): nq*k binary64 d64 values. */
+GPRT_HIP_API int gprt_hip_knn(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq, int k,
+                              const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int ptr_flags);
+/* Plain single-launch form: one query per lane in input order, FP64 distance
+ * arithmetic throughout, initial radius +inf, no compaction, no graph. */
+GPRT_HIP_API int gprt_hip_knn_plain(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq,
+                                    int k, uint32_t* ids_out, double* d2_out, int ptr_flags);
+/* Uneven batches: query set c (rows [query_offsets[c], query_offsets[c+1]))
+ * searches cloud c of a batched tree with its own k = k_per_cloud[c] (1..64).
+ * Results of query set c start at out_offsets[c] = sum_{c'<c} nq_c' * k_c'.
+ * query_offsets, k_per_cloud: host arrays. */
+GPRT_HIP_API int gprt_hip_knn_batched(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries,
+                                      const int64_t* query_offsets, const int* k_per_cloud,
+                                      const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out,
+                                      int ptr_flags);
+
+/* ---------------- fixed-radius range query ---------------- */
+/* r_nn! / range_query! (gprt-macros/src/lib.rs:729-750, which as shipped
+ * returns one flat id list without per-query split).  Phase 1: ids_out ==
+ * NULL fills offsets_out[0..nq] (exclusive prefix of per-query counts).
+ * Phase 2: ids_out with capacity ids_capacity >= offsets_out[nq] receives the
+ * ids of query q at [offsets_out[q], offsets_out[q+1]), ordered by (d64, id).
+ * offsets_out and ids_out follow ptr_flags. */
+GPRT_HIP_API int gprt_hip_range(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq,
+                                float radius, int64_t* offsets_out, uint32_t* ids_out, int64_t ids_capacity,
+                                int ptr_flags);
+
+/* ---------------- Barnes-Hut ---------------- */
+/* Accelerations a_i = sum_j G m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^1.5
+ * with the Barnes-Hut approximation over an LBVH of the bodies: a node of
+ * size s (largest box edge) at distance d from x_i to its centre of mass is
+ * taken whole when s < theta * d.  FP32 traversal, compensated (Neumaier)
+ * accumulation.  pos_mass: float4 (x, y, z, m); acc_out: 3 floats per body.
+ * Replaces barnes_hut! (gprt-macros/src/lib.rs:752-834). */
+GPRT_HIP_API int gprt_hip_bh_forces(gprt_hip_ctx* ctx, const float* pos_mass, int64_t n, float theta, float eps,
+                                    float G, int leaf_size, float* acc_out, int ptr_flags);
+
+/* ---------------- pieces exposed for parity tests ---------------- */
+/* 30-bit Morton codes of n points with the bbox of those points
+ * (gprt-core/src/morton.rs:19-48).  bbox_out: 6 floats (may be NULL). */
+GPRT_HIP_API int gprt_hip_morton_codes(gprt_hip_ctx* ctx, const float* pts, int64_t n, uint32_t* codes_out,
+                                       float* bbox_out, int ptr_flags);
+/* Initial radius of the given sampler / heuristic (k is used by the strided
+ * sampler only).  Runs on the device. */
+GPRT_HIP_API int gprt_hip_sample_radius(gprt_hip_ctx* ctx, const float* pts, int64_t n, const float* queries,
+                                        int64_t nq, int sampler, int heuristic, float param, int k,
+                                        float* radius_out, int ptr_flags);
+/* Stable LSD radix sort of (key, value) pairs by bits [begin_bit, end_bit). */
+GPRT_HIP_API int gprt_hip_sort_pairs(gprt_hip_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                                     int end_bit, int ptr_flags);
+
+/* ---------------- autotuner ---------------- */
+/* Times candidate configurations (traversal mapping, stack depth, leaf size,
+ * radius percentile and multiplier) on the given data and queries and writes
+ * the fastest to cfg_out and *leaf_size_out; with json_path != NULL also
+ * writes it as a Schedule-superset JSON file.  Replaces gprt_autotune!
+ * (gprt-macros/src/lib.rs:120-214) and the funnel profiler
+ * (gprt-profiler/src/main.rs:37-133). */
+GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t n, const float* queries, int64_t nq,
+                                   int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
+                                   const char* json_path, int ptr_flags);
+
+/* ---------------- the reference FFI (gprt-optix/src/lib.rs:13-31) ---------------- */
+typedef struct CGprtPipeline CGprtPipeline;
+typedef struct CGprtBvh CGprtBvh;
+
+/* main.cpp:101-180.  ir_data selects the device program: NULL/empty or a
+ * descriptor "gprt-hip:knn-heap" (register-heap k-NN, default),
+ * "gprt-hip:push" (per-query push of every hit), "gprt-hip:autorope" (the
+ * Barnes-Hut walk).  Any other bytes (an OptiX-IR blob) select the default. */
+GPRT_HIP_API CGprtPipeline* gprt_pipeline_create(const void* ir_data, size_t ir_size, int is_hw_triangle);
+GPRT_HIP_API void gprt_pipeline_destroy(CGprtPipeline* pipe);
+/* main.cpp:200-248: h_geom holds float4 spheres (x, y, z, r); the LBVH is
+ * built over the centres and the sphere radius is recorded. */
+GPRT_HIP_API CGprtBvh* gprt_bvh_build(CGprtPipeline* pipe, const float* h_geom, const float* h_aabbs, int count,
+                                      int geom_bytes);
+/* main.cpp:250-252: builds over the triangle AABB centres. */
+GPRT_HIP_API CGprtBvh* gprt_bvh_build_triangles(CGprtPipeline* pipe, const float* h_verts,
+                                                const unsigned int* h_indices, int num_triangles,
+                                                const float* h_aabbs);
+/* main.cpp:254-286: records the new sphere radius; no rebuild is needed. */
+GPRT_HIP_API void gprt_bvh_refit(CGprtPipeline* pipe, CGprtBvh* bvh, float radius, int count);
+GPRT_HIP_API void gprt_bvh_destroy(CGprtBvh* bvh);
+/* main.cpp:296-305 */
+GPRT_HIP_API void gprt_register_array(CGprtPipeline* pipe, const char* name, unsigned int capacity_per_query,
+                                      int num_queries);
+/* main.cpp:307-311 */
+GPRT_HIP_API void gprt_register_value(CGprtPipeline* pipe, const char* name);
+/* main.cpp:313-341: h_queries holds float4 (x, y, z, radius). */
+GPRT_HIP_API void gprt_execute(CGprtPipeline* pipe, CGprtBvh* bvh, const float* h_queries, int count);
+/* main.cpp:343-353 */
+GPRT_HIP_API void gprt_retrieve_array_lengths(CGprtPipeline* pipe, const char* name, unsigned int* h_lengths,
+                                              int num_queries);
+/* main.cpp:355-365 */
+GPRT_HIP_API void gprt_retrieve_array_flat(CGprtPipeline* pipe, const char* name, unsigned int* h_out,
+                                           int total_elements);
+/* main.cpp:367-369 */
+GPRT_HIP_API void gprt_retrieve_value(CGprtPipeline* pipe, const char* name, unsigned int* h_out);
+/* main.cpp:371-415: walks the caller's flattened octree (Body 28 B,
+ * NodeSpatial 16 B, NodeRouting 32 B) with its autorope links. */
+GPRT_HIP_API void gprt_execute_autorope_soa_ffi(CGprtPipeline* pipe, const void* h_bodies, int num_bodies,
+                                                const void* h_spatial, int num_nodes, const void* h_routing,
+                                                float theta, void* h_out_forces);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPRT_HIP_H */

@@ -1,0 +1,843 @@
+// api.hip -- the gprt_hip_* C ABI: contexts, trees, and the k-NN drivers.
+//
+// The batched driver is the device-side restatement of the k_nn! round loop
+// (gprt-macros/src/lib.rs:408-604): sampler -> query order -> bounded rounds
+// with growing radius -> unbounded last round, with no host round trip between
+// rounds.  The whole sequence is enqueued on the context stream and, when
+// cfg.use_graph is set, captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "gprt_internal.hpp"
+#include "knn_internal.hpp"
+
+struct gprt_hip_ctx {
+    gprt::Ctx c;
+};
+struct gprt_hip_tree {
+    gprt::Tree t;
+    gprt_hip_ctx* owner = nullptr;
+};
+
+namespace gprt {
+
+static thread_local std::string g_err_msg;
+
+void set_error(int code, const std::string& msg) {
+    (void)code;
+    g_err_msg = msg;
+}
+int fail(int code, const std::string& msg) {
+    set_error(code, msg);
+    return code;
+}
+
+int DevBuf::reserve(size_t need) {
+    if (need <= bytes && ptr) return GPRT_OK;
+    release();
+    size_t b = std::max<size_t>(need, 256);
+    b = (b + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        return fail(GPRT_ERR_ALLOC, "hipMalloc of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+    return GPRT_OK;
+}
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// workspace slots of a context
+enum {
+    WS_QKEYS = 0,  // uint64[nq]
+    WS_QPERM,      // uint32[nq]
+    WS_QS4,        // float4[nq]
+    WS_LISTA,      // int[nq]
+    WS_LISTB,      // int[nq]
+    WS_FB,         // int[nq]
+    WS_MISC,       // radius/kth/offsets
+    WS_BBOXTMP,    // tree-build scratch
+};
+
+// extra per-context buffers beyond the 8 generic slots
+struct CtxExtra {
+    DevBuf q_in, ids_out, d2_out, qcloud, kc, qoff, outbase, flag;
+};
+static std::mutex g_extra_mu;
+static std::vector<std::pair<Ctx*, CtxExtra*>> g_extra;
+static CtxExtra* extra(Ctx* c) {
+    std::lock_guard<std::mutex> lk(g_extra_mu);
+    for (auto& p : g_extra)
+        if (p.first == c) return p.second;
+    auto* e = new CtxExtra();
+    g_extra.push_back({c, e});
+    return e;
+}
+static void drop_extra(Ctx* c) {
+    std::lock_guard<std::mutex> lk(g_extra_mu);
+    for (size_t i = 0; i < g_extra.size(); ++i)
+        if (g_extra[i].first == c) {
+            CtxExtra* e = g_extra[i].second;
+            for (DevBuf* b : {&e->q_in, &e->ids_out, &e->d2_out, &e->qcloud, &e->kc, &e->qoff, &e->outbase, &e->flag})
+                b->release();
+            delete e;
+            g_extra.erase(g_extra.begin() + i);
+            return;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// input validation: finite and |x| <= 1e18 (keeps every d^2 finite in FP32)
+// ---------------------------------------------------------------------------
+__global__ void validate_kernel(const float* __restrict__ v, int64_t m, unsigned* flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    for (; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        float x = v[i];
+        bad |= !(fabsf(x) <= 1e18f);
+    }
+    if (__ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
+}
+
+static void launch_validate(hipStream_t s, const float* v, int64_t m, unsigned* flag) {
+    if (m <= 0) return;
+    int blocks = (int)std::min<int64_t>((m + 255) / 256, 1024);
+    hipLaunchKernelGGL(validate_kernel, dim3(blocks), dim3(256), 0, s, v, m, flag);
+}
+
+__global__ void set_qoff_kernel(int64_t* qoff, int64_t nq) {
+    qoff[0] = 0;
+    qoff[1] = nq;
+}
+
+}  // namespace gprt
+
+using namespace gprt;
+
+// ===========================================================================
+// library / context
+// ===========================================================================
+extern "C" int gprt_hip_version(void) { return 10000; }
+
+extern "C" const char* gprt_hip_last_error(void) { return g_err_msg.c_str(); }
+
+extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
+    if (!c) return;
+    c->sampler = GPRT_SAMPLER_LCG;
+    c->radius_heuristic = GPRT_RADIUS_PERCENTILE;
+    c->radius_param = 0.95f;
+    c->radius_mult = 3.0f;
+    c->max_rounds = 6;
+    c->traversal = GPRT_TRAV_THREAD;
+    c->stack_depth = 16;
+    c->guard_slots = 3;
+    c->sort_queries = 1;
+    c->use_graph = 1;
+    c->grid_blocks = 0;
+    c->block_threads = 0;
+    c->collect_counters = 0;
+}
+
+extern "C" int gprt_hip_num_devices(int* out) {
+    if (!out) return fail(GPRT_ERR_ARG, "null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *out = n;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
+    if (!out) return fail(GPRT_ERR_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(GPRT_ERR_HIP, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return fail(GPRT_ERR_ARG, "device index out of range");
+    GPRT_HIP_TRY(hipSetDevice(device));
+    auto* h = new gprt_hip_ctx();
+    Ctx& c = h->c;
+    c.device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
+        hipEventCreate(&c.ev2) != hipSuccess || hipEventCreate(&c.ev3) != hipSuccess) {
+        delete h;
+        return fail(GPRT_ERR_HIP, "stream/event creation failed");
+    }
+    if (c.counters.reserve(sizeof(unsigned) * CNT_TOTAL + 64) != GPRT_OK) {
+        delete h;
+        return GPRT_ERR_ALLOC;
+    }
+    *out = h;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
+    if (!h) return GPRT_OK;
+    Ctx& c = h->c;
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    for (auto& b : c.ws) b.release();
+    for (auto& b : c.sort_tmp) b.release();
+    c.counters.release();
+    drop_extra(&c);
+    for (hipEvent_t e : {c.ev0, c.ev1, c.ev2, c.ev3})
+        if (e) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete h;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_ctx_sync(gprt_hip_ctx* h) {
+    if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_device_alloc(gprt_hip_ctx* h, size_t bytes, void** out) {
+    if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
+    *out = nullptr;
+    GPRT_HIP_TRY(hipSetDevice(h->c.device));
+    if (hipMalloc(out, std::max<size_t>(bytes, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return fail(GPRT_ERR_ALLOC, "hipMalloc failed");
+    }
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_device_free(gprt_hip_ctx* h, void* p) {
+    if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    if (!p) return GPRT_OK;
+    GPRT_HIP_TRY(hipSetDevice(h->c.device));
+    GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
+    GPRT_HIP_TRY(hipFree(p));
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_copy(gprt_hip_ctx* h, void* dst, const void* src, size_t bytes, int kind) {
+    if (!h || (bytes && (!dst || !src))) return fail(GPRT_ERR_ARG, "null argument");
+    if (kind < 0 || kind > 2) return fail(GPRT_ERR_ARG, "bad copy kind");
+    if (!bytes) return GPRT_OK;
+    GPRT_HIP_TRY(hipSetDevice(h->c.device));
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    GPRT_HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, h->c.stream));
+    GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
+    return GPRT_OK;
+}
+
+// ===========================================================================
+// trees
+// ===========================================================================
+static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, int nclouds, int leaf_size,
+                        int flags, gprt_hip_tree** out) {
+    if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
+    *out = nullptr;
+    if (nclouds <= 0) return fail(GPRT_ERR_ARG, "nclouds must be positive");
+    if (leaf_size < 1 || leaf_size > 32) return fail(GPRT_ERR_ARG, "leaf_size must be in [1, 32]");
+    for (int c = 0; c < nclouds; ++c)
+        if (off[c + 1] < off[c] || off[0] != 0) return fail(GPRT_ERR_ARG, "cloud offsets must be non-decreasing from 0");
+    int64_t n = off[nclouds];
+    if (n > 0 && !pts) return fail(GPRT_ERR_ARG, "null points");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    CtxExtra* x = extra(&c);
+    const float* d_pts = pts;
+    if (!(flags & GPRT_PTR_DEVICE) && n > 0) {
+        GPRT_TRY(x->q_in.reserve(sizeof(float) * 3 * n));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c.stream));
+        d_pts = x->q_in.as<float>();
+    }
+    GPRT_TRY(x->flag.reserve(16));
+    GPRT_HIP_TRY(hipMemsetAsync(x->flag.ptr, 0, 16, c.stream));
+    launch_validate(c.stream, d_pts, 3 * n, x->flag.as<unsigned>());
+    unsigned bad = 0;
+    GPRT_HIP_TRY(hipMemcpyAsync(&bad, x->flag.ptr, 4, hipMemcpyDeviceToHost, c.stream));
+    GPRT_HIP_TRY(hipStreamSynchronize(c.stream));
+    if (bad) return fail(GPRT_ERR_INPUT, "points contain NaN/Inf or |coordinate| > 1e18");
+    auto* t = new gprt_hip_tree();
+    t->owner = h;
+    GPRT_HIP_TRY(hipEventRecord(c.ev2, c.stream));
+    int rc = tree_build(&c, d_pts, off, nclouds, leaf_size, &t->t);
+    if (rc != GPRT_OK) {
+        gprt_hip_tree_destroy(t);
+        return rc;
+    }
+    GPRT_HIP_TRY(hipEventRecord(c.ev3, c.stream));
+    GPRT_HIP_TRY(hipEventSynchronize(c.ev3));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c.ev2, c.ev3);
+    c.stats.ms_build = ms;
+    *out = t;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_tree_build(gprt_hip_ctx* h, const float* pts, int64_t n, int leaf_size, int flags,
+                                   gprt_hip_tree** out) {
+    if (n < 0) return fail(GPRT_ERR_ARG, "n < 0");
+    int64_t off[2] = {0, n};
+    return build_common(h, pts, off, 1, leaf_size, flags, out);
+}
+
+extern "C" int gprt_hip_tree_build_batched(gprt_hip_ctx* h, const float* pts, const int64_t* cloud_offsets,
+                                           int nclouds, int leaf_size, int flags, gprt_hip_tree** out) {
+    if (!cloud_offsets) return fail(GPRT_ERR_ARG, "null cloud_offsets");
+    return build_common(h, pts, cloud_offsets, nclouds, leaf_size, flags, out);
+}
+
+extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
+    if (!t) return GPRT_OK;
+    if (t->owner) (void)hipSetDevice(t->owner->c.device);
+    Tree& T = t->t;
+    for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.perm, &T.inv, &T.codes,
+                      &T.parent, &T.flags, &T.bh})
+        b->release();
+    delete t;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_tree_info(const gprt_hip_tree* t, gprt_hip_tree_desc* out) {
+    if (!t || !out) return fail(GPRT_ERR_ARG, "null tree/out");
+    out->n = t->t.n;
+    out->nclouds = t->t.nclouds;
+    out->leaf_size = t->t.leaf_size;
+    for (int a = 0; a < 6; ++a) out->bbox[a] = t->t.h_bbox[a];
+    out->key_bits = t->t.key_bits;
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_tree_export(gprt_hip_tree* t, uint64_t* keys, uint32_t* perm, float* nodes,
+                                    int32_t* roots) {
+    if (!t || !t->owner) return fail(GPRT_ERR_ARG, "null tree");
+    Ctx& c = t->owner->c;
+    Tree& T = t->t;
+    hipStream_t s = c.stream;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    if (keys && T.n) GPRT_HIP_TRY(hipMemcpyAsync(keys, T.codes.ptr, 8ull * T.n, hipMemcpyDeviceToHost, s));
+    if (perm && T.n) GPRT_HIP_TRY(hipMemcpyAsync(perm, T.perm.ptr, 4ull * T.n, hipMemcpyDeviceToHost, s));
+    if (nodes && T.n > 1)
+        GPRT_HIP_TRY(hipMemcpyAsync(nodes, T.nodes.ptr, sizeof(Node) * (T.n - 1), hipMemcpyDeviceToHost, s));
+    if (roots) GPRT_HIP_TRY(hipMemcpyAsync(roots, T.roots.ptr, 8ull * T.nclouds, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    return GPRT_OK;
+}
+
+// ===========================================================================
+// k-NN drivers
+// ===========================================================================
+namespace gprt {
+
+struct KnnPlan {
+    const Tree* tree;
+    const float* d_q;     // xyz, original order, device
+    int64_t nq;
+    int nclouds;          // query sets (== tree clouds)
+    int kmax;
+    int k;                // uniform k (batched: 0)
+    const int* d_kc;      // per cloud k or nullptr
+    const int64_t* d_qoff;
+    const int64_t* d_outbase;
+    int64_t max_q_per_cloud;
+    gprt_hip_config cfg;
+    uint32_t* d_ids;
+    double* d_d2;
+};
+
+static int knn_reserve(Ctx& c, const KnnPlan& p) {
+    size_t nq = (size_t)std::max<int64_t>(p.nq, 1);
+    GPRT_TRY(c.ws[WS_QKEYS].reserve(8 * nq));
+    GPRT_TRY(c.ws[WS_QPERM].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_QS4].reserve(16 * nq));
+    GPRT_TRY(c.ws[WS_LISTA].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_LISTB].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_FB].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_MISC].reserve(sizeof(float) * 101 * std::max(p.nclouds, 1) + 1024));
+    CtxExtra* x = extra(&c);
+    if (p.nclouds > 1) GPRT_TRY(x->qcloud.reserve(4 * nq));
+    // radix sort temporaries for nq keys
+    size_t nblocks = (nq + 4095) / 4096;
+    GPRT_TRY(c.sort_tmp[0].reserve(8 * nq));
+    GPRT_TRY(c.sort_tmp[1].reserve(4 * nq));
+    GPRT_TRY(c.sort_tmp[2].reserve(4 * 256 * nblocks));
+    return GPRT_OK;
+}
+
+static int knn_enqueue(Ctx& c, const KnnPlan& p) {
+    hipStream_t s = c.stream;
+    const Tree& T = *p.tree;
+    const gprt_hip_config& cfg = p.cfg;
+    unsigned* counters = c.counters.as<unsigned>();
+    unsigned long long* stats64 = (unsigned long long*)(counters + CNT_TOTAL);
+    float* kth = c.ws[WS_MISC].as<float>();
+    float* radius = kth + 100 * p.nclouds;
+    int nq = (int)p.nq;
+    CtxExtra* x = extra(&c);
+    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 4, 0u));
+    if (p.nclouds == 1) hipLaunchKernelGGL(set_qoff_kernel, dim3(1), dim3(1), 0, s, (int64_t*)p.d_qoff, p.nq);
+    launch_validate(s, p.d_q, 3 * p.nq, counters + CNT_TOTAL - 1);
+    GPRT_TRY(launch_sampler(s, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(), T.cloud_off.as<int64_t>(),
+                            p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic, cfg.radius_param, kth, radius));
+    uint64_t* keys = c.ws[WS_QKEYS].as<uint64_t>();
+    uint32_t* perm = c.ws[WS_QPERM].as<uint32_t>();
+    float4* qs4 = c.ws[WS_QS4].as<float4>();
+    GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.cloud_bbox.as<float>(), cfg.sort_queries, nq, keys,
+                               perm, qs4, counters, p.max_q_per_cloud));
+    int* qcloud = nullptr;
+    if (p.nclouds > 1) {
+        qcloud = x->qcloud.as<int>();
+        GPRT_TRY(launch_key_cloud(s, keys, nq, qcloud));
+    }
+    int cap = capacity_class(p.kmax, cfg.guard_slots);
+    int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : c.num_cus * 4;
+    int R = cfg.radius_heuristic == GPRT_RADIUS_INFINITE ? 1 : std::min(cfg.max_rounds, MAX_ROUNDS - 1) + 1;
+    int* lists[2] = {c.ws[WS_LISTA].as<int>(), c.ws[WS_LISTB].as<int>()};
+    RoundArgs ra{};
+    ra.nodes = T.nodes.as<Node>();
+    ra.pts = T.pts4.as<float4>();
+    ra.roots = T.roots.as<int2>();
+    ra.qs = qs4;
+    ra.qcloud = qcloud;
+    ra.kc = p.d_kc;
+    ra.k = p.k;
+    ra.radius0 = radius;
+    ra.mult = cfg.radius_mult;
+    ra.fb_list = c.ws[WS_FB].as<int>();
+    ra.counters = counters;
+    ra.stats64 = stats64;
+    ra.ids_out = p.d_ids;
+    ra.d2_out = p.d_d2;
+    ra.out_base = p.d_outbase;
+    ra.qoff = p.d_qoff;
+    ra.max_steps = T.n + 64;
+    for (int r = 0; r < R; ++r) {
+        ra.round = r;
+        ra.last = (r == R - 1);
+        ra.in_list = r == 0 ? nullptr : lists[r & 1];
+        ra.out_list = lists[(r + 1) & 1];
+        GPRT_TRY(launch_round(s, ra, cap, grid, cfg.stack_depth, cfg.collect_counters != 0));
+    }
+    ExactArgs ea{};
+    ea.mode = 1;
+    ea.nq = p.nq;
+    ea.qs = qs4;
+    ea.fb_list = c.ws[WS_FB].as<int>();
+    ea.qcloud = qcloud;
+    ea.kc = p.d_kc;
+    ea.k = p.k;
+    ea.nodes = T.nodes.as<Node>();
+    ea.pts = T.pts4.as<float4>();
+    ea.roots = T.roots.as<int2>();
+    ea.counters = counters;
+    ea.stats64 = stats64;
+    ea.ids_out = p.d_ids;
+    ea.d2_out = p.d_d2;
+    ea.out_base = p.d_outbase;
+    ea.qoff = p.d_qoff;
+    ea.max_steps = T.n + 64;
+    GPRT_TRY(launch_exact(s, ea, p.kmax, nq, cfg.collect_counters != 0));
+    return GPRT_OK;
+}
+
+static uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+static int knn_run(Ctx& c, const KnnPlan& p) {
+    GPRT_TRY(knn_reserve(c, p));
+    hipStream_t s = c.stream;
+    if (p.cfg.use_graph) {
+        // everything the captured kernels point at: a rebuilt tree or a grown
+        // workspace changes a pointer and forces a new capture
+        const Tree& T = *p.tree;
+        const void* ptrs[] = {T.nodes.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.cloud_bbox.ptr, T.cloud_off.ptr,
+                              c.ws[0].ptr, c.ws[1].ptr, c.ws[2].ptr, c.ws[3].ptr, c.ws[4].ptr, c.ws[5].ptr,
+                              c.ws[6].ptr, c.sort_tmp[0].ptr, c.sort_tmp[1].ptr, c.sort_tmp[2].ptr,
+                              c.counters.ptr, extra(&c)->qcloud.ptr};
+        uint64_t key = 1469598103934665603ull;
+        key = fnv(key, &p, sizeof(KnnPlan));
+        key = fnv(key, ptrs, sizeof(ptrs));
+        key = fnv(key, &T.n, sizeof(int));
+        if (!c.graph_exec || c.graph_key != key) {
+            if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+            if (c.graph) (void)hipGraphDestroy(c.graph);
+            c.graph_exec = nullptr;
+            c.graph = nullptr;
+            GPRT_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = knn_enqueue(c, p);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc != GPRT_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(GPRT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            c.graph = g;
+            GPRT_HIP_TRY(hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0));
+            c.graph_key = key;
+        }
+        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        GPRT_HIP_TRY(hipGraphLaunch(c.graph_exec, s));
+        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+    } else {
+        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        GPRT_TRY(knn_enqueue(c, p));
+        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+    }
+    return GPRT_OK;
+}
+
+static int finish_stats(Ctx& c, int64_t nq, bool read_counters) {
+    GPRT_HIP_TRY(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess) {
+        (void)hipGetLastError();
+        ms = 0;
+    }
+    c.stats.ms_query = ms;
+    c.stats.ms_total = ms;
+    c.stats.queries = nq;
+    if (read_counters) {
+        unsigned h[CNT_TOTAL + 4];
+        GPRT_HIP_TRY(hipMemcpy(h, c.counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
+        unsigned long long v64[2];
+        memcpy(v64, h + CNT_TOTAL, sizeof(v64));
+        c.stats.rounds_run = 0;
+        for (int r = 0; r < 16; ++r) {
+            c.stats.active_per_round[r] = h[CNT_ACTIVE + r];
+            if (r < MAX_ROUNDS && h[CNT_ACTIVE + r] > 0) c.stats.rounds_run = r + 1;
+        }
+        c.stats.fallback_queries = h[CNT_FALLBACK];
+        c.stats.partial_queries = h[CNT_PARTIAL];
+        c.stats.node_visits = (int64_t)v64[0];
+        c.stats.point_tests = (int64_t)v64[1];
+        if (h[CNT_TOTAL - 1]) return fail(GPRT_ERR_INPUT, "queries contain NaN/Inf or |coordinate| > 1e18");
+        if (h[CNT_ERROR]) return fail(GPRT_ERR_STATE, "traversal step guard tripped: corrupt tree");
+    }
+    return GPRT_OK;
+}
+
+}  // namespace gprt
+
+
+extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
+                            const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int flags) {
+    if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
+    if (k < 1 || k > 61) return fail(GPRT_ERR_ARG, "k must be in [1, 61]");
+    if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
+    if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
+    if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "tree has several clouds: use gprt_hip_knn_batched");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    gprt_hip_config cf;
+    if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
+    if (cf.guard_slots < 1 || k + cf.guard_slots > 64) return fail(GPRT_ERR_ARG, "k + guard_slots must be <= 64");
+    if (cf.max_rounds < 0) return fail(GPRT_ERR_ARG, "max_rounds < 0");
+    c.stats = gprt_hip_run_stats{};
+    if (nq == 0) return GPRT_OK;
+    CtxExtra* x = extra(&c);
+    bool dev = flags & GPRT_PTR_DEVICE;
+    const float* d_q = queries;
+    uint32_t* d_ids = ids_out;
+    double* d_d2 = d2_out;
+    hipStream_t s = c.stream;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(12ull * nq));
+        GPRT_TRY(x->ids_out.reserve(4ull * nq * k));
+        if (d2_out) GPRT_TRY(x->d2_out.reserve(8ull * nq * k));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, queries, 12ull * nq, hipMemcpyHostToDevice, s));
+        d_q = x->q_in.as<float>();
+        d_ids = x->ids_out.as<uint32_t>();
+        d_d2 = d2_out ? x->d2_out.as<double>() : nullptr;
+    }
+    GPRT_TRY(x->qoff.reserve(16));
+    KnnPlan p{};
+    p.tree = &t->t;
+    p.d_q = d_q;
+    p.nq = nq;
+    p.nclouds = 1;
+    p.kmax = k;
+    p.k = k;
+    p.d_kc = nullptr;
+    p.d_qoff = x->qoff.as<int64_t>();
+    p.d_outbase = nullptr;
+    p.max_q_per_cloud = nq;
+    p.cfg = cf;
+    p.d_ids = d_ids;
+    p.d_d2 = d_d2;
+    GPRT_TRY(knn_run(c, p));
+    if (!dev) {
+        GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
+        if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nq * k, hipMemcpyDeviceToHost, s));
+    }
+    if (flags & GPRT_NO_SYNC) return GPRT_OK;
+    return finish_stats(c, nq, true);
+}
+
+extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
+                                  uint32_t* ids_out, double* d2_out, int flags) {
+    if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
+    if (k < 1 || k > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
+    if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
+    if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
+    if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "plain form takes a single-cloud tree");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    c.stats = gprt_hip_run_stats{};
+    if (nq == 0) return GPRT_OK;
+    CtxExtra* x = extra(&c);
+    bool dev = flags & GPRT_PTR_DEVICE;
+    hipStream_t s = c.stream;
+    const float* d_q = queries;
+    uint32_t* d_ids = ids_out;
+    double* d_d2 = d2_out;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(12ull * nq));
+        GPRT_TRY(x->ids_out.reserve(4ull * nq * k));
+        if (d2_out) GPRT_TRY(x->d2_out.reserve(8ull * nq * k));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, queries, 12ull * nq, hipMemcpyHostToDevice, s));
+        d_q = x->q_in.as<float>();
+        d_ids = x->ids_out.as<uint32_t>();
+        d_d2 = d2_out ? x->d2_out.as<double>() : nullptr;
+    }
+    const Tree& T = t->t;
+    unsigned* counters = c.counters.as<unsigned>();
+    ExactArgs ea{};
+    ea.mode = 0;
+    ea.nq = nq;
+    ea.q_xyz = d_q;
+    ea.k = k;
+    ea.nodes = T.nodes.as<Node>();
+    ea.pts = T.pts4.as<float4>();
+    ea.roots = T.roots.as<int2>();
+    ea.counters = counters;
+    ea.stats64 = (unsigned long long*)(counters + CNT_TOTAL);
+    ea.ids_out = d_ids;
+    ea.d2_out = d_d2;
+    ea.max_steps = T.n + 64;
+    bool count = (flags & GPRT_COLLECT_COUNTERS) != 0;
+    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 4, 0u));
+    launch_validate(s, d_q, 3 * nq, counters + CNT_TOTAL - 1);
+    GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+    GPRT_TRY(launch_exact(s, ea, k, (int)nq, count));
+    GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+    if (!dev) {
+        GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
+        if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nq * k, hipMemcpyDeviceToHost, s));
+    }
+    if (flags & GPRT_NO_SYNC) return GPRT_OK;
+    return finish_stats(c, nq, true);
+}
+
+extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries,
+                                    const int64_t* query_offsets, const int* k_per_cloud,
+                                    const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int flags) {
+    if (!h || !t || !query_offsets || !k_per_cloud) return fail(GPRT_ERR_ARG, "null argument");
+    const Tree& T = t->t;
+    int nc = T.nclouds;
+    if (query_offsets[0] != 0) return fail(GPRT_ERR_ARG, "query_offsets[0] must be 0");
+    int kmax = 1;
+    std::vector<int64_t> outbase(nc + 1, 0);
+    int64_t maxq = 0;
+    for (int c = 0; c < nc; ++c) {
+        if (query_offsets[c + 1] < query_offsets[c]) return fail(GPRT_ERR_ARG, "query_offsets must be non-decreasing");
+        if (k_per_cloud[c] < 1 || k_per_cloud[c] > 61) return fail(GPRT_ERR_ARG, "k must be in [1, 61]");
+        kmax = std::max(kmax, k_per_cloud[c]);
+        outbase[c + 1] = outbase[c] + (query_offsets[c + 1] - query_offsets[c]) * k_per_cloud[c];
+        maxq = std::max(maxq, query_offsets[c + 1] - query_offsets[c]);
+    }
+    int64_t nq = query_offsets[nc], nout = outbase[nc];
+    if (nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "too many queries");
+    if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    gprt_hip_config cf;
+    if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
+    if (cf.guard_slots < 1 || kmax + cf.guard_slots > 64) return fail(GPRT_ERR_ARG, "k + guard_slots must be <= 64");
+    c.stats = gprt_hip_run_stats{};
+    if (nq == 0) return GPRT_OK;
+    CtxExtra* x = extra(&c);
+    hipStream_t s = c.stream;
+    bool dev = flags & GPRT_PTR_DEVICE;
+    const float* d_q = queries;
+    uint32_t* d_ids = ids_out;
+    double* d_d2 = d2_out;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(12ull * nq));
+        GPRT_TRY(x->ids_out.reserve(4ull * nout));
+        if (d2_out) GPRT_TRY(x->d2_out.reserve(8ull * nout));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, queries, 12ull * nq, hipMemcpyHostToDevice, s));
+        d_q = x->q_in.as<float>();
+        d_ids = x->ids_out.as<uint32_t>();
+        d_d2 = d2_out ? x->d2_out.as<double>() : nullptr;
+    }
+    GPRT_TRY(x->qoff.reserve(8ull * (nc + 1)));
+    GPRT_TRY(x->outbase.reserve(8ull * (nc + 1)));
+    GPRT_TRY(x->kc.reserve(4ull * nc));
+    GPRT_HIP_TRY(hipMemcpyAsync(x->qoff.ptr, query_offsets, 8ull * (nc + 1), hipMemcpyHostToDevice, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(x->outbase.ptr, outbase.data(), 8ull * (nc + 1), hipMemcpyHostToDevice, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(x->kc.ptr, k_per_cloud, 4ull * nc, hipMemcpyHostToDevice, s));
+    KnnPlan p{};
+    p.tree = &T;
+    p.d_q = d_q;
+    p.nq = nq;
+    p.nclouds = nc;
+    p.kmax = kmax;
+    p.k = 0;
+    p.d_kc = x->kc.as<int>();
+    p.d_qoff = x->qoff.as<int64_t>();
+    p.d_outbase = x->outbase.as<int64_t>();
+    p.max_q_per_cloud = maxq;
+    p.cfg = cf;
+    p.d_ids = d_ids;
+    p.d_d2 = d_d2;
+    GPRT_TRY(knn_run(c, p));
+    if (!dev) {
+        GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nout, hipMemcpyDeviceToHost, s));
+        if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nout, hipMemcpyDeviceToHost, s));
+    }
+    if (flags & GPRT_NO_SYNC) return GPRT_OK;
+    return finish_stats(c, nq, true);
+}
+
+extern "C" int gprt_hip_stats(gprt_hip_ctx* h, gprt_hip_run_stats* out) {
+    if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
+    *out = h->c.stats;
+    return GPRT_OK;
+}
+
+// ===========================================================================
+// pieces exposed for parity tests
+// ===========================================================================
+extern "C" int gprt_hip_morton_codes(gprt_hip_ctx* h, const float* pts, int64_t n, uint32_t* codes_out,
+                                     float* bbox_out, int flags) {
+    if (!h || (n > 0 && (!pts || !codes_out))) return fail(GPRT_ERR_ARG, "null argument");
+    if (n < 0 || n > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad n");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    CtxExtra* x = extra(&c);
+    hipStream_t s = c.stream;
+    bool dev = flags & GPRT_PTR_DEVICE;
+    const float* d_p = pts;
+    uint32_t* d_codes = codes_out;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(12ull * std::max<int64_t>(n, 1)));
+        GPRT_TRY(x->ids_out.reserve(4ull * std::max<int64_t>(n, 1)));
+        if (n) GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, pts, 12ull * n, hipMemcpyHostToDevice, s));
+        d_p = x->q_in.as<float>();
+        d_codes = x->ids_out.as<uint32_t>();
+    }
+    GPRT_TRY(c.ws[WS_MISC].reserve(64));
+    float* d_bbox = c.ws[WS_MISC].as<float>();
+    GPRT_TRY(morton_codes_dev(&c, d_p, (int)n, d_codes, d_bbox));
+    if (!dev && n) GPRT_HIP_TRY(hipMemcpyAsync(codes_out, d_codes, 4ull * n, hipMemcpyDeviceToHost, s));
+    if (bbox_out) GPRT_HIP_TRY(hipMemcpyAsync(bbox_out, d_bbox, 24, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    return GPRT_OK;
+}
+
+extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries,
+                                      int64_t nq, int sampler, int heuristic, float param, int k,
+                                      float* radius_out, int flags) {
+    if (!h || !radius_out) return fail(GPRT_ERR_ARG, "null argument");
+    if (n < 0 || nq < 0 || (n > 0 && !pts) || (nq > 0 && !queries)) return fail(GPRT_ERR_ARG, "bad arrays");
+    if (sampler != GPRT_SAMPLER_LCG && sampler != GPRT_SAMPLER_STRIDED) return fail(GPRT_ERR_ARG, "bad sampler");
+    if (heuristic < 0 || heuristic > 3) return fail(GPRT_ERR_ARG, "bad heuristic");
+    if (sampler == GPRT_SAMPLER_STRIDED && k < 1) return fail(GPRT_ERR_ARG, "k must be positive");
+    // the sampler reads points through a tree (sorted points + inverse permutation)
+    gprt_hip_tree* t = nullptr;
+    GPRT_TRY(gprt_hip_tree_build(h, pts, n, 8, flags, &t));
+    Ctx& c = h->c;
+    hipStream_t s = c.stream;
+    CtxExtra* x = extra(&c);
+    int rc = GPRT_OK;
+    const float* d_q = queries;
+    DevBuf qtmp;
+    if (!(flags & GPRT_PTR_DEVICE) && nq > 0) {
+        rc = qtmp.reserve(12ull * nq);
+        if (rc == GPRT_OK && hipMemcpyAsync(qtmp.ptr, queries, 12ull * nq, hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = fail(GPRT_ERR_HIP, "copy failed");
+        d_q = qtmp.as<float>();
+    }
+    if (rc == GPRT_OK) rc = x->qoff.reserve(16);
+    if (rc == GPRT_OK) rc = c.ws[WS_MISC].reserve(sizeof(float) * 101 + 64);
+    if (rc == GPRT_OK) {
+        int64_t qo[2] = {0, nq};
+        float* kth = c.ws[WS_MISC].as<float>();
+        float* rad = kth + 100;
+        (void)hipMemcpyAsync(x->qoff.ptr, qo, 16, hipMemcpyHostToDevice, s);
+        if (n > 0 && nq > 0)
+            rc = launch_sampler(s, sampler, t->t.pts4.as<float4>(), t->t.inv.as<uint32_t>(),
+                                t->t.cloud_off.as<int64_t>(), d_q, x->qoff.as<int64_t>(), 1, k, heuristic, param,
+                                kth, rad);
+        else
+            rc = launch_sampler(s, sampler, nullptr, nullptr, nullptr, nullptr, nullptr, 1, k, GPRT_RADIUS_INFINITE,
+                                param, kth, rad);
+        float r = 0;
+        if (rc == GPRT_OK && hipMemcpyAsync(&r, rad, 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess) {
+            if (!(n > 0 && nq > 0)) {
+                // no samples: the reference keeps its default radius
+                r = sampler == GPRT_SAMPLER_LCG ? 10.0f
+                                                : (heuristic == GPRT_RADIUS_FIXED ? param : 1.0f);
+            }
+            *radius_out = r;
+        } else if (rc == GPRT_OK) {
+            rc = fail(GPRT_ERR_HIP, "sampler readback failed");
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    qtmp.release();
+    gprt_hip_tree_destroy(t);
+    return rc;
+}
+
+extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
+                                   int end_bit, int flags) {
+    if (!h || (n > 0 && (!keys || !vals))) return fail(GPRT_ERR_ARG, "null argument");
+    if (n < 0 || n > 0x7FFFFFF0 || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
+        return fail(GPRT_ERR_ARG, "bad size/bit range");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    hipStream_t s = c.stream;
+    bool dev = flags & GPRT_PTR_DEVICE;
+    DevBuf kb, vb;
+    uint64_t* dk = keys;
+    uint32_t* dv = vals;
+    if (!dev && n > 0) {
+        GPRT_TRY(kb.reserve(8ull * n));
+        GPRT_TRY(vb.reserve(4ull * n));
+        GPRT_HIP_TRY(hipMemcpyAsync(kb.ptr, keys, 8ull * n, hipMemcpyHostToDevice, s));
+        GPRT_HIP_TRY(hipMemcpyAsync(vb.ptr, vals, 4ull * n, hipMemcpyHostToDevice, s));
+        dk = kb.as<uint64_t>();
+        dv = vb.as<uint32_t>();
+    }
+    int rc = radix_sort_pairs_u64(&c, dk, dv, (int)n, begin_bit, end_bit);
+    if (rc == GPRT_OK && !dev && n > 0) {
+        if (hipMemcpyAsync(keys, dk, 8ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(vals, dv, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess)
+            rc = fail(GPRT_ERR_HIP, "copy back failed");
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && rc == GPRT_OK) rc = fail(GPRT_ERR_HIP, "sync failed");
+    kb.release();
+    vb.release();
+    return rc;
+}

@@ -1,0 +1,170 @@
+// gprt_internal.hpp -- types and device helpers shared by the HIP translation
+// units of libgprt_hip.so.  Built for gfx950 only (64-lane wavefronts).
+//
+// Numeric contract (every TU is compiled with -ffp-contract=off, so a product
+// and a sum are rounded separately unless an fma is written explicitly):
+//   d64(q,p) = ((dx*dx + dy*dy) + dz*dz) in binary64, dx = (double)q.x-(double)p.x
+//   d32(q,p) = fma(dz,dz, fma(dy,dy, dx*dx)) in binary32, dx = q.x - p.x
+//   box32(q,B) / box64(q,B): the same formula applied to q - clamp(q, lo, hi).
+// Rounding is monotone, so box32(q,B) <= d32(q,p) and box64(q,B) <= d64(q,p)
+// for every point p inside B: pruning a box whose value exceeds a threshold
+// never drops a point whose own value is within it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+#include "../../include/gprt_hip.h"
+
+namespace gprt {
+
+// ---------------------------------------------------------------------------
+// error plumbing: every entry point returns an int status; the message of the
+// last failure on the calling thread is kept for gprt_hip_last_error().
+// ---------------------------------------------------------------------------
+void set_error(int code, const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define GPRT_HIP_TRY(expr)                                                         \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            return ::gprt::fail(GPRT_ERR_HIP, std::string(#expr) + ": " +          \
+                                                  hipGetErrorString(_e));          \
+    } while (0)
+
+#define GPRT_TRY(expr)            \
+    do {                          \
+        int _rc = (expr);         \
+        if (_rc != GPRT_OK) return _rc; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffer: grow-only, owned by a context or a tree
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t need);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ---------------------------------------------------------------------------
+// LBVH layout in HBM
+// ---------------------------------------------------------------------------
+// One internal node = the boxes of its two children plus their references,
+// 64 bytes, read with four 16-byte loads.
+//   lo[c].w = child reference (int bits): >= 0 internal node index,
+//             < 0 leaf bucket whose first sorted point is ~ref
+//   hi[c].w = number of points under the child (int bits)
+struct __align__(16) Node {
+    float4 alo, ahi, blo, bhi;
+};
+
+// Points in Morton order: (x, y, z, id) with id = index of the point in the
+// caller's cloud (int bits in .w).
+struct Tree {
+    int n = 0;          // total points (all clouds)
+    int nclouds = 0;
+    int leaf_size = 8;  // subtrees with <= leaf_size points are scanned linearly
+    DevBuf pts4;        // float4[n]
+    DevBuf nodes;       // Node[max(n-1,1)]
+    DevBuf roots;       // int2[nclouds]: (ref, count) of each cloud's root
+    DevBuf cloud_off;   // int64[nclouds+1] point offsets per cloud (device)
+    DevBuf cloud_bbox;  // float[6*nclouds]: per-cloud (min xyz, max xyz)
+    DevBuf perm;        // uint32[n]: sorted position -> input index
+    DevBuf inv;         // uint32[n]: input index -> sorted position
+    DevBuf codes;       // uint64[n]: sorted keys (cloud << 32 | morton)
+    DevBuf parent;      // int[n-1 + n]: parent of internal nodes, then of points
+    DevBuf flags;       // int[n-1]: refit arrival counters
+    DevBuf bh;          // Barnes-Hut per-node data (allocated on demand)
+    std::vector<int64_t> h_cloud_off;
+    float h_bbox[6] = {0, 0, 0, 0, 0, 0};
+    int key_bits = 30;  // significant bits of the sort key
+    int max_depth_bound = 64;
+};
+
+struct Ctx;
+
+// ---------------------------------------------------------------------------
+// numeric helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float d2f(float qx, float qy, float qz, float px, float py, float pz) {
+    float dx = qx - px, dy = qy - py, dz = qz - pz;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ double d2d(float qx, float qy, float qz, float px, float py, float pz) {
+    double dx = (double)qx - (double)px;
+    double dy = (double)qy - (double)py;
+    double dz = (double)qz - (double)pz;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+__device__ __forceinline__ float box2f(float qx, float qy, float qz, const float4& lo, const float4& hi) {
+    return d2f(qx, qy, qz, clampf(qx, lo.x, hi.x), clampf(qy, lo.y, hi.y), clampf(qz, lo.z, hi.z));
+}
+
+__device__ __forceinline__ double box2d(float qx, float qy, float qz, const float4& lo, const float4& hi) {
+    return d2d(qx, qy, qz, clampf(qx, lo.x, hi.x), clampf(qy, lo.y, hi.y), clampf(qz, lo.z, hi.z));
+}
+
+// Guard band for the FP32 pass.  |d32 - D| <= 6u D and |d64 - D| <= 6*2^-53 D
+// for the exact D, so every point whose (d64, id) ranks within the true top k
+// has d32 <= T * (1+6u)(1+e64)/((1-6u)(1-e64)) < T * (1 + 16u), T being the
+// k-th smallest d32.  BAND = 1 + 2^-19 = 1 + 32u leaves a factor-2 margin; the
+// additive term covers flushed subnormal squares (each < FLT_MIN).
+#define GPRT_BAND 1.0000019073486328125f
+#define GPRT_BAND_ABS 1e-36f
+__device__ __forceinline__ float band(float t) { return __builtin_fmaf(t, GPRT_BAND, GPRT_BAND_ABS); }
+
+// wave helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    int l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Wave-aggregated append to a global counter: every active lane with `pred`
+// gets a unique slot; returns the slot (or -1 when pred is false).  Must be
+// called by all lanes of the wave that are live at this point.
+__device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
+    uint64_t m = __ballot(pred);
+    if (m == 0) return -1;
+    int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    return pred ? (int)(base + __popcll(m & lanemask_lt())) : -1;
+}
+
+// ---------------------------------------------------------------------------
+// internal entry points between translation units
+// ---------------------------------------------------------------------------
+int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int nclouds, int leaf_size, Tree* t);
+int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
+int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
+int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
+
+// the runtime context
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    DevBuf ws[8];          // grow-only scratch slots used by the drivers
+    DevBuf sort_tmp[4];    // radix sort ping-pong + histograms
+    DevBuf counters;       // small device counters
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    gprt_hip_run_stats stats{};
+    // cached graph of the batched k-NN pipeline
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t graph_key = 0;
+};
+
+}  // namespace gprt

@@ -1,0 +1,744 @@
+// knn.hip -- exact k-NN traversal kernels over the LBVH.
+//
+//   * radius samplers (LCG of gprt-optix/src/dispatch.rs:65-103, strided of
+//     gprt-macros/src/lib.rs:417-450) evaluated on the device;
+//   * query ordering: Morton keys + radix sort + gather into float4;
+//   * knn_round_kernel: one TrueKNN round (gprt-macros/src/lib.rs:466-594) as a
+//     persistent launch.  FP32 traversal with a register candidate list of
+//     capacity C >= k + guard slots, FP64 re-rank of the guard band, on-device
+//     compaction of unresolved queries (replaces the host filter of
+//     lib.rs:517-583 and the host radius loop of :592-594);
+//   * knn_exact_kernel: FP64 traversal with an exact (d64, id) list, used as the
+//     plain single-launch form and as the fallback for queries whose guard band
+//     overflowed in the FP32 pass.
+#include <algorithm>
+#include <cfloat>
+
+#include "gprt_internal.hpp"
+#include "knn_internal.hpp"
+
+namespace gprt {
+
+// ---------------------------------------------------------------------------
+// LCG with O(log n) skip-ahead (Brown, "Random number generation with
+// arbitrary strides", 1994).  Constants of dispatch.rs:66-67 (Knuth's MMIX).
+// ---------------------------------------------------------------------------
+#define LCG_A 6364136223846793005ull
+#define LCG_C 1442695040888963407ull
+#define LCG_SEED 0x123456789ABCDEFull
+
+__device__ __forceinline__ uint64_t lcg_jump(uint64_t s, uint64_t m) {
+    uint64_t am = 1, ap = 0, cm = LCG_A, cp = LCG_C;
+    while (m) {
+        if (m & 1) {
+            am *= cm;
+            ap = ap * cm + cp;
+        }
+        cp = (cm + 1) * cp;
+        cm *= cm;
+        m >>= 1;
+    }
+    return am * s + ap;
+}
+
+// the sampler's own FP32 distance: (dx*dx + dy*dy) + dz*dz, no fma, as the
+// reference's `(q.x-p.x).powi(2) + ...` (dispatch.rs:80, lib.rs:434)
+__device__ __forceinline__ float d2_sampler(const float* q, const float* p) {
+    float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// smallest 4 of two ascending 4-lists (bitonic merge)
+__device__ __forceinline__ void merge4(float* a, const float* b) {
+    float m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m[i] = fminf(a[i], b[3 - i]);
+    float t;
+#define CE(i, j) if (m[i] > m[j]) { t = m[i]; m[i] = m[j]; m[j] = t; }
+    CE(0, 2) CE(1, 3) CE(0, 1) CE(2, 3)
+#undef CE
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = m[i];
+}
+
+__device__ __forceinline__ void ins4(float* d, float v) {
+    if (v < d[3]) {
+        d[3] = v;
+        float t;
+        if (d[3] < d[2]) { t = d[2]; d[2] = d[3]; d[3] = t; }
+        if (d[2] < d[1]) { t = d[1]; d[1] = d[2]; d[2] = t; }
+        if (d[1] < d[0]) { t = d[0]; d[0] = d[1]; d[1] = t; }
+    }
+}
+
+// grid (100, nclouds), 256 threads.  kth[c*100 + s] = sqrt of the 4th smallest
+// sampled d^2 of sample slot s, or -1 when the reference would not push it.
+__global__ __launch_bounds__(256) void sampler_lcg_kernel(const float4* __restrict__ pts4,
+                                                          const uint32_t* __restrict__ inv,
+                                                          const int64_t* __restrict__ poff,
+                                                          const float* __restrict__ qs,
+                                                          const int64_t* __restrict__ qoff, float* kth) {
+    const int c = blockIdx.y, s = blockIdx.x;
+    const int64_t n = poff[c + 1] - poff[c], nq = qoff[c + 1] - qoff[c];
+    const int64_t S = n < 5000 ? n : 5000, qss = nq < 100 ? nq : 100;
+    __shared__ float part[4][4];
+    if (s >= qss || n <= 0) {
+        if (threadIdx.x == 0) kth[c * 100 + s] = -1.0f;
+        return;
+    }
+    // step index (1-based) of this slot's query draw: 1 + s*(S+1)
+    const uint64_t qstep = 1ull + (uint64_t)s * (uint64_t)(S + 1);
+    const uint64_t qstate = lcg_jump(LCG_SEED, qstep);
+    const int64_t qi = (int64_t)((qstate >> 33) % (uint64_t)nq);
+    float q[3] = {qs[3 * (qoff[c] + qi)], qs[3 * (qoff[c] + qi) + 1], qs[3 * (qoff[c] + qi) + 2]};
+    const int64_t chunk = (S + 255) / 256;
+    int64_t t0 = threadIdx.x * chunk, t1 = t0 + chunk < S ? t0 + chunk : S;
+    float d[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+    if (t0 < t1) {
+        uint64_t st = lcg_jump(qstate, (uint64_t)t0);
+        for (int64_t t = t0; t < t1; ++t) {
+            st = st * LCG_A + LCG_C;
+            int64_t pi = (int64_t)((st >> 33) % (uint64_t)n);
+            const float4 p4 = pts4[inv[poff[c] + pi]];
+            const float p[3] = {p4.x, p4.y, p4.z};
+            float dd = d2_sampler(q, p);
+            if (dd < INFINITY) ins4(d, dd);
+        }
+    }
+    // wave merge, then across the 4 waves
+    for (int o = 32; o > 0; o >>= 1) {
+        float b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = __shfl_xor(d[i], o);
+        merge4(d, b);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int i = 0; i < 4; ++i) part[w][i] = d[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int ww = 1; ww < 4; ++ww) merge4(d, part[ww]);
+        // the reference pushes dists[3] when it is not f32::MAX; its entries
+        // are sqrt'd, and sqrt is monotone, so the 4th sqrt is sqrt(4th d^2)
+        kth[c * 100 + s] = (d[3] < FLT_MAX) ? sqrtf(d[3]) : -1.0f;
+    }
+}
+
+// grid (100, nclouds), 256 threads.  Strided sampler slot s: query s*qstep
+// against every pstep-th point (at most 1000), k smallest d^2 with
+// 1e-10 < d^2 (< f32::MAX); kth = sqrt(k-th) or -1.
+__global__ __launch_bounds__(256) void sampler_strided_kernel(const float4* __restrict__ pts4,
+                                                              const uint32_t* __restrict__ inv,
+                                                              const int64_t* __restrict__ poff,
+                                                              const float* __restrict__ qs,
+                                                              const int64_t* __restrict__ qoff, int k,
+                                                              float* kth) {
+    const int c = blockIdx.y, s = blockIdx.x;
+    const int64_t n = poff[c + 1] - poff[c], nq = qoff[c + 1] - qoff[c];
+    const int64_t qstep = nq / 100 > 1 ? nq / 100 : 1, pstep = n / 1000 > 1 ? n / 1000 : 1;
+    const int64_t qi = (int64_t)s * qstep;
+    __shared__ float v[1024];
+    if (qi >= nq) {
+        if (threadIdx.x == 0) kth[c * 100 + s] = -1.0f;
+        return;
+    }
+    const float* qp = qs + 3 * (qoff[c] + qi);
+    float q[3] = {qp[0], qp[1], qp[2]};
+    for (int j = threadIdx.x; j < 1024; j += 256) {
+        int64_t pi = (int64_t)j * pstep;
+        float dd = INFINITY;
+        if (j < 1000 && pi < n) {
+            const float4 p4 = pts4[inv[poff[c] + pi]];
+            const float p[3] = {p4.x, p4.y, p4.z};
+            dd = d2_sampler(q, p);
+            if (!(dd > 1e-10f && dd < FLT_MAX)) dd = INFINITY;
+        }
+        v[j] = dd;
+    }
+    __syncthreads();
+    // bitonic sort of 1024 values
+    for (int size = 2; size <= 1024; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int j = threadIdx.x; j < 1024; j += 256) {
+                int partner = j ^ stride;
+                if (partner > j) {
+                    bool up = (j & size) == 0;
+                    float a = v[j], b = v[partner];
+                    if ((a > b) == up) { v[j] = b; v[partner] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    if (threadIdx.x == 0) kth[c * 100 + s] = (k <= 1024 && v[k - 1] < INFINITY) ? sqrtf(v[k - 1]) : -1.0f;
+}
+
+// one thread per cloud: radius from the pushed samples (dispatch.rs:92-103 for
+// LCG, lib.rs:442-447 for strided).
+__global__ void sampler_finalize_kernel(const float* kth, int nclouds, int sampler, int heuristic, float param,
+                                        float* radius) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclouds) return;
+    float v[100];
+    int len = 0;
+    for (int s = 0; s < 100; ++s) {
+        float x = kth[c * 100 + s];
+        if (x >= 0.0f) {
+            int j = len++;
+            while (j > 0 && v[j - 1] > x) { v[j] = v[j - 1]; --j; }
+            v[j] = x;
+        }
+    }
+    float r;
+    if (sampler == GPRT_SAMPLER_LCG) {
+        r = 10.0f;
+        if (len > 0) {
+            if (heuristic == GPRT_RADIUS_MAX) r = v[len - 1] * 1.1f;
+            else if (heuristic == GPRT_RADIUS_PERCENTILE) {
+                float a = fminf((float)len * param, (float)(len - 1));
+                int idx = a > 0.0f ? (int)a : 0;
+                r = v[idx] * 1.1f;
+            } else r = param;
+        }
+    } else {
+        r = 1.0f;
+        if (heuristic == GPRT_RADIUS_FIXED) r = param;
+        else if (heuristic == GPRT_RADIUS_PERCENTILE && len > 0) {
+            float a = (float)len * param;
+            int idx = a > 0.0f ? (int)a : 0;
+            if (idx > len - 1) idx = len - 1;
+            r = v[idx] * 1.1f;
+        }
+    }
+    if (heuristic == GPRT_RADIUS_INFINITE) r = INFINITY;
+    radius[c] = r;
+}
+
+// ---------------------------------------------------------------------------
+// query ordering
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t expand10(uint32_t x) {
+    x &= 0x000003ffu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__device__ __forceinline__ uint32_t quant10(float p, float mn, float ext) {
+    float t = (p - mn) / ext * 1023.0f;
+    uint32_t u = !(t > 0.0f) ? 0u : (t >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)t);
+    return u < 1023u ? u : 1023u;
+}
+
+// grid (chunks, clouds): key = cloud << 32 | morton of the query in its cloud's
+// box (queries outside the box saturate to the boundary cells).
+__global__ void query_keys_kernel(const float* __restrict__ qs, const int64_t* __restrict__ qoff,
+                                  const float* __restrict__ bbox, int use_morton, uint64_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+    int c = blockIdx.y;
+    int64_t b = qoff[c], e = qoff[c + 1];
+    float bb[6];
+    for (int a = 0; a < 6; ++a) bb[a] = bbox[6 * c + a];
+    float ex = fmaxf(bb[3] - bb[0], 1e-6f), ey = fmaxf(bb[4] - bb[1], 1e-6f), ez = fmaxf(bb[5] - bb[2], 1e-6f);
+    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = 0;
+        if (use_morton)
+            m = (expand10(quant10(qs[3 * i], bb[0], ex)) << 2) | (expand10(quant10(qs[3 * i + 1], bb[1], ey)) << 1) |
+                expand10(quant10(qs[3 * i + 2], bb[2], ez));
+        keys[i] = ((uint64_t)c << 32) | m;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void query_gather_kernel(const float* __restrict__ qs, const uint32_t* __restrict__ perm, int nq,
+                                    float4* __restrict__ out, unsigned* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        uint32_t g = perm[i];
+        out[i] = make_float4(qs[3 * g], qs[3 * g + 1], qs[3 * g + 2], __int_as_float((int)g));
+    }
+    if (i == 0) counters[CNT_ACTIVE] = (unsigned)nq;
+}
+
+__global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 candidate list (ascending d32), capacity C, in registers
+// ---------------------------------------------------------------------------
+template <int C>
+struct CandList {
+    float d[C];
+    int s[C];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int j = 0; j < C; ++j) { d[j] = INFINITY; s[j] = -1; }
+    }
+    // insert (dd, ss); returns the value pushed out of the last slot (INF if
+    // that slot was empty).  Caller guarantees dd < d[C-1].
+    __device__ __forceinline__ float insert(float dd, int ss) {
+        float ev = d[C - 1];
+#pragma unroll
+        for (int j = C - 1; j >= 0; --j) {
+            float prev = j > 0 ? d[j - 1] : -INFINITY;
+            if (prev > dd) {
+                d[j] = prev;
+                s[j] = s[j > 0 ? j - 1 : 0];
+            } else if (d[j] > dd) {
+                d[j] = dd;
+                s[j] = ss;
+            }
+        }
+        return ev;
+    }
+    __device__ __forceinline__ float at(int k1) const {  // d[k1] with runtime k1
+        float v = d[0];
+#pragma unroll
+        for (int j = 1; j < C; ++j) v = (j == k1) ? d[j] : v;
+        return v;
+    }
+};
+
+// Per-lane traversal state of the FP32 pass
+template <int C>
+struct F32Query {
+    float qx, qy, qz;
+    float T0;      // round bound (radius^2), INF in the unbounded round
+    float T, tau;  // current k-th bound and guard threshold
+    float ev_min;  // smallest d32 pushed out of a full list
+    int cnt, k;
+    CandList<C> L;
+
+    __device__ __forceinline__ void visit_point(float4 p, int s) {
+        float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
+        if (dd <= tau) {
+            if (dd < L.d[C - 1]) {
+                float ev = L.insert(dd, s);
+                ev_min = fminf(ev_min, ev);
+                cnt = cnt < C ? cnt + 1 : C;
+                if (cnt >= k) {
+                    T = fminf(T0, L.at(k - 1));
+                    tau = band(T);
+                }
+            } else {
+                ev_min = fminf(ev_min, dd);
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// one bounded (or the final unbounded) round, persistent over the active list
+// ---------------------------------------------------------------------------
+template <int C, int BLOCK, int STACK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
+    __shared__ int2 stk[STACK][BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
+    const Node* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ pts = a.pts;
+    uint64_t visits = 0, tests = 0;
+    while (true) {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(&a.counters[CNT_WORK + a.round], 64u);
+        base = __shfl(base, 0);
+        if (base >= n_in) break;
+        const unsigned i = base + lane;
+        const bool valid = i < n_in;
+        int status = 0;  // 0 done, 1 unresolved, 2 fallback
+        int s = 0;
+        if (valid) {
+            s = a.in_list ? a.in_list[i] : (int)i;
+            const float4 q = a.qs[s];
+            const int c = a.qcloud ? a.qcloud[s] : 0;
+            const int k = a.kc ? a.kc[c] : a.k;
+            F32Query<C> Q;
+            Q.qx = q.x; Q.qy = q.y; Q.qz = q.z;
+            float R = a.last ? INFINITY : a.radius0[c];
+            if (!a.last)
+                for (int r = 0; r < a.round; ++r) R *= a.mult;
+            Q.T0 = a.last ? INFINITY : R * R;
+            Q.T = Q.T0;
+            Q.tau = band(Q.T);
+            Q.ev_min = INFINITY;
+            Q.cnt = 0;
+            Q.k = k;
+            Q.L.init();
+            bool overflow = false;
+            const int2 root = a.roots[c];
+            int ref = root.x;
+            int sp = 0;
+            if (root.y > 0) {
+                if (ref < 0) {
+                    for (int t = 0; t < root.y; ++t) Q.visit_point(pts[~ref + t], ~ref + t);
+                    if (COUNT) tests += root.y;
+                } else {
+                    // a tree path never repeats a node: more steps than nodes
+                    // means a corrupt tree; bail out to the exact kernel
+                    int steps = 0;
+                    while (true) {
+                        if (++steps > a.max_steps) { overflow = true; break; }
+                        const Node nd = nodes[ref];
+                        if (COUNT) ++visits;
+                        const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
+                        const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
+                        const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                        // leaf buckets are scanned at once, nearer first
+                        const bool lA = rA < 0, lB = rB < 0;
+                        if (lA && lB && dB < dA) {
+                            if (dB <= Q.tau) {
+                                int f = ~rB, m = __float_as_int(nd.bhi.w);
+                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
+                                if (COUNT) tests += m;
+                            }
+                            if (dA <= Q.tau) {
+                                int f = ~rA, m = __float_as_int(nd.ahi.w);
+                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
+                                if (COUNT) tests += m;
+                            }
+                        } else {
+                            if (lA && dA <= Q.tau) {
+                                int f = ~rA, m = __float_as_int(nd.ahi.w);
+                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
+                                if (COUNT) tests += m;
+                            }
+                            if (lB && dB <= Q.tau) {
+                                int f = ~rB, m = __float_as_int(nd.bhi.w);
+                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
+                                if (COUNT) tests += m;
+                            }
+                        }
+                        const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
+                        if (gA && gB) {
+                            const bool nearA = dA <= dB;
+                            if (sp < STACK) {
+                                stk[sp][tid] = make_int2(nearA ? rB : rA, __float_as_int(nearA ? dB : dA));
+                                ++sp;
+                            } else {
+                                overflow = true;
+                            }
+                            ref = nearA ? rA : rB;
+                            continue;
+                        }
+                        if (gA) { ref = rA; continue; }
+                        if (gB) { ref = rB; continue; }
+                        bool found = false;
+                        while (sp > 0) {
+                            --sp;
+                            const int2 e = stk[sp][tid];
+                            if (__int_as_float(e.y) <= Q.tau) { ref = e.x; found = true; break; }
+                        }
+                        if (!found) break;
+                    }
+                }
+            }
+            // resolve
+            const float kth = Q.cnt >= k ? Q.L.at(k - 1) : INFINITY;
+            const bool resolved = a.last ? true : (Q.cnt >= k && kth <= Q.T0);
+            if (!resolved) {
+                status = 1;
+            } else {
+                const float tf = band(Q.cnt >= k ? kth : INFINITY);
+                if (overflow || Q.ev_min <= tf) {
+                    status = 2;
+                } else {
+                    // FP64 re-rank of the guard band
+                    double dd[C];
+                    uint32_t id[C];
+#pragma unroll
+                    for (int j = 0; j < C; ++j) {
+                        const bool in = j < Q.cnt && Q.L.d[j] <= tf;
+                        dd[j] = INFINITY;
+                        id[j] = GPRT_ID_NONE;
+                        if (in) {
+                            const float4 p = pts[Q.L.s[j]];
+                            dd[j] = d2d(Q.qx, Q.qy, Q.qz, p.x, p.y, p.z);
+                            id[j] = (uint32_t)__float_as_int(p.w);
+                        }
+                    }
+                    bool swapped = true;
+                    while (swapped) {
+                        swapped = false;
+#pragma unroll
+                        for (int j = 0; j + 1 < C; ++j) {
+                            const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
+                            if (gt) {
+                                double t = dd[j]; dd[j] = dd[j + 1]; dd[j + 1] = t;
+                                uint32_t u = id[j]; id[j] = id[j + 1]; id[j + 1] = u;
+                                swapped = true;
+                            }
+                        }
+                    }
+                    const int g = __float_as_int(q.w);
+                    const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
+#pragma unroll
+                    for (int j = 0; j < C; ++j)
+                        if (j < k) {
+                            a.ids_out[ob + j] = id[j];
+                            if (a.d2_out) a.d2_out[ob + j] = dd[j];
+                        }
+                    if (Q.cnt < k) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
+                }
+            }
+        }
+        const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
+        if (slot >= 0) a.out_list[slot] = s;
+        const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
+        if (fslot >= 0) a.fb_list[fslot] = s;
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            visits += __shfl_xor(visits, o);
+            tests += __shfl_xor(tests, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats64[0], (unsigned long long)visits);
+            atomicAdd(&a.stats64[1], (unsigned long long)tests);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exact FP64 traversal (plain single-launch form and band-overflow fallback)
+// ---------------------------------------------------------------------------
+template <int KMAX>
+struct ExactList {
+    double d[KMAX];
+    uint32_t id[KMAX];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) { d[j] = INFINITY; id[j] = GPRT_ID_NONE; }
+    }
+    // insert into the first k slots, ascending (d, id)
+    __device__ __forceinline__ void insert(double dd, uint32_t ii, int k) {
+#pragma unroll
+        for (int j = KMAX - 1; j >= 0; --j) {
+            if (j >= k) continue;
+            const double pd = j > 0 ? d[j - 1] : -INFINITY;
+            const uint32_t pi = j > 0 ? id[j - 1] : 0u;
+            const bool prev_gt = pd > dd || (pd == dd && pi > ii);
+            const bool cur_gt = d[j] > dd || (d[j] == dd && id[j] > ii);
+            if (prev_gt) { d[j] = pd; id[j] = pi; }
+            else if (cur_gt) { d[j] = dd; id[j] = ii; }
+        }
+    }
+    __device__ __forceinline__ void worst(int k, double& wd, uint32_t& wi) const {
+        wd = d[0]; wi = id[0];
+#pragma unroll
+        for (int j = 1; j < KMAX; ++j)
+            if (j == k - 1) { wd = d[j]; wi = id[j]; }
+    }
+};
+
+__device__ __forceinline__ float f64_lower_f32(double v) {
+    // v >= 0 (a squared distance): step one ulp down when rounding went up
+    float f = (float)v;
+    if ((double)f > v) f = __int_as_float(__float_as_int(f) - 1);
+    return f;
+}
+
+template <int KMAX>
+__device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, const float4* __restrict__ pts,
+                                            int2 root, float qx, float qy, float qz, int k, ExactList<KMAX>& L,
+                                            uint64_t& visits, uint64_t& tests, int max_steps) {
+    int stk_ref[EXACT_STACK];
+    float stk_d[EXACT_STACK];
+    int sp = 0;
+    double wd = INFINITY;
+    uint32_t wi = GPRT_ID_NONE;
+    auto scan = [&](int f, int m) {
+        for (int t = 0; t < m; ++t) {
+            const float4 p = pts[f + t];
+            const double dd = d2d(qx, qy, qz, p.x, p.y, p.z);
+            const uint32_t ii = (uint32_t)__float_as_int(p.w);
+            if (dd < wd || (dd == wd && ii < wi)) {
+                L.insert(dd, ii, k);
+                L.worst(k, wd, wi);
+            }
+        }
+        tests += m;
+    };
+    if (root.y <= 0) return true;
+    int ref = root.x;
+    if (ref < 0) { scan(~ref, root.y); return true; }
+    int steps = 0;
+    while (true) {
+        if (++steps > max_steps) return false;
+        const Node nd = nodes[ref];
+        ++visits;
+        const double dA = box2d(qx, qy, qz, nd.alo, nd.ahi);
+        const double dB = box2d(qx, qy, qz, nd.blo, nd.bhi);
+        const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+        const bool lA = rA < 0, lB = rB < 0;
+        // a box whose value equals the worst can still hold a smaller id
+        if (lA && lB && dB < dA) {
+            if (dB <= wd) scan(~rB, __float_as_int(nd.bhi.w));
+            if (dA <= wd) scan(~rA, __float_as_int(nd.ahi.w));
+        } else {
+            if (lA && dA <= wd) scan(~rA, __float_as_int(nd.ahi.w));
+            if (lB && dB <= wd) scan(~rB, __float_as_int(nd.bhi.w));
+        }
+        const bool gA = !lA && dA <= wd, gB = !lB && dB <= wd;
+        if (gA && gB) {
+            const bool nearA = dA <= dB;
+            if (sp >= EXACT_STACK) return false;
+            stk_ref[sp] = nearA ? rB : rA;
+            stk_d[sp] = f64_lower_f32(nearA ? dB : dA);
+            ++sp;
+            ref = nearA ? rA : rB;
+            continue;
+        }
+        if (gA) { ref = rA; continue; }
+        if (gB) { ref = rB; continue; }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if ((double)stk_d[sp] <= wd) { ref = stk_ref[sp]; found = true; break; }
+        }
+        if (!found) break;
+    }
+    return true;
+}
+
+// mode 0: plain form, query i of the caller's xyz array (input order)
+// mode 1: fallback, sorted position fb_list[i]
+template <int KMAX, bool COUNT>
+__global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned n = a.mode == 0 ? (unsigned)a.nq : a.counters[CNT_FALLBACK];
+    if ((unsigned)i >= n) return;
+    float qx, qy, qz;
+    int g, c = 0;
+    if (a.mode == 0) {
+        qx = a.q_xyz[3 * (int64_t)i]; qy = a.q_xyz[3 * (int64_t)i + 1]; qz = a.q_xyz[3 * (int64_t)i + 2];
+        g = i;
+    } else {
+        const int s = a.fb_list[i];
+        const float4 q = a.qs[s];
+        qx = q.x; qy = q.y; qz = q.z;
+        g = __float_as_int(q.w);
+        c = a.qcloud ? a.qcloud[s] : 0;
+    }
+    const int k = a.kc ? a.kc[c] : a.k;
+    ExactList<KMAX> L;
+    L.init();
+    uint64_t visits = 0, tests = 0;
+    if (!exact_query<KMAX>(a.nodes, a.pts, a.roots[c], qx, qy, qz, k, L, visits, tests, a.max_steps))
+        atomicAdd(&a.counters[CNT_ERROR], 1u);
+    const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        if (j < k) {
+            a.ids_out[ob + j] = L.id[j];
+            if (a.d2_out) a.d2_out[ob + j] = L.d[j];
+        }
+    if (COUNT) {
+        atomicAdd(&a.stats64[0], (unsigned long long)visits);
+        atomicAdd(&a.stats64[1], (unsigned long long)tests);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_t* inv, const int64_t* poff,
+                   const float* qs,
+                   const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth,
+                   float* radius) {
+    if (heuristic != GPRT_RADIUS_INFINITE && !(sampler == GPRT_SAMPLER_STRIDED && heuristic != GPRT_RADIUS_PERCENTILE)) {
+        if (sampler == GPRT_SAMPLER_LCG)
+            hipLaunchKernelGGL(sampler_lcg_kernel, dim3(100, nclouds), dim3(256), 0, s, pts4, inv, poff, qs, qoff,
+                               kth);
+        else
+            hipLaunchKernelGGL(sampler_strided_kernel, dim3(100, nclouds), dim3(256), 0, s, pts4, inv, poff, qs, qoff,
+                               k, kth);
+    } else {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3((100 * nclouds + 255) / 256), dim3(256), 0, s, (unsigned*)kth,
+                           100 * nclouds, 0xBF800000u /* -1.0f */);
+    }
+    hipLaunchKernelGGL(sampler_finalize_kernel, dim3((nclouds + 63) / 64), dim3(64), 0, s, kth, nclouds, sampler,
+                       heuristic, param, radius);
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "sampler launch failed");
+}
+
+int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
+                      int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
+                      int64_t max_per_cloud) {
+    hipStream_t s = ctx->stream;
+    int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
+    hipLaunchKernelGGL(query_keys_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, keys,
+                       perm);
+    int cloud_bits = 0;
+    while ((1 << cloud_bits) < nclouds) ++cloud_bits;
+    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 0, 30));
+    if (cloud_bits > 0) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 32, 32 + cloud_bits));
+    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, nq, qs4, counters);
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
+}
+
+__global__ void key_cloud_kernel(const uint64_t* keys, int n, int* cloud) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cloud[i] = (int)(keys[i] >> 32);
+}
+
+int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud) {
+    hipLaunchKernelGGL(key_cloud_kernel, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, cloud);
+    return GPRT_OK;
+}
+
+int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
+    if (n > 0) hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return GPRT_OK;
+}
+
+template <int C, int STACK>
+static void launch_round_t(hipStream_t s, const RoundArgs& a, int grid, bool count) {
+    if (count)
+        hipLaunchKernelGGL((knn_round_kernel<C, 256, STACK, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((knn_round_kernel<C, 256, STACK, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int C>
+static int launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int stack, bool count) {
+    if (stack <= 16) launch_round_t<C, 16>(s, a, grid, count);
+    else launch_round_t<C, 32>(s, a, grid, count);
+    return GPRT_OK;
+}
+
+int capacity_class(int kmax, int guard) {
+    int need = kmax + guard;
+    if (need <= 8) return 8;
+    if (need <= 16) return 16;
+    if (need <= 32) return 32;
+    return 64;
+}
+
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int stack, bool count) {
+    switch (cap) {
+        case 8: launch_round_c<8>(s, a, grid, stack, count); break;
+        case 16: launch_round_c<16>(s, a, grid, stack, count); break;
+        case 32: launch_round_c<32>(s, a, grid, stack, count); break;
+        default: launch_round_c<64>(s, a, grid, stack, count); break;
+    }
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
+}
+
+int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count) {
+    dim3 grid((grid_threads + 255) / 256);
+    if (grid.x == 0) return GPRT_OK;
+#define LX(KM)                                                                          \
+    if (count) hipLaunchKernelGGL((knn_exact_kernel<KM, true>), grid, dim3(256), 0, s, a); \
+    else hipLaunchKernelGGL((knn_exact_kernel<KM, false>), grid, dim3(256), 0, s, a);
+    if (kmax <= 8) { LX(8) }
+    else if (kmax <= 16) { LX(16) }
+    else if (kmax <= 32) { LX(32) }
+    else { LX(64) }
+#undef LX
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "exact launch failed");
+}
+
+}  // namespace gprt

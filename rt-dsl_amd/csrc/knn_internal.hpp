@@ -1,0 +1,78 @@
+// knn_internal.hpp -- launch arguments and launchers of the k-NN kernels.
+#pragma once
+#include "gprt_internal.hpp"
+
+namespace gprt {
+
+// device counter slots (unsigned[CNT_TOTAL])
+constexpr int MAX_ROUNDS = 16;
+constexpr int CNT_ACTIVE = 0;     // [r]: queries active in round r (r = 0 .. MAX_ROUNDS)
+constexpr int CNT_WORK = 20;      // [r]: work-queue head of round r
+constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
+constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
+constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
+constexpr int CNT_TOTAL = 48;
+
+// Karras depth bound: 30 Morton bits + up to 32 cloud bits... a path of the
+// radix tree has strictly increasing common-prefix lengths, so its length is
+// at most (key bits) + (index tie-break bits) = 30 + cloud bits + 32 <= 96.
+constexpr int EXACT_STACK = 96;
+
+struct RoundArgs {
+    const Node* nodes;
+    const float4* pts;
+    const int2* roots;
+    const float4* qs;        // queries in processing order, w = global query index
+    const int* qcloud;       // cloud of each sorted query (nullptr: cloud 0)
+    const int* kc;           // k per cloud (nullptr: uniform k)
+    int k;
+    const float* radius0;    // initial radius per cloud
+    float mult;
+    int round;
+    int last;                // 1: unbounded round, every query resolves
+    const int* in_list;      // nullptr: identity over [0, count)
+    int* out_list;
+    int* fb_list;
+    unsigned* counters;
+    unsigned long long* stats64;
+    uint32_t* ids_out;
+    double* d2_out;
+    const int64_t* out_base; // per cloud output offset (nullptr: g * k)
+    const int64_t* qoff;     // per cloud first global query index
+    int max_steps;           // node-visit guard (> number of internal nodes)
+};
+
+struct ExactArgs {
+    int mode;                // 0 plain (caller's xyz array, input order), 1 fallback list
+    int64_t nq;
+    const float* q_xyz;
+    const float4* qs;
+    const int* fb_list;
+    const int* qcloud;
+    const int* kc;
+    int k;
+    const Node* nodes;
+    const float4* pts;
+    const int2* roots;
+    unsigned* counters;
+    unsigned long long* stats64;
+    uint32_t* ids_out;
+    double* d2_out;
+    const int64_t* out_base;
+    const int64_t* qoff;
+    int max_steps;
+};
+
+int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_t* inv, const int64_t* poff,
+                   const float* qs,
+                   const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth, float* radius);
+int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
+                      int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
+                      int64_t max_per_cloud);
+int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud);
+int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
+int capacity_class(int kmax, int guard);
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int stack, bool count);
+int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
+
+}  // namespace gprt

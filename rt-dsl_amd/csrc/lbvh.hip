@@ -1,0 +1,477 @@
+// lbvh.hip -- GPU LBVH build: per-cloud bounds, 30-bit Morton keys, stable LSD
+// radix sort staged through LDS, Karras hierarchy, atomic bottom-up refit.
+//
+// Replaces the reference's closed OptiX GAS build (gprt-optix/host/main.cpp:
+// 200-248), its per-round refit kernel (kernels.cu:3-16, main.cpp:254-286)
+// and the host-side Scene::build (gprt-core/src/lib.rs:135-210) / Morton sort
+// (gprt-core/src/morton.rs:19-56).
+#include <algorithm>
+#include <vector>
+
+#include "gprt_internal.hpp"
+
+namespace gprt {
+
+// ---------------------------------------------------------------------------
+// bounds: float min/max through order-preserving int encoding + atomics
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int f2ord(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__global__ void bbox_init_kernel(int* enc, int nclouds) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nclouds) {
+        for (int a = 0; a < 3; ++a) {
+            enc[6 * i + a] = f2ord(3.402823466e38f);       // f32::MAX (morton.rs:25)
+            enc[6 * i + 3 + a] = f2ord(-3.402823466e38f);  // f32::MIN (morton.rs:26)
+        }
+    }
+}
+
+// grid: (chunks, clouds).  Each block reduces its slice of its cloud.
+__global__ void bbox_reduce_kernel(const float* __restrict__ pts, const int64_t* __restrict__ off, int* enc) {
+    int c = blockIdx.y;
+    int64_t b = off[c], e = off[c + 1];
+    float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        for (int a = 0; a < 3; ++a) {
+            float v = pts[3 * i + a];
+            mn[a] = fminf(mn[a], v);
+            mx[a] = fmaxf(mx[a], v);
+        }
+    }
+    // wave reduce
+    for (int o = 32; o > 0; o >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], o));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o));
+        }
+    if (__lane_id() == 0 && b < e) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&enc[6 * c + a], f2ord(mn[a]));
+            atomicMax(&enc[6 * c + 3 + a], f2ord(mx[a]));
+        }
+    }
+}
+
+__global__ void bbox_decode_kernel(const int* enc, float* bbox, int nclouds) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6 * nclouds) bbox[i] = ord2f(enc[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Morton keys (gprt-core/src/morton.rs:4-48)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t expand_bits(uint32_t x) {
+    x &= 0x000003ffu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// Rust `f32 as u32` (saturating, NaN -> 0) followed by .min(1023)
+__device__ __forceinline__ uint32_t quantize(float p, float mn, float ext) {
+    float t = (p - mn) / ext * 1023.0f;
+    uint32_t u = !(t > 0.0f) ? 0u : (t >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)t);
+    return u < 1023u ? u : 1023u;
+}
+
+__device__ __forceinline__ uint32_t morton_of(float x, float y, float z, const float* bb) {
+    float ex = fmaxf(bb[3] - bb[0], 1e-6f);
+    float ey = fmaxf(bb[4] - bb[1], 1e-6f);
+    float ez = fmaxf(bb[5] - bb[2], 1e-6f);
+    return (expand_bits(quantize(x, bb[0], ex)) << 2) | (expand_bits(quantize(y, bb[1], ey)) << 1) |
+           expand_bits(quantize(z, bb[2], ez));
+}
+
+// grid: (chunks, clouds).  key = cloud << 32 | morton; vals = input index.
+__global__ void morton_keys_kernel(const float* __restrict__ pts, const int64_t* __restrict__ off,
+                                   const float* __restrict__ bbox, uint64_t* __restrict__ keys,
+                                   uint32_t* __restrict__ vals) {
+    int c = blockIdx.y;
+    int64_t b = off[c], e = off[c + 1];
+    float bb[6];
+    for (int a = 0; a < 6; ++a) bb[a] = bbox[6 * c + a];
+    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = morton_of(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], bb);
+        keys[i] = ((uint64_t)c << 32) | m;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void morton_codes_kernel(const float* __restrict__ pts, int n, const float* __restrict__ bbox,
+                                    uint32_t* __restrict__ codes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float bb[6];
+        for (int a = 0; a < 6; ++a) bb[a] = bbox[a];
+        codes[i] = morton_of(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], bb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort, 8-bit digits, 4096-key tiles (256 lanes x 16 keys)
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_BINS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+// hist layout: digit-major [RS_BINS][nblocks] so one exclusive scan yields the
+// output offset of (digit, tile).
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict__ keys, int n, int shift,
+                                                             uint32_t* __restrict__ hist, int nblocks) {
+    __shared__ uint32_t h[RS_BINS];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+        int idx = base + i * RS_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// single-block exclusive scan of m uint32 values, in place
+__global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t* data, int m) {
+    __shared__ uint32_t sums[1024];
+    int t = threadIdx.x;
+    int per = (m + 1023) / 1024;
+    int b = t * per, e = min(b + per, m);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += data[i];
+    sums[t] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 sums
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t v = t >= o ? sums[t - o] : 0;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t > 0 ? sums[t - 1] : 0;
+    for (int i = b; i < e; ++i) {
+        uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                                int n, int shift,
+                                                                const uint32_t* __restrict__ hist, int nblocks) {
+    __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
+    __shared__ uint32_t boff[RS_BINS];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int i = t; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
+    boff[t] = hist[t * nblocks + blockIdx.x];
+    __syncthreads();
+
+    const int base = blockIdx.x * RS_TILE + w * (64 * RS_ITEMS);
+    K k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+        int idx = base + i * 64 + lane;
+        bool valid = idx < n;
+        k[i] = valid ? kin[idx] : (K)0;
+        v[i] = valid ? vin[idx] : 0u;
+        uint32_t d = (uint32_t)(k[i] >> shift) & 0xFFu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            bool b = (d >> bit) & 1u;
+            uint64_t m = __ballot(b);
+            peers &= b ? m : ~m;
+        }
+        uint32_t cur = valid ? wcnt[w][d] : 0u;
+        rank[i] = cur + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lane == __ffsll((unsigned long long)peers) - 1) wcnt[w][d] = cur + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        uint32_t s = 0;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            uint32_t c = wcnt[ww][t];
+            wcnt[ww][t] = s;
+            s += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+        int idx = base + i * 64 + lane;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(k[i] >> shift) & 0xFFu;
+            uint32_t pos = boff[d] + wcnt[w][d] + rank[i];
+            kout[pos] = k[i];
+            vout[pos] = v[i];
+        }
+    }
+}
+
+template <typename K>
+static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
+    if (n <= 1 || begin_bit >= end_bit) return GPRT_OK;
+    int nblocks = (n + RS_TILE - 1) / RS_TILE;
+    GPRT_TRY(ctx->sort_tmp[0].reserve(sizeof(K) * (size_t)n));
+    GPRT_TRY(ctx->sort_tmp[1].reserve(sizeof(uint32_t) * (size_t)n));
+    GPRT_TRY(ctx->sort_tmp[2].reserve(sizeof(uint32_t) * (size_t)RS_BINS * nblocks));
+    K* k0 = keys;
+    uint32_t* v0 = vals;
+    K* k1 = ctx->sort_tmp[0].as<K>();
+    uint32_t* v1 = ctx->sort_tmp[1].as<uint32_t>();
+    uint32_t* hist = ctx->sort_tmp[2].as<uint32_t>();
+    int passes = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, n, shift, hist,
+                           nblocks);
+        hipLaunchKernelGGL(scan_exclusive_kernel, dim3(1), dim3(1024), 0, ctx->stream, hist, RS_BINS * nblocks);
+        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, v0, k1, v1,
+                           n, shift, hist, nblocks);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    GPRT_HIP_TRY(hipGetLastError());
+    if (k0 != keys) {
+        GPRT_HIP_TRY(hipMemcpyAsync(keys, k0, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
+        GPRT_HIP_TRY(hipMemcpyAsync(vals, v0, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return GPRT_OK;
+}
+
+int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
+    return radix_sort_impl<uint64_t>(ctx, keys, vals, n, begin_bit, end_bit);
+}
+int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
+    return radix_sort_impl<uint32_t>(ctx, keys, vals, n, begin_bit, end_bit);
+}
+
+// ---------------------------------------------------------------------------
+// gather sorted points: pts4[i] = (xyz of input perm[i], id local to its cloud)
+// ---------------------------------------------------------------------------
+__global__ void gather_points_kernel(const float* __restrict__ pts, const uint32_t* __restrict__ perm,
+                                     const uint64_t* __restrict__ keys, const int64_t* __restrict__ off,
+                                     float4* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint32_t src = perm[i];
+        int c = (int)(keys[i] >> 32);
+        int id = (int)((int64_t)src - off[c]);
+        out[i] = make_float4(pts[3 * src], pts[3 * src + 1], pts[3 * src + 2], __int_as_float(id));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Karras hierarchy over the sorted keys with the index as tie-break
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int delta_fn(const uint64_t* __restrict__ keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = keys[i], b = keys[j];
+    if (a == b) return 64 + __clz((unsigned)(i ^ j));
+    return __clzll((long long)(a ^ b));
+}
+
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, int leaf_size,
+                              const int64_t* __restrict__ off, Node* __restrict__ nodes, int* __restrict__ parent,
+                              int2* __restrict__ roots) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int dr = delta_fn(keys, n, i, i + 1), dl = delta_fn(keys, n, i, i - 1);
+    int d = dr > dl ? 1 : -1;
+    int dmin = min(dr, dl);
+    int lmax = 2;
+    while (delta_fn(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta_fn(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta_fn(keys, n, i, j);
+    int s = 0;
+    for (int div = 2;; div <<= 1) {
+        int t = (l + div - 1) / div;
+        if (delta_fn(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int cntA = gamma - lo + 1, cntB = hi - gamma;
+    // child refs: single points are always leaves; small subtrees become leaf buckets
+    int refA = (lo == gamma || cntA <= leaf_size) ? ~lo : gamma;
+    int refB = (hi == gamma + 1 || cntB <= leaf_size) ? ~(gamma + 1) : gamma + 1;
+    Node* nd = nodes + i;
+    nd->alo.w = __int_as_float(refA);
+    nd->ahi.w = __int_as_float(cntA);
+    nd->blo.w = __int_as_float(refB);
+    nd->bhi.w = __int_as_float(cntB);
+    // parents for the refit, as (node << 1 | side): internal children at
+    // [0, n-1), points at [n-1, 2n-1)
+    parent[(lo == gamma ? (n - 1) : 0) + gamma] = (i << 1) | 0;
+    parent[(hi == gamma + 1 ? (n - 1) : 0) + gamma + 1] = (i << 1) | 1;
+    if (i == 0) parent[0] = -1;
+    // cloud roots: the node whose range is exactly one cloud
+    int c = (int)(keys[lo] >> 32);
+    if ((int64_t)lo == off[c] && (int64_t)hi == off[c + 1] - 1 && (hi - lo + 1) > leaf_size) roots[c] = make_int2(i, hi - lo + 1);
+}
+
+__global__ void roots_small_kernel(const int64_t* __restrict__ off, int nclouds, int leaf_size, int2* roots) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nclouds) {
+        int64_t cnt = off[c + 1] - off[c];
+        if (cnt <= leaf_size) roots[c] = make_int2(~(int)off[c], (int)cnt);  // empty cloud: count 0
+    }
+}
+
+// bottom-up refit: one thread per point; the second thread to reach a node
+// merges the two child boxes and continues upward.
+__global__ void refit_kernel(const float4* __restrict__ pts4, int n, Node* nodes, const int* __restrict__ parent,
+                             int* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pts4[i];
+    float lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
+    int link = parent[(n - 1) + i];
+    while (link >= 0) {
+        int node = link >> 1;
+        bool sideA = (link & 1) == 0;
+        Node* nd = nodes + node;
+        float* slo = sideA ? &nd->alo.x : &nd->blo.x;
+        float* shi = sideA ? &nd->ahi.x : &nd->bhi.x;
+        slo[0] = lo[0]; slo[1] = lo[1]; slo[2] = lo[2];
+        shi[0] = hi[0]; shi[1] = hi[1]; shi[2] = hi[2];
+        // release our box, acquire the sibling's (agent scope: the sibling
+        // may run on another CU whose L1 is not coherent with ours)
+        int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const float* olo = sideA ? &nd->blo.x : &nd->alo.x;
+        const float* ohi = sideA ? &nd->bhi.x : &nd->ahi.x;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], __hip_atomic_load(olo + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            hi[a] = fmaxf(hi[a], __hip_atomic_load(ohi + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+        link = parent[node];
+    }
+}
+
+__global__ void invert_perm_kernel(const uint32_t* __restrict__ perm, uint32_t* __restrict__ inv, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (uint32_t)i;
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, int leaf_size, Tree* t) {
+    int64_t n64 = h_off[nclouds];
+    if (n64 > (int64_t)0x7FFFFFF0) return fail(GPRT_ERR_ARG, "too many points for one tree");
+    int n = (int)n64;
+    t->n = n;
+    t->nclouds = nclouds;
+    t->leaf_size = leaf_size;
+    t->h_cloud_off.assign(h_off, h_off + nclouds + 1);
+    int cloud_bits = 0;
+    while ((1 << cloud_bits) < nclouds) ++cloud_bits;
+    t->key_bits = 30 + cloud_bits;
+    t->max_depth_bound = 30 + cloud_bits + 32;
+    hipStream_t s = ctx->stream;
+
+    GPRT_TRY(t->cloud_off.reserve(sizeof(int64_t) * (nclouds + 1)));
+    GPRT_TRY(t->cloud_bbox.reserve(sizeof(float) * 6 * nclouds));
+    GPRT_TRY(t->roots.reserve(sizeof(int2) * nclouds));
+    GPRT_TRY(t->pts4.reserve(sizeof(float4) * std::max(n, 1)));
+    GPRT_TRY(t->nodes.reserve(sizeof(Node) * std::max(n - 1, 1)));
+    GPRT_TRY(t->perm.reserve(sizeof(uint32_t) * std::max(n, 1)));
+    GPRT_TRY(t->inv.reserve(sizeof(uint32_t) * std::max(n, 1)));
+    GPRT_TRY(t->codes.reserve(sizeof(uint64_t) * std::max(n, 1)));
+    GPRT_TRY(t->parent.reserve(sizeof(int) * std::max(2 * n - 1, 1)));
+    GPRT_TRY(t->flags.reserve(sizeof(int) * std::max(n - 1, 1)));
+    GPRT_TRY(ctx->ws[7].reserve(sizeof(int) * 6 * nclouds));
+    int* enc = ctx->ws[7].as<int>();
+
+    GPRT_HIP_TRY(hipMemcpyAsync(t->cloud_off.ptr, h_off, sizeof(int64_t) * (nclouds + 1), hipMemcpyHostToDevice, s));
+    int64_t maxc = 0;
+    for (int c = 0; c < nclouds; ++c) maxc = std::max(maxc, h_off[c + 1] - h_off[c]);
+    int chunks = std::max(1, std::min(cdiv(maxc, 256 * 8), 2048));
+
+    hipLaunchKernelGGL(bbox_init_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, enc, nclouds);
+    hipLaunchKernelGGL(bbox_reduce_kernel, dim3(chunks, nclouds), dim3(256), 0, s, d_pts,
+                       t->cloud_off.as<int64_t>(), enc);
+    hipLaunchKernelGGL(bbox_decode_kernel, dim3(cdiv(6 * nclouds, 256)), dim3(256), 0, s, enc,
+                       t->cloud_bbox.as<float>(), nclouds);
+    hipLaunchKernelGGL(roots_small_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, t->cloud_off.as<int64_t>(),
+                       nclouds, leaf_size, t->roots.as<int2>());
+    if (n > 0) {
+        hipLaunchKernelGGL(morton_keys_kernel, dim3(std::max(1, std::min(cdiv(maxc, 256), 4096)), nclouds),
+                           dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->cloud_bbox.as<float>(),
+                           t->codes.as<uint64_t>(), t->perm.as<uint32_t>());
+        GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 0, 30));
+        if (cloud_bits > 0)
+            GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 32,
+                                          32 + cloud_bits));
+        hipLaunchKernelGGL(gather_points_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts,
+                           t->perm.as<uint32_t>(), t->codes.as<uint64_t>(), t->cloud_off.as<int64_t>(),
+                           t->pts4.as<float4>(), n);
+        hipLaunchKernelGGL(invert_perm_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, t->perm.as<uint32_t>(),
+                           t->inv.as<uint32_t>(), n);
+    }
+    if (n > 1) {
+        hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->flags.as<int>(), n - 1, 0);
+        hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), n,
+                           leaf_size, t->cloud_off.as<int64_t>(), t->nodes.as<Node>(), t->parent.as<int>(),
+                           t->roots.as<int2>());
+        hipLaunchKernelGGL(refit_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, t->pts4.as<float4>(), n,
+                           t->nodes.as<Node>(), t->parent.as<int>(), t->flags.as<int>());
+    }
+    GPRT_HIP_TRY(hipGetLastError());
+    GPRT_HIP_TRY(hipMemcpyAsync(t->h_bbox, t->cloud_bbox.ptr, sizeof(float) * 6, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    return GPRT_OK;
+}
+
+int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6) {
+    hipStream_t s = ctx->stream;
+    GPRT_TRY(ctx->ws[7].reserve(64));
+    int64_t h_off[2] = {0, n};
+    int* enc = ctx->ws[7].as<int>();
+    int64_t* d_off = (int64_t*)(ctx->ws[7].as<char>() + 32);
+    GPRT_HIP_TRY(hipMemcpyAsync(d_off, h_off, sizeof(h_off), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(bbox_init_kernel, dim3(1), dim3(64), 0, s, enc, 1);
+    if (n > 0) {
+        hipLaunchKernelGGL(bbox_reduce_kernel, dim3(std::max(1, std::min(cdiv(n, 2048), 2048)), 1), dim3(256), 0, s,
+                           d_pts, d_off, enc);
+    }
+    hipLaunchKernelGGL(bbox_decode_kernel, dim3(1), dim3(64), 0, s, enc, d_bbox6, 1);
+    if (n > 0)
+        hipLaunchKernelGGL(morton_codes_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts, n, d_bbox6, d_codes);
+    GPRT_HIP_TRY(hipGetLastError());
+    return GPRT_OK;
+}
+
+}  // namespace gprt

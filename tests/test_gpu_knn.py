@@ -1,0 +1,183 @@
+"""k-NN parity on the GPU: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bar: bit-exact ids (and binary64 distances) against oracle.knn_brute /
+oracle.kdtree_knn, which are pinned in test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+import gprt_hip
+import oracle
+from gprt_hip import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_knn(ids, ref, d2=None, d2ref=None, label=""):
+    ids = np.asarray(ids, dtype=np.uint32).reshape(ref.shape)
+    bad = np.nonzero((ids != ref).any(axis=1))[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} queries differ, first {bad[:5]}: got {ids[bad[:3]]} want {ref[bad[:3]]}"
+    if d2 is not None:
+        assert np.array_equal(np.asarray(d2).reshape(d2ref.shape), d2ref), f"{label}: distances differ"
+
+
+def test_knn_uniform_plain_and_batched(ctx):
+    pts, q = D.uniform_cloud()  # 100K x 10K, k = 5
+    ref, d2ref = oracle.knn_brute(pts, q, 5, with_d2=True)
+    t = ctx.build(pts, leaf_size=8)
+    ids, d2 = t.k_nn_plain(q, 5, with_d2=True)
+    _check_knn(ids, ref, d2, d2ref, "plain")
+    ids, d2 = t.k_nn(q, 5, with_d2=True)
+    _check_knn(ids, ref, d2, d2ref, "batched")
+    st = ctx.stats()
+    assert st["queries"] == len(q)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 13, 16, 32, 61])
+def test_knn_k_values(ctx, k):
+    rng = np.random.default_rng(10 + k)
+    pts = rng.random((20_000, 3), dtype=np.float32)
+    q = rng.random((3_000, 3), dtype=np.float32)
+    ref = oracle.knn_brute(pts, q, k)
+    t = ctx.build(pts, leaf_size=8)
+    _check_knn(t.k_nn(q, k), ref, label=f"batched k={k}")
+    _check_knn(t.k_nn_plain(q, k), ref, label=f"plain k={k}")
+
+
+@pytest.mark.parametrize("leaf", [1, 2, 4, 16, 32])
+def test_knn_leaf_sizes(ctx, leaf):
+    rng = np.random.default_rng(leaf)
+    pts = rng.normal(size=(30_000, 3)).astype(np.float32)
+    q = rng.normal(size=(2_000, 3)).astype(np.float32)
+    ref = oracle.knn_brute(pts, q, 5)
+    t = ctx.build(pts, leaf_size=leaf)
+    _check_knn(t.k_nn(q, 5), ref, label=f"leaf={leaf}")
+    _check_knn(t.k_nn_plain(q, 5), ref, label=f"plain leaf={leaf}")
+
+
+@pytest.mark.parametrize("cfgkw", [
+    dict(),
+    dict(radius_heuristic=gprt_hip.RADIUS_INFINITE),
+    dict(radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=15, radius_mult=2.0),
+    dict(radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=3),
+    dict(sampler=gprt_hip.SAMPLER_STRIDED, radius_param=0.10),
+    dict(radius_heuristic=gprt_hip.RADIUS_MAX),
+    dict(sort_queries=0),
+    dict(use_graph=0),
+    dict(stack_depth=32, grid_blocks=7),
+    dict(guard_slots=1),
+])
+def test_knn_config_independence(ctx, cfgkw):
+    """The result must not depend on the initial radius, multiplier, rounds,
+    query order, graph use, stack depth or grid size."""
+    pts, q = D.uniform_cloud(50_000, 5_000, seed=7)
+    ref = oracle.knn_brute(pts, q, 5)
+    t = ctx.build(pts)
+    cfg = gprt_hip.default_config(**cfgkw)
+    _check_knn(t.k_nn(q, 5, cfg), ref, label=str(cfgkw))
+    # second call replays the cached graph
+    _check_knn(t.k_nn(q, 5, cfg), ref, label=str(cfgkw) + " replay")
+
+
+def test_knn_ties_lattice(ctx):
+    """Integer lattice: many exactly equal distances; ids break the ties and the
+    guard band overflows into the exact kernel."""
+    g = np.arange(24, dtype=np.float32)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).copy()
+    rng = np.random.default_rng(1)
+    q = np.concatenate([pts[rng.integers(0, len(pts), 500)],
+                        pts[rng.integers(0, len(pts), 500)] + 0.5,
+                        rng.integers(0, 24, (500, 3)).astype(np.float32) + np.float32(0.25)]).astype(np.float32)
+    for k in (1, 6, 7, 27):
+        ref = oracle.knn_brute(pts, q, k)
+        t = ctx.build(pts, leaf_size=4)
+        _check_knn(t.k_nn(q, k), ref, label=f"lattice k={k}")
+        _check_knn(t.k_nn_plain(q, k), ref, label=f"lattice plain k={k}")
+    assert ctx.stats()["queries"] == len(q)
+
+
+def test_knn_duplicates(ctx):
+    rng = np.random.default_rng(3)
+    base = rng.random((200, 3), dtype=np.float32)
+    pts = np.repeat(base, 50, axis=0)  # every point 50 times
+    q = base[rng.integers(0, 200, 1000)]
+    ref = oracle.knn_brute(pts, q, 8)
+    t = ctx.build(pts)
+    _check_knn(t.k_nn(q, 8), ref, label="duplicates")
+    _check_knn(t.k_nn_plain(q, 8), ref, label="duplicates plain")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 9])
+def test_knn_small_clouds(ctx, n):
+    """Clouds smaller than k: n ids then 0xFFFFFFFF padding."""
+    rng = np.random.default_rng(n)
+    pts = rng.random((n, 3), dtype=np.float32)
+    q = rng.random((100, 3), dtype=np.float32)
+    ref = oracle.knn_brute(pts, q, 5)
+    t = ctx.build(pts)
+    _check_knn(t.k_nn(q, 5), ref, label=f"n={n}")
+    _check_knn(t.k_nn_plain(q, 5), ref, label=f"plain n={n}")
+
+
+def test_knn_empty_inputs(ctx):
+    pts = np.random.default_rng(0).random((1000, 3), dtype=np.float32)
+    t = ctx.build(pts)
+    assert t.k_nn(np.zeros((0, 3), np.float32), 5).shape == (0, 5)
+    t0 = ctx.build(np.zeros((0, 3), np.float32))
+    ids = t0.k_nn(pts[:10], 3)
+    assert (ids == gprt_hip.ID_NONE).all()
+
+
+def test_knn_rejects_nan(ctx):
+    pts = np.random.default_rng(0).random((1000, 3), dtype=np.float32)
+    bad = pts.copy()
+    bad[17, 1] = np.nan
+    with pytest.raises(gprt_hip.GprtError) as e:
+        ctx.build(bad)
+    assert e.value.code == gprt_hip.ERR_INPUT
+    t = ctx.build(pts)
+    q = pts[:100].copy()
+    q[3, 0] = np.inf
+    with pytest.raises(gprt_hip.GprtError) as e:
+        t.k_nn(q, 5)
+    assert e.value.code == gprt_hip.ERR_INPUT
+
+
+def test_knn_road_network_subset(ctx):
+    """1e5-metre coordinates: FP32 spacing ~8 mm, many near-ties in FP32."""
+    pts, q = D.road_network(nq=50_000)
+    ref, _, _, _ = oracle.kdtree_knn(pts, q, 5)
+    t = ctx.build(pts)
+    _check_knn(t.k_nn(q, 5), ref, label="road batched")
+    _check_knn(t.k_nn_plain(q, 5), ref, label="road plain")
+
+
+def test_knn_lidar_full(ctx):
+    """The north-star workload at full size: 1,310,720 points x 400,000 queries,
+    k = 5 and k = 32, against the CPU KD-tree (itself pinned to brute force)."""
+    pts, q = D.lidar_scan()
+    t = ctx.build(pts)
+    for k in (5, 32):
+        ref, _, _, _ = oracle.kdtree_knn(pts, q, k)
+        _check_knn(t.k_nn(q, k), ref, label=f"lidar batched k={k}")
+        st = ctx.stats()
+        print(f"lidar k={k}: rounds={st['rounds_run']} fallback={st['fallback_queries']} "
+              f"active={st['active_per_round'][:8]} ms={st['ms_query']:.3f}")
+        _check_knn(t.k_nn_plain(q, k), ref, label=f"lidar plain k={k}")
+
+
+def test_knn_device_pointers(ctx):
+    """Inputs and outputs resident in HBM (GPRT_PTR_DEVICE)."""
+    pts, q = D.uniform_cloud(50_000, 4_000, seed=9)
+    ref = oracle.knn_brute(pts, q, 5)
+    dp = gprt_hip.DeviceArray.from_host(ctx, pts)
+    dq = gprt_hip.DeviceArray.from_host(ctx, q)
+    t = ctx.build(dp)
+    _check_knn(t.k_nn(dq, 5).to_host(), ref, label="device batched")
+    _check_knn(t.k_nn_plain(dq, 5).to_host(), ref, label="device plain")
+    # enqueue-only calls, then one sync
+    out = gprt_hip.DeviceArray(ctx, (len(q), 5), np.uint32)
+    for _ in range(3):
+        t.k_nn(dq, 5, ids_out=out, sync=False)
+    ctx.sync()
+    _check_knn(out.to_host(), ref, label="device no-sync")
