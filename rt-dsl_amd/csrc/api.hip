@@ -375,7 +375,7 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     size_t nblocks = (nq + 4095) / 4096;
     GPRT_TRY(c.sort_tmp[0].reserve(8 * nq));
     GPRT_TRY(c.sort_tmp[1].reserve(4 * nq));
-    GPRT_TRY(c.sort_tmp[2].reserve(4 * 256 * nblocks));
+    GPRT_TRY(c.sort_tmp[2].reserve(4 * 256 * nblocks + 1024));
     return GPRT_OK;
 }
 

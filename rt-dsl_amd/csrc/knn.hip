@@ -172,31 +172,43 @@ __global__ __launch_bounds__(256) void sampler_strided_kernel(const float4* __re
     if (threadIdx.x == 0) kth[c * 100 + s] = (k <= 1024 && v[k - 1] < INFINITY) ? sqrtf(v[k - 1]) : -1.0f;
 }
 
-// one thread per cloud: radius from the pushed samples (dispatch.rs:92-103 for
-// LCG, lib.rs:442-447 for strided).
-__global__ void sampler_finalize_kernel(const float* kth, int nclouds, int sampler, int heuristic, float param,
-                                        float* radius) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nclouds) return;
-    float v[100];
-    int len = 0;
-    for (int s = 0; s < 100; ++s) {
-        float x = kth[c * 100 + s];
-        if (x >= 0.0f) {
-            int j = len++;
-            while (j > 0 && v[j - 1] > x) { v[j] = v[j - 1]; --j; }
-            v[j] = x;
+// one block (128 threads) per cloud: radius from the pushed samples
+// (dispatch.rs:92-103 for LCG, lib.rs:442-447 for strided).  The <= 100 values
+// are ordered by rank (value, slot) so the block needs no sequential sort.
+__global__ __launch_bounds__(128) void sampler_finalize_kernel(const float* kth, int nclouds, int sampler,
+                                                               int heuristic, float param, float* radius) {
+    __shared__ float v[100];
+    __shared__ float sorted[100];
+    __shared__ int len_s;
+    const int c = blockIdx.x, t = threadIdx.x;
+    if (t < 100) v[t] = kth[c * 100 + t];
+    __syncthreads();
+    // rank among the pushed values (negative = not pushed)
+    if (t < 100 && v[t] >= 0.0f) {
+        int r = 0;
+        for (int j = 0; j < 100; ++j) {
+            const float x = v[j];
+            r += x >= 0.0f && (x < v[t] || (x == v[t] && j < t));
         }
+        sorted[r] = v[t];
     }
+    if (t == 0) {
+        int n = 0;
+        for (int j = 0; j < 100; ++j) n += v[j] >= 0.0f;
+        len_s = n;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    const int len = len_s;
     float r;
     if (sampler == GPRT_SAMPLER_LCG) {
         r = 10.0f;
         if (len > 0) {
-            if (heuristic == GPRT_RADIUS_MAX) r = v[len - 1] * 1.1f;
+            if (heuristic == GPRT_RADIUS_MAX) r = sorted[len - 1] * 1.1f;
             else if (heuristic == GPRT_RADIUS_PERCENTILE) {
                 float a = fminf((float)len * param, (float)(len - 1));
                 int idx = a > 0.0f ? (int)a : 0;
-                r = v[idx] * 1.1f;
+                r = sorted[idx] * 1.1f;
             } else r = param;
         }
     } else {
@@ -206,7 +218,7 @@ __global__ void sampler_finalize_kernel(const float* kth, int nclouds, int sampl
             float a = (float)len * param;
             int idx = a > 0.0f ? (int)a : 0;
             if (idx > len - 1) idx = len - 1;
-            r = v[idx] * 1.1f;
+            r = sorted[idx] * 1.1f;
         }
     }
     if (heuristic == GPRT_RADIUS_INFINITE) r = INFINITY;
@@ -267,79 +279,121 @@ __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 candidate list (ascending d32), capacity C, in registers
+// FP32 candidate list (ascending d32), capacity C, in registers.
+//
+// The list holds C = S + k + G slots: the first S = C - k - G are -inf
+// sentinels that never move, then the k nearest candidates, then G guard slots
+// for candidates inside the band of the k-th.  The k-th candidate therefore sits
+// at the fixed index C - 1 - G whatever k is, so no slot is ever addressed with
+// a runtime index (which would push the whole list to scratch memory).
 // ---------------------------------------------------------------------------
 template <int C>
 struct CandList {
     float d[C];
     int s[C];
-    __device__ __forceinline__ void init() {
+    __device__ __forceinline__ void init(int sentinels) {
 #pragma unroll
-        for (int j = 0; j < C; ++j) { d[j] = INFINITY; s[j] = -1; }
-    }
-    // insert (dd, ss); returns the value pushed out of the last slot (INF if
-    // that slot was empty).  Caller guarantees dd < d[C-1].
-    __device__ __forceinline__ float insert(float dd, int ss) {
-        float ev = d[C - 1];
-#pragma unroll
-        for (int j = C - 1; j >= 0; --j) {
-            float prev = j > 0 ? d[j - 1] : -INFINITY;
-            if (prev > dd) {
-                d[j] = prev;
-                s[j] = s[j > 0 ? j - 1 : 0];
-            } else if (d[j] > dd) {
-                d[j] = dd;
-                s[j] = ss;
-            }
+        for (int j = 0; j < C; ++j) {
+            d[j] = j < sentinels ? -INFINITY : INFINITY;
+            s[j] = -1;
         }
-        return ev;
     }
-    __device__ __forceinline__ float at(int k1) const {  // d[k1] with runtime k1
-        float v = d[0];
+    // insert (dd, ss), dd >= 0; returns the value pushed out of the last slot
+    // (INF if that slot was empty).  Caller guarantees dd < d[C-1].
+    __device__ __forceinline__ float insert(float dd, int ss) {
+        const float ev = d[C - 1];
 #pragma unroll
-        for (int j = 1; j < C; ++j) v = (j == k1) ? d[j] : v;
-        return v;
+        for (int j = C - 1; j >= 1; --j) {
+            const bool shift = d[j - 1] > dd;
+            const bool here = !shift && d[j] > dd;
+            const float nd = shift ? d[j - 1] : (here ? dd : d[j]);
+            const int ns = shift ? s[j - 1] : (here ? ss : s[j]);
+            d[j] = nd;
+            s[j] = ns;
+        }
+        if (d[0] > dd) { d[0] = dd; s[0] = ss; }
+        return ev;
     }
 };
 
-// Per-lane traversal state of the FP32 pass
-template <int C>
+template <int C, int G>
 struct F32Query {
     float qx, qy, qz;
     float T0;      // round bound (radius^2), INF in the unbounded round
     float T, tau;  // current k-th bound and guard threshold
     float ev_min;  // smallest d32 pushed out of a full list
-    int cnt, k;
     CandList<C> L;
 
+    __device__ __forceinline__ void start(float x, float y, float z, float t0, int k, bool active) {
+        qx = x; qy = y; qz = z;
+        T0 = t0;
+        T = t0;
+        tau = active ? band(t0) : -1.0f;  // inactive lanes accept nothing
+        ev_min = INFINITY;
+        L.init(C - k - G);
+    }
     __device__ __forceinline__ void visit_point(float4 p, int s) {
-        float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
+        const float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
         if (dd <= tau) {
             if (dd < L.d[C - 1]) {
-                float ev = L.insert(dd, s);
-                ev_min = fminf(ev_min, ev);
-                cnt = cnt < C ? cnt + 1 : C;
-                if (cnt >= k) {
-                    T = fminf(T0, L.at(k - 1));
-                    tau = band(T);
-                }
+                ev_min = fminf(ev_min, L.insert(dd, s));
+                T = fminf(T0, L.d[C - 1 - G]);
+                tau = band(T);
             } else {
                 ev_min = fminf(ev_min, dd);
             }
         }
     }
+    __device__ __forceinline__ float kth() const { return L.d[C - 1 - G]; }
 };
 
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// View of read-only device memory in the constant address space: wave-uniform
+// loads through it become scalar loads (s_load_dwordx16 for a 64-byte node).
+#define GPRT_AS4 __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const GPRT_AS4 T* const_view(const T* p) {
+    return (const GPRT_AS4 T*)(uintptr_t)p;
+}
+__device__ __forceinline__ float4 ld4(const GPRT_AS4 float4* p) {
+    const GPRT_AS4 float* f = (const GPRT_AS4 float*)p;
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+__device__ __forceinline__ Node ldnode(const GPRT_AS4 Node* p) {
+    const GPRT_AS4 float4* f = (const GPRT_AS4 float4*)p;
+    Node n;
+    n.alo = ld4(f);
+    n.ahi = ld4(f + 1);
+    n.blo = ld4(f + 2);
+    n.bhi = ld4(f + 3);
+    return n;
+}
+__device__ __forceinline__ int2 ld2i(const GPRT_AS4 int2* p) {
+    const GPRT_AS4 int* f = (const GPRT_AS4 int*)p;
+    return make_int2(f[0], f[1]);
+}
+
 // ---------------------------------------------------------------------------
-// one bounded (or the final unbounded) round, persistent over the active list
+// one bounded (or the final unbounded) round as a persistent launch.  Each
+// wavefront takes 64 consecutive queries of the active list (Morton order, so
+// they are spatial neighbours) and walks the tree as a packet: node and leaf
+// point loads are wave-uniform (scalar loads), every lane tests them against its
+// own query and its own threshold, and a child is visited when any lane needs
+// it.  The traversal stack is one uniform LDS stack per wavefront; the radix
+// tree depth bound keeps it from overflowing.
 // ---------------------------------------------------------------------------
-template <int C, int BLOCK, int STACK, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
-    __shared__ int2 stk[STACK][BLOCK];
-    const int tid = threadIdx.x, lane = tid & 63;
+constexpr int PACKET_STACK = EXACT_STACK;
+
+template <int C, int G, bool COUNT>
+__global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
+    __shared__ int stk[4][PACKET_STACK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
-    const Node* __restrict__ nodes = a.nodes;
-    const float4* __restrict__ pts = a.pts;
+    const GPRT_AS4 Node* nodes = const_view(a.nodes);      // uniform node loads
+    const GPRT_AS4 float4* upts = const_view(a.pts);       // uniform leaf-point loads
+    const GPRT_AS4 int2* roots = const_view(a.roots);
+    const float4* __restrict__ pts = a.pts;                // per-lane loads (re-rank)
     uint64_t visits = 0, tests = 0;
     while (true) {
         unsigned base = 0;
@@ -348,109 +402,112 @@ __global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
         if (base >= n_in) break;
         const unsigned i = base + lane;
         const bool valid = i < n_in;
-        int status = 0;  // 0 done, 1 unresolved, 2 fallback
-        int s = 0;
-        if (valid) {
-            s = a.in_list ? a.in_list[i] : (int)i;
-            const float4 q = a.qs[s];
-            const int c = a.qcloud ? a.qcloud[s] : 0;
-            const int k = a.kc ? a.kc[c] : a.k;
-            F32Query<C> Q;
-            Q.qx = q.x; Q.qy = q.y; Q.qz = q.z;
-            float R = a.last ? INFINITY : a.radius0[c];
-            if (!a.last)
-                for (int r = 0; r < a.round; ++r) R *= a.mult;
-            Q.T0 = a.last ? INFINITY : R * R;
-            Q.T = Q.T0;
-            Q.tau = band(Q.T);
-            Q.ev_min = INFINITY;
-            Q.cnt = 0;
-            Q.k = k;
-            Q.L.init();
-            bool overflow = false;
-            const int2 root = a.roots[c];
-            int ref = root.x;
+        const int s = valid ? (a.in_list ? a.in_list[i] : (int)i) : 0;
+        const float4 q = valid ? a.qs[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int c = valid && a.qcloud ? a.qcloud[s] : 0;
+        const int k = a.kc ? a.kc[c] : a.k;
+        float R = a.last ? INFINITY : a.radius0[c];
+        if (!a.last)
+            for (int r = 0; r < a.round; ++r) R *= a.mult;
+        const float T0 = a.last ? INFINITY : R * R;
+        F32Query<C, G> Q;
+        Q.start(q.x, q.y, q.z, T0, k, false);
+        bool overflow = false;
+        // lanes of one wave may belong to different clouds (at cloud borders of
+        // the sorted order): walk once per distinct cloud
+        uint64_t todo = __ballot(valid);
+        while (todo) {
+            const int c0 = uniform(__shfl(c, __ffsll((unsigned long long)todo) - 1));
+            const bool mine = valid && c == c0;
+            todo &= ~__ballot(mine);
+            if (mine) Q.tau = band(T0);
+            const int2 root = ld2i(roots + c0);
+            if (root.y > 0 && root.x < 0) {
+                const int f = uniform(~root.x);
+                for (int t = 0; t < root.y; ++t) Q.visit_point(ld4(upts + f + t), f + t);
+                if (COUNT) tests += root.y;
+            }
+            if (root.y <= 0 || root.x < 0) {
+                Q.tau = -1.0f;  // this cloud's lanes are done
+                continue;
+            }
             int sp = 0;
-            if (root.y > 0) {
-                if (ref < 0) {
-                    for (int t = 0; t < root.y; ++t) Q.visit_point(pts[~ref + t], ~ref + t);
-                    if (COUNT) tests += root.y;
+            int node = uniform(root.x);
+            int steps = 0;
+            while (true) {
+                if (++steps > a.max_steps) { overflow = true; break; }
+                const Node nd = ldnode(nodes + node);
+                if (COUNT) ++visits;
+                const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
+                const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
+                const int rA = uniform(__float_as_int(nd.alo.w)), rB = uniform(__float_as_int(nd.blo.w));
+                const bool needA = dA <= Q.tau, needB = dB <= Q.tau;
+                const bool anyA = __ballot(needA) != 0, anyB = __ballot(needB) != 0;
+                // lanes vote for the child nearer to them
+                const int votesB = __popcll(__ballot(needB && (!needA || dB < dA)));
+                const int votesA = __popcll(__ballot(needA && (!needB || dA <= dB)));
+                const bool bfirst = votesB > votesA;
+                // leaf buckets are scanned at once
+                if (rA < 0 && rB < 0 && bfirst) {
+                    if (anyB) {
+                        const int f = ~rB, m = uniform(__float_as_int(nd.bhi.w));
+                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
+                        if (COUNT) tests += m;
+                    }
+                    if (anyA) {
+                        const int f = ~rA, m = uniform(__float_as_int(nd.ahi.w));
+                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
+                        if (COUNT) tests += m;
+                    }
                 } else {
-                    // a tree path never repeats a node: more steps than nodes
-                    // means a corrupt tree; bail out to the exact kernel
-                    int steps = 0;
-                    while (true) {
-                        if (++steps > a.max_steps) { overflow = true; break; }
-                        const Node nd = nodes[ref];
-                        if (COUNT) ++visits;
-                        const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
-                        const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
-                        const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
-                        // leaf buckets are scanned at once, nearer first
-                        const bool lA = rA < 0, lB = rB < 0;
-                        if (lA && lB && dB < dA) {
-                            if (dB <= Q.tau) {
-                                int f = ~rB, m = __float_as_int(nd.bhi.w);
-                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
-                                if (COUNT) tests += m;
-                            }
-                            if (dA <= Q.tau) {
-                                int f = ~rA, m = __float_as_int(nd.ahi.w);
-                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
-                                if (COUNT) tests += m;
-                            }
-                        } else {
-                            if (lA && dA <= Q.tau) {
-                                int f = ~rA, m = __float_as_int(nd.ahi.w);
-                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
-                                if (COUNT) tests += m;
-                            }
-                            if (lB && dB <= Q.tau) {
-                                int f = ~rB, m = __float_as_int(nd.bhi.w);
-                                for (int t = 0; t < m; ++t) Q.visit_point(pts[f + t], f + t);
-                                if (COUNT) tests += m;
-                            }
-                        }
-                        const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
-                        if (gA && gB) {
-                            const bool nearA = dA <= dB;
-                            if (sp < STACK) {
-                                stk[sp][tid] = make_int2(nearA ? rB : rA, __float_as_int(nearA ? dB : dA));
-                                ++sp;
-                            } else {
-                                overflow = true;
-                            }
-                            ref = nearA ? rA : rB;
-                            continue;
-                        }
-                        if (gA) { ref = rA; continue; }
-                        if (gB) { ref = rB; continue; }
-                        bool found = false;
-                        while (sp > 0) {
-                            --sp;
-                            const int2 e = stk[sp][tid];
-                            if (__int_as_float(e.y) <= Q.tau) { ref = e.x; found = true; break; }
-                        }
-                        if (!found) break;
+                    if (rA < 0 && anyA) {
+                        const int f = ~rA, m = uniform(__float_as_int(nd.ahi.w));
+                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
+                        if (COUNT) tests += m;
+                    }
+                    if (rB < 0 && anyB) {
+                        const int f = ~rB, m = uniform(__float_as_int(nd.bhi.w));
+                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
+                        if (COUNT) tests += m;
                     }
                 }
+                const bool goA = rA >= 0 && anyA, goB = rB >= 0 && anyB;
+                if (goA && goB) {
+                    if (sp >= PACKET_STACK) { overflow = true; break; }
+                    if (lane == 0) stk[w][sp] = bfirst ? rA : rB;
+                    ++sp;
+                    node = bfirst ? rB : rA;
+                    continue;
+                }
+                if (goA) { node = rA; continue; }
+                if (goB) { node = rB; continue; }
+                if (sp == 0) break;
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                node = uniform(stk[w][sp]);
             }
-            // resolve
-            const float kth = Q.cnt >= k ? Q.L.at(k - 1) : INFINITY;
-            const bool resolved = a.last ? true : (Q.cnt >= k && kth <= Q.T0);
+            Q.tau = -1.0f;  // this cloud's lanes are done
+            if (overflow) break;
+        }
+        // resolve
+        int status = 0;  // 0 done, 1 unresolved, 2 fallback
+        if (valid) {
+            const float kth = Q.kth();
+            const bool full = kth < INFINITY;
+            const bool resolved = a.last ? true : (full && kth <= T0);
             if (!resolved) {
                 status = 1;
             } else {
-                const float tf = band(Q.cnt >= k ? kth : INFINITY);
+                const float tf = band(kth);  // INF when the cloud holds fewer than k points
                 if (overflow || Q.ev_min <= tf) {
                     status = 2;
                 } else {
-                    // FP64 re-rank of the guard band
+                    // FP64 re-rank of the guard band, (d64, id) order
                     double dd[C];
                     uint32_t id[C];
 #pragma unroll
                     for (int j = 0; j < C; ++j) {
-                        const bool in = j < Q.cnt && Q.L.d[j] <= tf;
+                        const bool in = Q.L.d[j] >= 0.0f && Q.L.d[j] <= tf && Q.L.s[j] >= 0;
                         dd[j] = INFINITY;
                         id[j] = GPRT_ID_NONE;
                         if (in) {
@@ -465,11 +522,10 @@ __global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
 #pragma unroll
                         for (int j = 0; j + 1 < C; ++j) {
                             const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
-                            if (gt) {
-                                double t = dd[j]; dd[j] = dd[j + 1]; dd[j + 1] = t;
-                                uint32_t u = id[j]; id[j] = id[j + 1]; id[j + 1] = u;
-                                swapped = true;
-                            }
+                            const double lo = gt ? dd[j + 1] : dd[j], hi = gt ? dd[j] : dd[j + 1];
+                            const uint32_t li = gt ? id[j + 1] : id[j], hiid = gt ? id[j] : id[j + 1];
+                            dd[j] = lo; dd[j + 1] = hi; id[j] = li; id[j + 1] = hiid;
+                            swapped |= gt;
                         }
                     }
                     const int g = __float_as_int(q.w);
@@ -480,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
                             a.ids_out[ob + j] = id[j];
                             if (a.d2_out) a.d2_out[ob + j] = dd[j];
                         }
-                    if (Q.cnt < k) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
+                    if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
                 }
             }
         }
@@ -490,10 +546,7 @@ __global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
         if (fslot >= 0) a.fb_list[fslot] = s;
     }
     if (COUNT) {
-        for (int o = 32; o > 0; o >>= 1) {
-            visits += __shfl_xor(visits, o);
-            tests += __shfl_xor(tests, o);
-        }
+        // visits/tests are wave-uniform in the packet walk: count once per wave
         if (lane == 0) {
             atomicAdd(&a.stats64[0], (unsigned long long)visits);
             atomicAdd(&a.stats64[1], (unsigned long long)tests);
@@ -504,32 +557,31 @@ __global__ __launch_bounds__(BLOCK) void knn_round_kernel(RoundArgs a) {
 // ---------------------------------------------------------------------------
 // exact FP64 traversal (plain single-launch form and band-overflow fallback)
 // ---------------------------------------------------------------------------
+// Exact (d64, id) list: the first KMAX - k slots hold (-inf, 0) sentinels that
+// never move, so the k-th entry is always the last slot.
 template <int KMAX>
 struct ExactList {
     double d[KMAX];
     uint32_t id[KMAX];
-    __device__ __forceinline__ void init() {
+    __device__ __forceinline__ void init(int k) {
 #pragma unroll
-        for (int j = 0; j < KMAX; ++j) { d[j] = INFINITY; id[j] = GPRT_ID_NONE; }
-    }
-    // insert into the first k slots, ascending (d, id)
-    __device__ __forceinline__ void insert(double dd, uint32_t ii, int k) {
-#pragma unroll
-        for (int j = KMAX - 1; j >= 0; --j) {
-            if (j >= k) continue;
-            const double pd = j > 0 ? d[j - 1] : -INFINITY;
-            const uint32_t pi = j > 0 ? id[j - 1] : 0u;
-            const bool prev_gt = pd > dd || (pd == dd && pi > ii);
-            const bool cur_gt = d[j] > dd || (d[j] == dd && id[j] > ii);
-            if (prev_gt) { d[j] = pd; id[j] = pi; }
-            else if (cur_gt) { d[j] = dd; id[j] = ii; }
+        for (int j = 0; j < KMAX; ++j) {
+            d[j] = j < KMAX - k ? -INFINITY : INFINITY;
+            id[j] = j < KMAX - k ? 0u : GPRT_ID_NONE;
         }
     }
-    __device__ __forceinline__ void worst(int k, double& wd, uint32_t& wi) const {
-        wd = d[0]; wi = id[0];
+    // insert (dd, ii) which is known to beat the last slot
+    __device__ __forceinline__ void insert(double dd, uint32_t ii) {
 #pragma unroll
-        for (int j = 1; j < KMAX; ++j)
-            if (j == k - 1) { wd = d[j]; wi = id[j]; }
+        for (int j = KMAX - 1; j >= 1; --j) {
+            const bool shift = d[j - 1] > dd || (d[j - 1] == dd && id[j - 1] > ii);
+            const bool here = !shift && (d[j] > dd || (d[j] == dd && id[j] > ii));
+            const double nd = shift ? d[j - 1] : (here ? dd : d[j]);
+            const uint32_t ni = shift ? id[j - 1] : (here ? ii : id[j]);
+            d[j] = nd;
+            id[j] = ni;
+        }
+        if (d[0] > dd || (d[0] == dd && id[0] > ii)) { d[0] = dd; id[0] = ii; }
     }
 };
 
@@ -547,20 +599,16 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
     int stk_ref[EXACT_STACK];
     float stk_d[EXACT_STACK];
     int sp = 0;
-    double wd = INFINITY;
-    uint32_t wi = GPRT_ID_NONE;
     auto scan = [&](int f, int m) {
         for (int t = 0; t < m; ++t) {
             const float4 p = pts[f + t];
             const double dd = d2d(qx, qy, qz, p.x, p.y, p.z);
             const uint32_t ii = (uint32_t)__float_as_int(p.w);
-            if (dd < wd || (dd == wd && ii < wi)) {
-                L.insert(dd, ii, k);
-                L.worst(k, wd, wi);
-            }
+            if (dd < L.d[KMAX - 1] || (dd == L.d[KMAX - 1] && ii < L.id[KMAX - 1])) L.insert(dd, ii);
         }
         tests += m;
     };
+    const double& wd = L.d[KMAX - 1];
     if (root.y <= 0) return true;
     int ref = root.x;
     if (ref < 0) { scan(~ref, root.y); return true; }
@@ -624,16 +672,16 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
     }
     const int k = a.kc ? a.kc[c] : a.k;
     ExactList<KMAX> L;
-    L.init();
+    L.init(k);
     uint64_t visits = 0, tests = 0;
     if (!exact_query<KMAX>(a.nodes, a.pts, a.roots[c], qx, qy, qz, k, L, visits, tests, a.max_steps))
         atomicAdd(&a.counters[CNT_ERROR], 1u);
     const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
-        if (j < k) {
-            a.ids_out[ob + j] = L.id[j];
-            if (a.d2_out) a.d2_out[ob + j] = L.d[j];
+        if (j >= KMAX - k) {
+            a.ids_out[ob + j - (KMAX - k)] = L.id[j];
+            if (a.d2_out) a.d2_out[ob + j - (KMAX - k)] = L.d[j];
         }
     if (COUNT) {
         atomicAdd(&a.stats64[0], (unsigned long long)visits);
@@ -659,8 +707,8 @@ int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_
         hipLaunchKernelGGL(fill_u32_kernel, dim3((100 * nclouds + 255) / 256), dim3(256), 0, s, (unsigned*)kth,
                            100 * nclouds, 0xBF800000u /* -1.0f */);
     }
-    hipLaunchKernelGGL(sampler_finalize_kernel, dim3((nclouds + 63) / 64), dim3(64), 0, s, kth, nclouds, sampler,
-                       heuristic, param, radius);
+    hipLaunchKernelGGL(sampler_finalize_kernel, dim3(nclouds), dim3(128), 0, s, kth, nclouds, sampler, heuristic,
+                       param, radius);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "sampler launch failed");
 }
 
@@ -673,7 +721,7 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
                        perm);
     int cloud_bits = 0;
     while ((1 << cloud_bits) < nclouds) ++cloud_bits;
-    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 0, 30));
+    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 6, 30));
     if (cloud_bits > 0) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 32, 32 + cloud_bits));
     hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, nq, qs4, counters);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
@@ -694,35 +742,32 @@ int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
     return GPRT_OK;
 }
 
-template <int C, int STACK>
-static void launch_round_t(hipStream_t s, const RoundArgs& a, int grid, bool count) {
-    if (count)
-        hipLaunchKernelGGL((knn_round_kernel<C, 256, STACK, true>), dim3(grid), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((knn_round_kernel<C, 256, STACK, false>), dim3(grid), dim3(256), 0, s, a);
-}
-
-template <int C>
-static int launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int stack, bool count) {
-    if (stack <= 16) launch_round_t<C, 16>(s, a, grid, count);
-    else launch_round_t<C, 32>(s, a, grid, count);
-    return GPRT_OK;
-}
+constexpr int GUARD = 3;
 
 int capacity_class(int kmax, int guard) {
-    int need = kmax + guard;
+    (void)guard;
+    int need = kmax + GUARD;
     if (need <= 8) return 8;
     if (need <= 16) return 16;
     if (need <= 32) return 32;
     return 64;
 }
 
+template <int C>
+static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, bool count) {
+    if (count)
+        hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
 int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int stack, bool count) {
+    (void)stack;
     switch (cap) {
-        case 8: launch_round_c<8>(s, a, grid, stack, count); break;
-        case 16: launch_round_c<16>(s, a, grid, stack, count); break;
-        case 32: launch_round_c<32>(s, a, grid, stack, count); break;
-        default: launch_round_c<64>(s, a, grid, stack, count); break;
+        case 8: launch_round_c<8>(s, a, grid, count); break;
+        case 16: launch_round_c<16>(s, a, grid, count); break;
+        case 32: launch_round_c<32>(s, a, grid, count); break;
+        default: launch_round_c<64>(s, a, grid, count); break;
     }
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }

@@ -126,8 +126,9 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
-// hist layout: digit-major [RS_BINS][nblocks] so one exclusive scan yields the
-// output offset of (digit, tile).
+// hist layout: tile-major [nblocks][RS_BINS] (each tile writes 1 KB
+// contiguously); rs_offsets_kernel turns it into the output offset of every
+// (tile, digit) pair.
 template <typename K>
 __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict__ keys, int n, int shift,
                                                              uint32_t* __restrict__ hist, int nblocks) {
@@ -141,30 +142,51 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict
         if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 0xFFu], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[blockIdx.x * RS_BINS + threadIdx.x] = h[threadIdx.x];
 }
 
-// single-block exclusive scan of m uint32 values, in place
-__global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t* data, int m) {
-    __shared__ uint32_t sums[1024];
-    int t = threadIdx.x;
-    int per = (m + 1023) / 1024;
-    int b = t * per, e = min(b + per, m);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += data[i];
-    sums[t] = s;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 sums
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = t >= o ? sums[t - o] : 0;
-        __syncthreads();
-        sums[t] += v;
-        __syncthreads();
+// One block of 1024 threads = 4 groups x 256 digits.  Group g scans tiles
+// [g*nb/4, (g+1)*nb/4) per digit; the groups and the digit totals are then
+// combined so hist[b][d] = (keys with digit < d) + (keys with digit d in tiles < b).
+__global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nblocks) {
+    __shared__ uint32_t gsum[4][RS_BINS];
+    __shared__ uint32_t dtot[RS_BINS];
+    const int d = threadIdx.x & 255, g = threadIdx.x >> 8;
+    const int per = (nblocks + 3) / 4;
+    const int b0 = min(g * per, nblocks), b1 = min(b0 + per, nblocks);
+    uint32_t run = 0;
+    int b = b0;
+    for (; b + 4 <= b1; b += 4) {
+        uint32_t v0 = hist[(b + 0) * RS_BINS + d], v1 = hist[(b + 1) * RS_BINS + d];
+        uint32_t v2 = hist[(b + 2) * RS_BINS + d], v3 = hist[(b + 3) * RS_BINS + d];
+        run += v0 + v1 + v2 + v3;
     }
-    uint32_t run = t > 0 ? sums[t - 1] : 0;
-    for (int i = b; i < e; ++i) {
-        uint32_t v = data[i];
-        data[i] = run;
+    for (; b < b1; ++b) run += hist[b * RS_BINS + d];
+    gsum[g][d] = run;
+    __syncthreads();
+    if (g == 0) {
+        uint32_t t = 0;
+        for (int gg = 0; gg < 4; ++gg) {
+            uint32_t v = gsum[gg][d];
+            gsum[gg][d] = t;
+            t += v;
+        }
+        dtot[d] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int dd = 0; dd < RS_BINS; ++dd) {
+            uint32_t v = dtot[dd];
+            dtot[dd] = t;
+            t += v;
+        }
+    }
+    __syncthreads();
+    run = dtot[d] + gsum[g][d];
+    for (b = b0; b < b1; ++b) {
+        uint32_t v = hist[b * RS_BINS + d];
+        hist[b * RS_BINS + d] = run;
         run += v;
     }
 }
@@ -179,7 +201,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
     __shared__ uint32_t boff[RS_BINS];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     for (int i = t; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    boff[t] = hist[t * nblocks + blockIdx.x];
+    boff[t] = hist[blockIdx.x * RS_BINS + t];
     __syncthreads();
 
     const int base = blockIdx.x * RS_TILE + w * (64 * RS_ITEMS);
@@ -246,7 +268,7 @@ static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_b
     for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
         hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, n, shift, hist,
                            nblocks);
-        hipLaunchKernelGGL(scan_exclusive_kernel, dim3(1), dim3(1024), 0, ctx->stream, hist, RS_BINS * nblocks);
+        hipLaunchKernelGGL(rs_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, hist, nblocks);
         hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, v0, k1, v1,
                            n, shift, hist, nblocks);
         std::swap(k0, k1);
