@@ -100,6 +100,7 @@ typedef struct {
     int grid_blocks;      /* persistent grid size, 0 = automatic */
     int block_threads;    /* 0 = automatic (256) */
     int collect_counters; /* count node visits / point tests (slower; for the roofline) */
+    int time_kernels;     /* with use_graph == 0: time the first traversal round on its own */
 } gprt_hip_config;
 
 typedef struct {
@@ -114,6 +115,8 @@ typedef struct {
     double ms_build;           /* last tree build */
     int64_t node_visits;       /* only with counters enabled */
     int64_t point_tests;
+    int64_t rerank_reads;      /* FP64 re-rank point reads (counters enabled) */
+    double ms_kernel;          /* first traversal round alone (time_kernels) */
 } gprt_hip_run_stats;
 
 typedef struct {

@@ -13,13 +13,6 @@
 #include "gprt_internal.hpp"
 #include "knn_internal.hpp"
 
-struct gprt_hip_ctx {
-    gprt::Ctx c;
-};
-struct gprt_hip_tree {
-    gprt::Tree t;
-    gprt_hip_ctx* owner = nullptr;
-};
 
 namespace gprt {
 
@@ -70,6 +63,11 @@ enum {
 // extra per-context buffers beyond the 8 generic slots
 struct CtxExtra {
     DevBuf q_in, ids_out, d2_out, qcloud, kc, qoff, outbase, flag;
+    // range query
+    DevBuf r_counts, r_d2, r_big, r_keys, r_perm, r_qs4, r_sums, r_off;
+    // Barnes-Hut
+    DevBuf bh_xyz, bh_mass, bh_acc;
+    Tree bh_tree;
 };
 static std::mutex g_extra_mu;
 static std::vector<std::pair<Ctx*, CtxExtra*>> g_extra;
@@ -86,7 +84,13 @@ static void drop_extra(Ctx* c) {
     for (size_t i = 0; i < g_extra.size(); ++i)
         if (g_extra[i].first == c) {
             CtxExtra* e = g_extra[i].second;
-            for (DevBuf* b : {&e->q_in, &e->ids_out, &e->d2_out, &e->qcloud, &e->kc, &e->qoff, &e->outbase, &e->flag})
+            for (DevBuf* b : {&e->q_in, &e->ids_out, &e->d2_out, &e->qcloud, &e->kc, &e->qoff, &e->outbase, &e->flag,
+                              &e->r_counts, &e->r_d2, &e->r_big, &e->r_keys, &e->r_perm, &e->r_qs4, &e->r_sums,
+                              &e->r_off, &e->bh_xyz, &e->bh_mass, &e->bh_acc})
+                b->release();
+            Tree& T = e->bh_tree;
+            for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.perm, &T.inv, &T.codes,
+                              &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
                 b->release();
             delete e;
             g_extra.erase(g_extra.begin() + i);
@@ -118,6 +122,13 @@ __global__ void set_qoff_kernel(int64_t* qoff, int64_t nq) {
     qoff[1] = nq;
 }
 
+int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
+              int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
+              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr);
+int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps, float G, float* d_acc,
+           DevBuf& mass_buf, bool count);
+int bh_prepare(Ctx* ctx, const float4* d_pm, int n, float* d_xyz, unsigned* flag);
+
 }  // namespace gprt
 
 using namespace gprt;
@@ -144,6 +155,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->grid_blocks = 0;
     c->block_threads = 0;
     c->collect_counters = 0;
+    c->time_kernels = 0;
 }
 
 extern "C" int gprt_hip_num_devices(int* out) {
@@ -307,7 +319,7 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
     if (t->owner) (void)hipSetDevice(t->owner->c.device);
     Tree& T = t->t;
     for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.perm, &T.inv, &T.codes,
-                      &T.parent, &T.flags, &T.bh})
+                      &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
         b->release();
     delete t;
     return GPRT_OK;
@@ -389,7 +401,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     float* radius = kth + 100 * p.nclouds;
     int nq = (int)p.nq;
     CtxExtra* x = extra(&c);
-    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 4, 0u));
+    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
     if (p.nclouds == 1) hipLaunchKernelGGL(set_qoff_kernel, dim3(1), dim3(1), 0, s, (int64_t*)p.d_qoff, p.nq);
     launch_validate(s, p.d_q, 3 * p.nq, counters + CNT_TOTAL - 1);
     GPRT_TRY(launch_sampler(s, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(), T.cloud_off.as<int64_t>(),
@@ -426,13 +438,17 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.out_base = p.d_outbase;
     ra.qoff = p.d_qoff;
     ra.max_steps = T.n + 64;
+    const bool time_k = cfg.time_kernels && !cfg.use_graph;
     for (int r = 0; r < R; ++r) {
         ra.round = r;
         ra.last = (r == R - 1);
         ra.in_list = r == 0 ? nullptr : lists[r & 1];
         ra.out_list = lists[(r + 1) & 1];
-        GPRT_TRY(launch_round(s, ra, cap, grid, cfg.stack_depth, cfg.collect_counters != 0));
+        if (r == 0 && time_k) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
+        GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.collect_counters != 0));
+        if (r == 0 && time_k) GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
     }
+    c.kernel_timed = time_k;
     ExactArgs ea{};
     ea.mode = 1;
     ea.nq = p.nq;
@@ -516,10 +532,16 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters) {
     c.stats.ms_total = ms;
     c.stats.queries = nq;
     if (read_counters) {
-        unsigned h[CNT_TOTAL + 4];
+        unsigned h[CNT_TOTAL + 2 * CNT_STATS64];
         GPRT_HIP_TRY(hipMemcpy(h, c.counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
-        unsigned long long v64[2];
+        unsigned long long v64[CNT_STATS64];
         memcpy(v64, h + CNT_TOTAL, sizeof(v64));
+        c.stats.rerank_reads = (int64_t)v64[2];
+        if (c.kernel_timed) {
+            float mk = 0;
+            if (hipEventElapsedTime(&mk, c.ev2, c.ev3) == hipSuccess) c.stats.ms_kernel = mk;
+            c.kernel_timed = false;
+        }
         c.stats.rounds_run = 0;
         for (int r = 0; r < 16; ++r) {
             c.stats.active_per_round[r] = h[CNT_ACTIVE + r];
@@ -634,7 +656,7 @@ extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float
     ea.d2_out = d_d2;
     ea.max_steps = T.n + 64;
     bool count = (flags & GPRT_COLLECT_COUNTERS) != 0;
-    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 4, 0u));
+    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
     launch_validate(s, d_q, 3 * nq, counters + CNT_TOTAL - 1);
     GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
     GPRT_TRY(launch_exact(s, ea, k, (int)nq, count));
@@ -840,4 +862,127 @@ extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* va
     kb.release();
     vb.release();
     return rc;
+}
+
+// ===========================================================================
+// fixed-radius range query
+// ===========================================================================
+extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, float radius,
+                              int64_t* offsets_out, uint32_t* ids_out, int64_t ids_capacity, int flags) {
+    if (!h || !t || !offsets_out) return fail(GPRT_ERR_ARG, "null ctx/tree/offsets_out");
+    if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
+    if (nq > 0 && !queries) return fail(GPRT_ERR_ARG, "null queries");
+    if (!(radius >= 0.0f && radius <= 1e18f)) return fail(GPRT_ERR_ARG, "radius must be finite and >= 0");
+    if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "range query takes a single-cloud tree");
+    if (ids_out && ids_capacity < 0) return fail(GPRT_ERR_ARG, "negative capacity");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    CtxExtra* x = extra(&c);
+    hipStream_t s = c.stream;
+    const bool dev = flags & GPRT_PTR_DEVICE;
+    c.stats = gprt_hip_run_stats{};
+    if (nq == 0) {
+        int64_t z = 0;
+        if (dev) GPRT_HIP_TRY(hipMemcpy(offsets_out, &z, 8, hipMemcpyHostToDevice));
+        else offsets_out[0] = 0;
+        return GPRT_OK;
+    }
+    const float* d_q = queries;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(12ull * nq));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, queries, 12ull * nq, hipMemcpyHostToDevice, s));
+        d_q = x->q_in.as<float>();
+    }
+    int64_t* d_off = offsets_out;
+    if (!dev) {
+        GPRT_TRY(x->r_off.reserve(8ull * (nq + 1)));
+        d_off = x->r_off.as<int64_t>();
+    }
+    GPRT_TRY(x->flag.reserve(16));
+    GPRT_HIP_TRY(hipMemsetAsync(x->flag.ptr, 0, 16, s));
+    launch_validate(s, d_q, 3 * nq, x->flag.as<unsigned>());
+    unsigned bad = 0;
+    GPRT_HIP_TRY(hipMemcpyAsync(&bad, x->flag.ptr, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return fail(GPRT_ERR_INPUT, "queries contain NaN/Inf or |coordinate| > 1e18");
+    const Tree& T = t->t;
+    int64_t total = 0;
+    if (!ids_out) {
+        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, nullptr, 0, x->r_counts, x->r_d2, x->r_big, x->r_keys,
+                           x->r_perm, x->r_qs4, x->r_sums, &total));
+        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+        if (!dev) GPRT_HIP_TRY(hipMemcpyAsync(offsets_out, d_off, 8ull * (nq + 1), hipMemcpyDeviceToHost, s));
+    } else {
+        if (!dev) {
+            total = offsets_out[nq];
+            GPRT_HIP_TRY(hipMemcpyAsync(d_off, offsets_out, 8ull * (nq + 1), hipMemcpyHostToDevice, s));
+        } else {
+            GPRT_HIP_TRY(hipMemcpyAsync(&total, d_off + nq, 8, hipMemcpyDeviceToHost, s));
+            GPRT_HIP_TRY(hipStreamSynchronize(s));
+        }
+        if (total > ids_capacity) return fail(GPRT_ERR_CAPACITY, "ids_capacity is smaller than offsets[nq]");
+        uint32_t* d_ids = ids_out;
+        if (!dev) {
+            GPRT_TRY(x->ids_out.reserve(4ull * std::max<int64_t>(total, 1)));
+            d_ids = x->ids_out.as<uint32_t>();
+        }
+        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, d_ids, total, x->r_counts, x->r_d2, x->r_big,
+                           x->r_keys, x->r_perm, x->r_qs4, x->r_sums, nullptr));
+        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+        if (!dev && total) GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * total, hipMemcpyDeviceToHost, s));
+    }
+    return finish_stats(c, nq, true);
+}
+
+// ===========================================================================
+// Barnes-Hut
+// ===========================================================================
+extern "C" int gprt_hip_bh_forces(gprt_hip_ctx* h, const float* pos_mass, int64_t n, float theta, float eps, float G,
+                                  int leaf_size, float* acc_out, int flags) {
+    if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    if (n < 0 || n > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad n");
+    if (n > 0 && (!pos_mass || !acc_out)) return fail(GPRT_ERR_ARG, "null pos_mass/acc_out");
+    if (!(theta >= 0.0f && theta <= 1e6f) || !(eps >= 0.0f && eps <= 1e18f) || !(fabsf(G) <= 1e30f))
+        return fail(GPRT_ERR_ARG, "theta, eps and G must be finite (theta, eps >= 0)");
+    if (leaf_size < 1 || leaf_size > 32) return fail(GPRT_ERR_ARG, "leaf_size must be in [1, 32]");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    CtxExtra* x = extra(&c);
+    hipStream_t s = c.stream;
+    const bool dev = flags & GPRT_PTR_DEVICE;
+    c.stats = gprt_hip_run_stats{};
+    if (n == 0) return GPRT_OK;
+    const float4* d_pm = (const float4*)pos_mass;
+    if (!dev) {
+        GPRT_TRY(x->q_in.reserve(16ull * n));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, pos_mass, 16ull * n, hipMemcpyHostToDevice, s));
+        d_pm = x->q_in.as<float4>();
+    }
+    float* d_acc = acc_out;
+    if (!dev) {
+        GPRT_TRY(x->bh_acc.reserve(12ull * n));
+        d_acc = x->bh_acc.as<float>();
+    }
+    GPRT_TRY(x->bh_xyz.reserve(12ull * n));
+    GPRT_TRY(x->flag.reserve(16));
+    GPRT_HIP_TRY(hipMemsetAsync(x->flag.ptr, 0, 16, s));
+    GPRT_TRY(bh_prepare(&c, d_pm, (int)n, x->bh_xyz.as<float>(), x->flag.as<unsigned>()));
+    launch_validate(s, x->bh_xyz.as<float>(), 3 * n, x->flag.as<unsigned>());
+    unsigned bad = 0;
+    GPRT_HIP_TRY(hipMemcpyAsync(&bad, x->flag.ptr, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return fail(GPRT_ERR_INPUT, "bodies contain NaN/Inf, |coordinate| > 1e18 or a negative mass");
+    int64_t off[2] = {0, n};
+    GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
+    GPRT_TRY(tree_build(&c, x->bh_xyz.as<float>(), off, 1, leaf_size, &x->bh_tree));
+    GPRT_TRY(bh_run(&c, x->bh_tree, d_pm, (int)n, theta, eps, G, d_acc, x->bh_mass, (flags & GPRT_COLLECT_COUNTERS) != 0));
+    GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
+    if (!dev) GPRT_HIP_TRY(hipMemcpyAsync(acc_out, d_acc, 12ull * n, hipMemcpyDeviceToHost, s));
+    GPRT_TRY(finish_stats(c, n, true));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.ev2, c.ev3) == hipSuccess) c.stats.ms_total = ms;
+    c.stats.ms_build = c.stats.ms_total - c.stats.ms_query;
+    return GPRT_OK;
 }

@@ -1,0 +1,166 @@
+// autotune.hip -- picks the batched k-NN schedule for a data set.
+//
+// Replaces the reference's hill-climb macro gprt_autotune!
+// (gprt-macros/src/lib.rs:120-214: perturb percentile and multiplier, keep a
+// change when every query got k ids and the run was faster) and the funnel
+// profiler (gprt-profiler/src/main.rs:37-133: grid over percentile x multiplier
+// x cap on growing subsets, saturation gate).  Here the knobs are the ones that
+// matter on the GPU -- traversal mapping, leaf size, initial-radius rule
+// (percentile / unbounded) and growth multiplier -- and the gate is stronger:
+// every candidate's ids must equal those of the reference schedule bit for bit.
+// Timing uses hipEvents around the query phase (median of the timed repeats).
+// The result is written as a JSON superset of the reference's Schedule /
+// wisdom format (gprt-ir/src/lib.rs:58-89).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gprt_internal.hpp"
+
+using namespace gprt;
+
+namespace {
+
+struct Cand {
+    gprt_hip_config cfg;
+    int leaf;
+};
+
+int time_config(gprt_hip_ctx* h, gprt_hip_tree* t, const float* d_q, int64_t nq, int k, const gprt_hip_config& cfg,
+                uint32_t* d_ids, double* ms_out) {
+    std::vector<double> ts;
+    for (int r = 0; r < 4; ++r) {
+        GPRT_TRY(gprt_hip_knn(h, t, d_q, nq, k, &cfg, d_ids, nullptr, GPRT_PTR_DEVICE));
+        gprt_hip_run_stats st;
+        gprt_hip_stats(h, &st);
+        if (r > 0) ts.push_back(st.ms_query);
+    }
+    std::sort(ts.begin(), ts.end());
+    *ms_out = ts[ts.size() / 2];
+    return GPRT_OK;
+}
+
+std::string schedule_json(const gprt_hip_config& c, int leaf, double ms) {
+    char heur[96];
+    if (c.radius_heuristic == GPRT_RADIUS_PERCENTILE) snprintf(heur, sizeof heur, "{\"SampledPercentile\": %.6g}", c.radius_param);
+    else if (c.radius_heuristic == GPRT_RADIUS_FIXED) snprintf(heur, sizeof heur, "{\"Fixed\": %.9g}", c.radius_param);
+    else snprintf(heur, sizeof heur, "\"SampledMax\"");
+    char buf[1024];
+    snprintf(buf, sizeof buf,
+             "{\"radius_increment_mult\": %.6g, \"max_hits_per_query\": 2000, \"use_morton_lbv\": %s, "
+             "\"radius_heuristic\": %s, \"memory_strategy\": \"PayloadRegisterHeap\", "
+             "\"build_strategy\": \"PreferFastTrace\", \"geom_type\": \"Spheres\", "
+             "\"gprt_hip\": {\"sampler\": %d, \"radius_heuristic\": %d, \"radius_param\": %.6g, "
+             "\"radius_mult\": %.6g, \"max_rounds\": %d, \"traversal\": %d, \"stack_depth\": %d, "
+             "\"guard_slots\": %d, \"sort_queries\": %d, \"use_graph\": %d, \"grid_blocks\": %d, "
+             "\"leaf_size\": %d, \"ms_query\": %.6g}}\n",
+             c.radius_mult, c.sort_queries ? "true" : "false", heur, c.sampler, c.radius_heuristic, c.radius_param,
+             c.radius_mult, c.max_rounds, c.traversal, c.stack_depth, c.guard_slots, c.sort_queries, c.use_graph,
+             c.grid_blocks, leaf, ms);
+    return buf;
+}
+
+}  // namespace
+
+extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
+                                 int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
+                                 const char* json_path, int flags) {
+    if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
+    if (n <= 0 || nq <= 0 || k < 1 || k > 61) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 61");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    DevBuf dq, dids, dref;
+    const float* d_q = queries;
+    if (!(flags & GPRT_PTR_DEVICE)) {
+        GPRT_TRY(dq.reserve(12ull * nq));
+        GPRT_HIP_TRY(hipMemcpy(dq.ptr, queries, 12ull * nq, hipMemcpyHostToDevice));
+        d_q = dq.as<float>();
+    }
+    GPRT_TRY(dids.reserve(4ull * nq * k));
+    GPRT_TRY(dref.reserve(4ull * nq * k));
+    const int leaves[3] = {4, 8, 16};
+    gprt_hip_tree* trees[3] = {nullptr, nullptr, nullptr};
+    int rc = GPRT_OK;
+    for (int i = 0; i < 3 && rc == GPRT_OK; ++i) rc = gprt_hip_tree_build(h, pts, n, leaves[i], flags & GPRT_PTR_DEVICE, &trees[i]);
+    // reference answer: default schedule on the leaf-8 tree
+    gprt_hip_config base;
+    gprt_hip_config_default(&base);
+    if (rc == GPRT_OK) rc = gprt_hip_knn(h, trees[1], d_q, nq, k, &base, dref.as<uint32_t>(), nullptr, GPRT_PTR_DEVICE);
+    std::vector<uint32_t> ref, got;
+    if (rc == GPRT_OK) {
+        ref.resize((size_t)nq * k);
+        if (hipMemcpy(ref.data(), dref.ptr, 4ull * nq * k, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(GPRT_ERR_HIP, "copy failed");
+    }
+    // coordinate descent from the default, one knob at a time (like the
+    // reference's hill climb), bounded by budget_trials timed candidates
+    Cand best{base, 8};
+    double best_ms = 1e30;
+    int trials = 0;
+    auto tree_of = [&](int leaf) { return trees[leaf == 4 ? 0 : (leaf == 8 ? 1 : 2)]; };
+    auto try_cand = [&](const Cand& cd) -> int {
+        if (trials >= std::max(budget_trials, 1)) return GPRT_OK;
+        ++trials;
+        double ms = 0;
+        GPRT_TRY(time_config(h, tree_of(cd.leaf), d_q, nq, k, cd.cfg, dids.as<uint32_t>(), &ms));
+        got.resize((size_t)nq * k);
+        GPRT_HIP_TRY(hipMemcpy(got.data(), dids.ptr, 4ull * nq * k, hipMemcpyDeviceToHost));
+        if (got != ref) return fail(GPRT_ERR_STATE, "autotune: a candidate schedule changed the result");
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = cd;
+        }
+        return GPRT_OK;
+    };
+    if (rc == GPRT_OK) rc = try_cand(best);
+    for (int pass = 0; pass < 2 && rc == GPRT_OK; ++pass) {
+        const Cand start = best;
+        for (int leaf : leaves) {
+            if (leaf == start.leaf || rc != GPRT_OK) continue;
+            Cand cd = best;
+            cd.leaf = leaf;
+            rc = try_cand(cd);
+        }
+        for (int trav : {GPRT_TRAV_THREAD, GPRT_TRAV_WAVE}) {
+            if (trav == best.cfg.traversal || rc != GPRT_OK) continue;
+            Cand cd = best;
+            cd.cfg.traversal = trav;
+            rc = try_cand(cd);
+        }
+        for (float p : {0.5f, 0.8f, 0.95f, 0.99f}) {
+            if (rc != GPRT_OK) break;
+            Cand cd = best;
+            cd.cfg.radius_heuristic = GPRT_RADIUS_PERCENTILE;
+            cd.cfg.radius_param = p;
+            rc = try_cand(cd);
+        }
+        if (rc == GPRT_OK) {
+            Cand cd = best;
+            cd.cfg.radius_heuristic = GPRT_RADIUS_INFINITE;
+            rc = try_cand(cd);
+        }
+        for (float m : {1.5f, 2.0f, 3.0f, 5.0f}) {
+            if (rc != GPRT_OK || best.cfg.radius_heuristic == GPRT_RADIUS_INFINITE) break;
+            Cand cd = best;
+            cd.cfg.radius_mult = m;
+            rc = try_cand(cd);
+        }
+    }
+    for (auto* t : trees) gprt_hip_tree_destroy(t);
+    dq.release();
+    dids.release();
+    dref.release();
+    if (rc != GPRT_OK) return rc;
+    *cfg_out = best.cfg;
+    *leaf_size_out = best.leaf;
+    if (json_path) {
+        FILE* f = fopen(json_path, "w");
+        if (!f) return fail(GPRT_ERR_ARG, std::string("cannot write ") + json_path);
+        const std::string js = schedule_json(best.cfg, best.leaf, best_ms);
+        fputs(js.c_str(), f);
+        fclose(f);
+    }
+    return GPRT_OK;
+}

@@ -80,6 +80,8 @@ struct Tree {
     DevBuf codes;       // uint64[n]: sorted keys (cloud << 32 | morton)
     DevBuf parent;      // int[n-1 + n]: parent of internal nodes, then of points
     DevBuf flags;       // int[n-1]: refit arrival counters
+    DevBuf ranges;      // int2[n-1]: sorted-point range [lo, hi] of each internal node
+    DevBuf ready;       // int[n]: refit work list (+ its counter at the end)
     DevBuf bh;          // Barnes-Hut per-node data (allocated on demand)
     std::vector<int64_t> h_cloud_off;
     float h_bbox[6] = {0, 0, 0, 0, 0, 0};
@@ -165,6 +167,16 @@ struct Ctx {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t graph_key = 0;
+    bool kernel_timed = false;
 };
 
 }  // namespace gprt
+
+// the opaque handles of the C ABI
+struct gprt_hip_ctx {
+    gprt::Ctx c;
+};
+struct gprt_hip_tree {
+    gprt::Tree t;
+    gprt_hip_ctx* owner = nullptr;
+};

@@ -374,6 +374,15 @@ __device__ __forceinline__ int2 ld2i(const GPRT_AS4 int2* p) {
     return make_int2(f[0], f[1]);
 }
 
+// Work queue of a persistent round: a wave takes the next 64 queries.  The
+// head is read first, so once the queue is drained the remaining waves leave
+// without contending on the atomic.
+__device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
+    unsigned h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (h >= n_in) return h;
+    return atomicAdd(head, 64u);
+}
+
 // ---------------------------------------------------------------------------
 // one bounded (or the final unbounded) round as a persistent launch.  Each
 // wavefront takes 64 consecutive queries of the active list (Morton order, so
@@ -383,6 +392,49 @@ __device__ __forceinline__ int2 ld2i(const GPRT_AS4 int2* p) {
 // it.  The traversal stack is one uniform LDS stack per wavefront; the radix
 // tree depth bound keeps it from overflowing.
 // ---------------------------------------------------------------------------
+// FP64 re-rank of the guard band in (d64, id) order and output of the first k.
+// Candidates are the list entries with 0 <= d32 <= tf (sentinels are -inf).
+template <int C>
+__device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float* ld, const int* ls, float qx,
+                                                 float qy, float qz, float tf, int g, int c, int k,
+                                                 const float4* __restrict__ pts) {
+    double dd[C];
+    uint32_t id[C];
+    int reads = 0;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const bool in = ld[j] >= 0.0f && ld[j] <= tf && ls[j] >= 0;
+        dd[j] = INFINITY;
+        id[j] = GPRT_ID_NONE;
+        reads += in;
+        if (in) {
+            const float4 p = pts[ls[j]];
+            dd[j] = d2d(qx, qy, qz, p.x, p.y, p.z);
+            id[j] = (uint32_t)__float_as_int(p.w);
+        }
+    }
+    bool swapped = true;
+    while (swapped) {
+        swapped = false;
+#pragma unroll
+        for (int j = 0; j + 1 < C; ++j) {
+            const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
+            const double lo = gt ? dd[j + 1] : dd[j], hi = gt ? dd[j] : dd[j + 1];
+            const uint32_t li = gt ? id[j + 1] : id[j], hiid = gt ? id[j] : id[j + 1];
+            dd[j] = lo; dd[j + 1] = hi; id[j] = li; id[j + 1] = hiid;
+            swapped |= gt;
+        }
+    }
+    const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
+#pragma unroll
+    for (int j = 0; j < C; ++j)
+        if (j < k) {
+            a.ids_out[ob + j] = id[j];
+            if (a.d2_out) a.d2_out[ob + j] = dd[j];
+        }
+    return reads;
+}
+
 constexpr int PACKET_STACK = EXACT_STACK;
 
 template <int C, int G, bool COUNT>
@@ -390,14 +442,15 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
     __shared__ int stk[4][PACKET_STACK];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
+    if (n_in == 0) return;
     const GPRT_AS4 Node* nodes = const_view(a.nodes);      // uniform node loads
     const GPRT_AS4 float4* upts = const_view(a.pts);       // uniform leaf-point loads
     const GPRT_AS4 int2* roots = const_view(a.roots);
     const float4* __restrict__ pts = a.pts;                // per-lane loads (re-rank)
-    uint64_t visits = 0, tests = 0;
+    uint64_t visits = 0, tests = 0, reranks = 0;
     while (true) {
         unsigned base = 0;
-        if (lane == 0) base = atomicAdd(&a.counters[CNT_WORK + a.round], 64u);
+        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
         base = __shfl(base, 0);
         if (base >= n_in) break;
         const unsigned i = base + lane;
@@ -502,40 +555,9 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
                 if (overflow || Q.ev_min <= tf) {
                     status = 2;
                 } else {
-                    // FP64 re-rank of the guard band, (d64, id) order
-                    double dd[C];
-                    uint32_t id[C];
-#pragma unroll
-                    for (int j = 0; j < C; ++j) {
-                        const bool in = Q.L.d[j] >= 0.0f && Q.L.d[j] <= tf && Q.L.s[j] >= 0;
-                        dd[j] = INFINITY;
-                        id[j] = GPRT_ID_NONE;
-                        if (in) {
-                            const float4 p = pts[Q.L.s[j]];
-                            dd[j] = d2d(Q.qx, Q.qy, Q.qz, p.x, p.y, p.z);
-                            id[j] = (uint32_t)__float_as_int(p.w);
-                        }
-                    }
-                    bool swapped = true;
-                    while (swapped) {
-                        swapped = false;
-#pragma unroll
-                        for (int j = 0; j + 1 < C; ++j) {
-                            const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
-                            const double lo = gt ? dd[j + 1] : dd[j], hi = gt ? dd[j] : dd[j + 1];
-                            const uint32_t li = gt ? id[j + 1] : id[j], hiid = gt ? id[j] : id[j + 1];
-                            dd[j] = lo; dd[j + 1] = hi; id[j] = li; id[j + 1] = hiid;
-                            swapped |= gt;
-                        }
-                    }
-                    const int g = __float_as_int(q.w);
-                    const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
-#pragma unroll
-                    for (int j = 0; j < C; ++j)
-                        if (j < k) {
-                            a.ids_out[ob + j] = id[j];
-                            if (a.d2_out) a.d2_out[ob + j] = dd[j];
-                        }
+                    const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
+                                                       k, pts);
+                    if (COUNT) reranks += rr;
                     if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
                 }
             }
@@ -547,9 +569,153 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
     }
     if (COUNT) {
         // visits/tests are wave-uniform in the packet walk: count once per wave
+        for (int o = 32; o > 0; o >>= 1) reranks += __shfl_xor(reranks, o);
         if (lane == 0) {
             atomicAdd(&a.stats64[0], (unsigned long long)visits);
             atomicAdd(&a.stats64[1], (unsigned long long)tests);
+            atomicAdd(&a.stats64[2], (unsigned long long)reranks);
+        }
+    }
+}
+
+// Scan a leaf bucket [f, f+m) issuing the point loads four at a time, so a
+// bucket costs one memory latency per four points instead of one per point.
+template <class Q4>
+__device__ __forceinline__ void scan_bucket(Q4& Q, const float4* __restrict__ pts, int f, int m) {
+    for (int t0 = 0; t0 < m; t0 += 4) {
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t0 + u < m) p[u] = pts[f + t0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (t0 + u < m) Q.visit_point(p[u], f + t0 + u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same round with one independent walk per lane (closest child first).
+// The stack holds node references only (4 bytes, STACK per lane in LDS,
+// [depth][lane] so lanes hit distinct banks); a popped node that no longer
+// intersects the threshold costs one node load and prunes both children,
+// since a child box lies inside its parent's box.  A walk deeper than the
+// stack is finished by the exact FP64 kernel.
+// ---------------------------------------------------------------------------
+template <int C, int G, int STACK, bool COUNT>
+__global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
+    __shared__ int stk[STACK][256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
+    if (n_in == 0) return;
+    const Node* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ pts = a.pts;
+    uint64_t visits = 0, tests = 0, reranks = 0;
+    while (true) {
+        unsigned base = 0;
+        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
+        base = __shfl(base, 0);
+        if (base >= n_in) break;
+        const unsigned i = base + lane;
+        const bool valid = i < n_in;
+        int status = 0;
+        int s = 0;
+        if (valid) {
+            s = a.in_list ? a.in_list[i] : (int)i;
+            const float4 q = a.qs[s];
+            const int c = a.qcloud ? a.qcloud[s] : 0;
+            const int k = a.kc ? a.kc[c] : a.k;
+            float R = a.last ? INFINITY : a.radius0[c];
+            if (!a.last)
+                for (int r = 0; r < a.round; ++r) R *= a.mult;
+            const float T0 = a.last ? INFINITY : R * R;
+            F32Query<C, G> Q;
+            Q.start(q.x, q.y, q.z, T0, k, true);
+            bool overflow = false;
+            const int2 root = a.roots[c];
+            if (root.y > 0 && root.x < 0) {
+                scan_bucket(Q, pts, ~root.x, root.y);
+                if (COUNT) tests += root.y;
+            } else if (root.y > 0) {
+                int sp = 0, node = root.x, steps = 0;
+                while (true) {
+                    if (++steps > a.max_steps) { overflow = true; break; }
+                    const Node nd = nodes[node];
+                    if (COUNT) ++visits;
+                    const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
+                    const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
+                    const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                    const bool lA = rA < 0, lB = rB < 0;
+                    const bool bfirst = dB < dA;
+                    // leaf buckets are scanned at once, nearer first
+                    if (lA && lB && bfirst) {
+                        if (dB <= Q.tau) {
+                            const int f = ~rB, m = __float_as_int(nd.bhi.w);
+                            scan_bucket(Q, pts, f, m);
+                            if (COUNT) tests += m;
+                        }
+                        if (dA <= Q.tau) {
+                            const int f = ~rA, m = __float_as_int(nd.ahi.w);
+                            scan_bucket(Q, pts, f, m);
+                            if (COUNT) tests += m;
+                        }
+                    } else {
+                        if (lA && dA <= Q.tau) {
+                            const int f = ~rA, m = __float_as_int(nd.ahi.w);
+                            scan_bucket(Q, pts, f, m);
+                            if (COUNT) tests += m;
+                        }
+                        if (lB && dB <= Q.tau) {
+                            const int f = ~rB, m = __float_as_int(nd.bhi.w);
+                            scan_bucket(Q, pts, f, m);
+                            if (COUNT) tests += m;
+                        }
+                    }
+                    const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
+                    if (gA && gB) {
+                        if (sp >= STACK) { overflow = true; break; }
+                        stk[sp][tid] = bfirst ? rA : rB;
+                        ++sp;
+                        node = bfirst ? rB : rA;
+                        continue;
+                    }
+                    if (gA) { node = rA; continue; }
+                    if (gB) { node = rB; continue; }
+                    if (sp == 0) break;
+                    node = stk[--sp][tid];
+                }
+            }
+            const float kth = Q.kth();
+            const bool full = kth < INFINITY;
+            const bool resolved = a.last ? true : (full && kth <= T0);
+            if (!resolved) {
+                status = 1;
+            } else {
+                const float tf = band(kth);
+                if (overflow || Q.ev_min <= tf) {
+                    status = 2;
+                } else {
+                    const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
+                                                       k, pts);
+                    if (COUNT) reranks += rr;
+                    if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
+                }
+            }
+        }
+        const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
+        if (slot >= 0) a.out_list[slot] = s;
+        const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
+        if (fslot >= 0) a.fb_list[fslot] = s;
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            visits += __shfl_xor(visits, o);
+            tests += __shfl_xor(tests, o);
+            reranks += __shfl_xor(reranks, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats64[0], (unsigned long long)visits);
+            atomicAdd(&a.stats64[1], (unsigned long long)tests);
+            atomicAdd(&a.stats64[2], (unsigned long long)reranks);
         }
     }
 }
@@ -600,11 +766,18 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
     float stk_d[EXACT_STACK];
     int sp = 0;
     auto scan = [&](int f, int m) {
-        for (int t = 0; t < m; ++t) {
-            const float4 p = pts[f + t];
-            const double dd = d2d(qx, qy, qz, p.x, p.y, p.z);
-            const uint32_t ii = (uint32_t)__float_as_int(p.w);
-            if (dd < L.d[KMAX - 1] || (dd == L.d[KMAX - 1] && ii < L.id[KMAX - 1])) L.insert(dd, ii);
+        for (int t0 = 0; t0 < m; t0 += 4) {
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + u < m) p[u] = pts[f + t0 + u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + u < m) {
+                    const double dd = d2d(qx, qy, qz, p[u].x, p[u].y, p[u].z);
+                    const uint32_t ii = (uint32_t)__float_as_int(p[u].w);
+                    if (dd < L.d[KMAX - 1] || (dd == L.d[KMAX - 1] && ii < L.id[KMAX - 1])) L.insert(dd, ii);
+                }
         }
         tests += m;
     };
@@ -712,6 +885,10 @@ int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "sampler launch failed");
 }
 
+static int g_query_sort_bits = 24;
+int query_sort_bits() { return g_query_sort_bits; }
+extern "C" GPRT_HIP_API void gprt_hip_debug_set_query_sort_bits(int b) { g_query_sort_bits = b < 8 ? 8 : (b > 30 ? 30 : b); }
+
 int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
                       int64_t max_per_cloud) {
@@ -721,7 +898,7 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
                        perm);
     int cloud_bits = 0;
     while ((1 << cloud_bits) < nclouds) ++cloud_bits;
-    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 6, 30));
+    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 30 - query_sort_bits(), 30));
     if (cloud_bits > 0) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 32, 32 + cloud_bits));
     hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, nq, qs4, counters);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
@@ -754,20 +931,22 @@ int capacity_class(int kmax, int guard) {
 }
 
 template <int C>
-static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, bool count) {
-    if (count)
-        hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
+static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, bool count) {
+    if (trav == GPRT_TRAV_WAVE) {
+        if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false>), dim3(grid), dim3(256), 0, s, a);
+    }
 }
 
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int stack, bool count) {
-    (void)stack;
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, bool count) {
     switch (cap) {
-        case 8: launch_round_c<8>(s, a, grid, count); break;
-        case 16: launch_round_c<16>(s, a, grid, count); break;
-        case 32: launch_round_c<32>(s, a, grid, count); break;
-        default: launch_round_c<64>(s, a, grid, count); break;
+        case 8: launch_round_c<8>(s, a, grid, trav, count); break;
+        case 16: launch_round_c<16>(s, a, grid, trav, count); break;
+        case 32: launch_round_c<32>(s, a, grid, trav, count); break;
+        default: launch_round_c<64>(s, a, grid, trav, count); break;
     }
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }

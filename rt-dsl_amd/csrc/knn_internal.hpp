@@ -12,6 +12,7 @@ constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
 constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
 constexpr int CNT_TOTAL = 48;
+constexpr int CNT_STATS64 = 4;   // u64 statistics after the counters: visits, tests, re-rank reads
 
 // Karras depth bound: 30 Morton bits + up to 32 cloud bits... a path of the
 // radix tree has strictly increasing common-prefix lengths, so its length is
@@ -72,7 +73,7 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
 int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int stack, bool count);
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
 
 }  // namespace gprt

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "gprt_internal.hpp"
+#include "refit.hpp"
 
 namespace gprt {
 
@@ -316,7 +317,7 @@ __device__ __forceinline__ int delta_fn(const uint64_t* __restrict__ keys, int n
 
 __global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, int leaf_size,
                               const int64_t* __restrict__ off, Node* __restrict__ nodes, int* __restrict__ parent,
-                              int2* __restrict__ roots) {
+                              int2* __restrict__ roots, int2* __restrict__ ranges) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int dr = delta_fn(keys, n, i, i + 1), dl = delta_fn(keys, n, i, i - 1);
@@ -337,6 +338,7 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, int leaf
     }
     int gamma = i + s * d + min(d, 0);
     int lo = min(i, j), hi = max(i, j);
+    ranges[i] = make_int2(lo, hi);
     int cntA = gamma - lo + 1, cntB = hi - gamma;
     // child refs: single points are always leaves; small subtrees become leaf buckets
     int refA = (lo == gamma || cntA <= leaf_size) ? ~lo : gamma;
@@ -361,37 +363,6 @@ __global__ void roots_small_kernel(const int64_t* __restrict__ off, int nclouds,
     if (c < nclouds) {
         int64_t cnt = off[c + 1] - off[c];
         if (cnt <= leaf_size) roots[c] = make_int2(~(int)off[c], (int)cnt);  // empty cloud: count 0
-    }
-}
-
-// bottom-up refit: one thread per point; the second thread to reach a node
-// merges the two child boxes and continues upward.
-__global__ void refit_kernel(const float4* __restrict__ pts4, int n, Node* nodes, const int* __restrict__ parent,
-                             int* flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float4 p = pts4[i];
-    float lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
-    int link = parent[(n - 1) + i];
-    while (link >= 0) {
-        int node = link >> 1;
-        bool sideA = (link & 1) == 0;
-        Node* nd = nodes + node;
-        float* slo = sideA ? &nd->alo.x : &nd->blo.x;
-        float* shi = sideA ? &nd->ahi.x : &nd->bhi.x;
-        slo[0] = lo[0]; slo[1] = lo[1]; slo[2] = lo[2];
-        shi[0] = hi[0]; shi[1] = hi[1]; shi[2] = hi[2];
-        // release our box, acquire the sibling's (agent scope: the sibling
-        // may run on another CU whose L1 is not coherent with ours)
-        int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const float* olo = sideA ? &nd->blo.x : &nd->alo.x;
-        const float* ohi = sideA ? &nd->bhi.x : &nd->ahi.x;
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fminf(lo[a], __hip_atomic_load(olo + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            hi[a] = fmaxf(hi[a], __hip_atomic_load(ohi + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        }
-        link = parent[node];
     }
 }
 
@@ -434,6 +405,8 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     GPRT_TRY(t->codes.reserve(sizeof(uint64_t) * std::max(n, 1)));
     GPRT_TRY(t->parent.reserve(sizeof(int) * std::max(2 * n - 1, 1)));
     GPRT_TRY(t->flags.reserve(sizeof(int) * std::max(n - 1, 1)));
+    GPRT_TRY(t->ranges.reserve(sizeof(int2) * std::max(n - 1, 1)));
+    GPRT_TRY(t->ready.reserve(sizeof(int) * (std::max(n, 1) + 4)));
     GPRT_TRY(ctx->ws[7].reserve(sizeof(int) * 6 * nclouds));
     int* enc = ctx->ws[7].as<int>();
 
@@ -465,11 +438,14 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     }
     if (n > 1) {
         hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->flags.as<int>(), n - 1, 0);
+        int* ready = t->ready.as<int>();
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ready + n, 1, 0);
         hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), n,
                            leaf_size, t->cloud_off.as<int64_t>(), t->nodes.as<Node>(), t->parent.as<int>(),
-                           t->roots.as<int2>());
-        hipLaunchKernelGGL(refit_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, t->pts4.as<float4>(), n,
-                           t->nodes.as<Node>(), t->parent.as<int>(), t->flags.as<int>());
+                           t->roots.as<int2>(), t->ranges.as<int2>());
+        BoxSlots slots{t->nodes.as<Node>(), t->pts4.as<float4>()};
+        launch_refit<BoxP>(s, slots, t->parent.as<int>(), t->ranges.as<int2>(), n, t->flags.as<int>(), ready,
+                           (unsigned*)(ready + n));
     }
     GPRT_HIP_TRY(hipGetLastError());
     GPRT_HIP_TRY(hipMemcpyAsync(t->h_bbox, t->cloud_bbox.ptr, sizeof(float) * 6, hipMemcpyDeviceToHost, s));

@@ -1,0 +1,305 @@
+// range.hip -- fixed-radius range query over the LBVH (r_nn! / range_query!,
+// gprt-macros/src/lib.rs:729-750; the predicate is Sphere::intersect's
+// d^2 <= r^2, gprt-core/src/lib.rs:82-93).
+//
+// Contract: query q returns every point with d64(q,p) <= (double)r*(double)r,
+// ordered by (d64, id), as a CSR list.  The reference's version returns one
+// flat array of at most 1e6 ids for all queries with no per-query split
+// (lib.rs:662-668); this one keeps the split.
+//
+// Pipeline: Morton-ordered queries -> FP32 traversal with the guard threshold
+// band(fl32(r*r)) collecting candidates, each confirmed in FP64 -> per-query
+// counts -> offsets (exclusive scan) -> the same traversal writing ids and
+// d64 -> per-query sort of (d64, id).
+#include <algorithm>
+#include <cfloat>
+
+#include "gprt_internal.hpp"
+#include "knn_internal.hpp"
+
+namespace gprt {
+
+// per-lane walk with a private stack of EXACT_STACK entries (never overflows:
+// the radix-tree depth bound).  mode 0 counts, mode 1 writes.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nodes, const float4* __restrict__ pts,
+                                                    int2 root, const float4* __restrict__ qs, int nq, float r_all,
+                                                    const float* __restrict__ radii,
+                                                    int64_t* __restrict__ counts, const int64_t* __restrict__ off,
+                                                    uint32_t* __restrict__ ids, double* __restrict__ d2,
+                                                    int64_t capacity, int max_steps, unsigned* err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float4 q = qs[i];
+    const int g = __float_as_int(q.w);
+    const float r = radii ? radii[g] : r_all;
+    const double r2 = (double)r * (double)r;
+    const float tau = band(r * r);
+    int64_t cnt = 0;
+    int64_t pos = WRITE ? off[g] : 0, end = WRITE ? min(off[g + 1], capacity) : 0;
+    auto visit = [&](int f, int m) {
+        for (int t = 0; t < m; ++t) {
+            const float4 p = pts[f + t];
+            if (d2f(q.x, q.y, q.z, p.x, p.y, p.z) <= tau) {
+                const double dd = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+                if (dd <= r2) {
+                    if (WRITE) {
+                        if (pos < end) {
+                            ids[pos] = (uint32_t)__float_as_int(p.w);
+                            d2[pos] = dd;
+                        }
+                        ++pos;
+                    } else {
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    };
+    if (root.y > 0) {
+        if (root.x < 0) {
+            visit(~root.x, root.y);
+        } else {
+            int stk[EXACT_STACK];
+            int sp = 0, node = root.x, steps = 0;
+            while (true) {
+                if (++steps > max_steps || sp >= EXACT_STACK) { atomicAdd(err, 1u); break; }
+                const Node nd = nodes[node];
+                const float dA = box2f(q.x, q.y, q.z, nd.alo, nd.ahi);
+                const float dB = box2f(q.x, q.y, q.z, nd.blo, nd.bhi);
+                const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                const bool inA = dA <= tau, inB = dB <= tau;
+                if (rA < 0 && inA) visit(~rA, __float_as_int(nd.ahi.w));
+                if (rB < 0 && inB) visit(~rB, __float_as_int(nd.bhi.w));
+                const bool gA = rA >= 0 && inA, gB = rB >= 0 && inB;
+                if (gA && gB) {
+                    stk[sp++] = rB;
+                    node = rA;
+                    continue;
+                }
+                if (gA) { node = rA; continue; }
+                if (gB) { node = rB; continue; }
+                if (sp == 0) break;
+                node = stk[--sp];
+            }
+        }
+    }
+    if (!WRITE) counts[g] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of int64 counts -> offsets[0..n] (three phases)
+// ---------------------------------------------------------------------------
+constexpr int SCAN_TILE = 1024;
+
+__global__ __launch_bounds__(256) void scan_tile_sums(const int64_t* __restrict__ in, int n, int64_t* sums) {
+    __shared__ int64_t red[256];
+    int64_t s = 0;
+    const int base = blockIdx.x * SCAN_TILE;
+    for (int j = threadIdx.x; j < SCAN_TILE; j += 256)
+        if (base + j < n) s += in[base + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(int64_t* sums, int m, int64_t* total) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x, per = (m + 1023) / 1024;
+    const int b = min(t * per, m), e = min(b + per, m);
+    int64_t s = 0;
+    for (int j = b; j < e; ++j) s += sums[j];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;
+    for (int j = b; j < e; ++j) {
+        int64_t v = sums[j];
+        sums[j] = run;
+        run += v;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+__global__ __launch_bounds__(256) void scan_tile_apply(const int64_t* __restrict__ in, int n,
+                                                       const int64_t* __restrict__ sums, int64_t* __restrict__ out) {
+    __shared__ int64_t v[SCAN_TILE];
+    const int base = blockIdx.x * SCAN_TILE;
+    for (int j = threadIdx.x; j < SCAN_TILE; j += 256) v[j] = base + j < n ? in[base + j] : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = sums[blockIdx.x];
+        for (int j = 0; j < SCAN_TILE; ++j) {
+            int64_t x = v[j];
+            v[j] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < SCAN_TILE; j += 256)
+        if (base + j < n) out[base + j] = v[j];
+}
+
+__global__ void set_last_kernel(int64_t* out, int n, const int64_t* total) { out[n] = *total; }
+
+// ---------------------------------------------------------------------------
+// per-query sort of (d64, id).  Segments of <= 64 are sorted by one wave with
+// a register bitonic network; longer ones are queued for a block each.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool pair_gt(double da, uint32_t ia, double db, uint32_t ib) {
+    return da > db || (da == db && ia > ib);
+}
+
+__global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __restrict__ off, int nq,
+                                                            uint32_t* ids, double* d2, int64_t capacity,
+                                                            int* big_list, unsigned* big_count) {
+    const int lane = threadIdx.x & 63;
+    const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (seg >= nq) return;  // whole wave leaves together
+    const int64_t b = off[seg], e = min(off[seg + 1], capacity);
+    const int64_t m = e - b;
+    if (m <= 1) return;
+    if (m > 64) {
+        if (lane == 0) big_list[atomicAdd(big_count, 1u)] = seg;
+        return;
+    }
+    double d = lane < m ? d2[b + lane] : INFINITY;
+    uint32_t id = lane < m ? ids[b + lane] : 0xFFFFFFFFu;
+    for (int size = 2; size <= 64; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const double od = __shfl_xor(d, stride);
+            const uint32_t oi = __shfl_xor(id, stride);
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            // keep the smaller element on the lower lane of an ascending pair
+            const bool take_other = lower == up ? pair_gt(d, id, od, oi) : pair_gt(od, oi, d, id);
+            if (take_other) { d = od; id = oi; }
+        }
+    if (lane < m) {
+        d2[b + lane] = d;
+        ids[b + lane] = id;
+    }
+}
+
+// one block per long segment: bitonic network over the next power of two in
+// LDS when it fits (<= 2048), else an in-place odd-even transposition sort.
+__global__ __launch_bounds__(256) void segsort_big_kernel(const int64_t* __restrict__ off, const int* big_list,
+                                                          const unsigned* big_count, uint32_t* ids, double* d2,
+                                                          int64_t capacity) {
+    __shared__ double sd[2048];
+    __shared__ uint32_t si[2048];
+    if (blockIdx.x >= *big_count) return;
+    const int seg = big_list[blockIdx.x];
+    const int64_t b = off[seg], e = min(off[seg + 1], capacity);
+    const int m = (int)(e - b);
+    if (m <= 2048) {
+        int P = 1;
+        while (P < m) P <<= 1;
+        for (int j = threadIdx.x; j < P; j += 256) {
+            sd[j] = j < m ? d2[b + j] : INFINITY;
+            si[j] = j < m ? ids[b + j] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (int size = 2; size <= P; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int j = threadIdx.x; j < P; j += 256) {
+                    const int p = j ^ stride;
+                    if (p > j) {
+                        const bool up = (j & size) == 0;
+                        if (pair_gt(sd[j], si[j], sd[p], si[p]) == up) {
+                            double td = sd[j]; sd[j] = sd[p]; sd[p] = td;
+                            uint32_t ti = si[j]; si[j] = si[p]; si[p] = ti;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int j = threadIdx.x; j < m; j += 256) {
+            d2[b + j] = sd[j];
+            ids[b + j] = si[j];
+        }
+        return;
+    }
+    // long lists: odd-even transposition directly in global memory
+    for (int pass = 0; pass < m; ++pass) {
+        for (int j = 2 * threadIdx.x + (pass & 1); j + 1 < m; j += 512) {
+            const double da = d2[b + j], db = d2[b + j + 1];
+            const uint32_t ia = ids[b + j], ib = ids[b + j + 1];
+            if (pair_gt(da, ia, db, ib)) {
+                d2[b + j] = db; d2[b + j + 1] = da;
+                ids[b + j] = ib; ids[b + j + 1] = ia;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
+              int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
+              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii) {
+    hipStream_t s = ctx->stream;
+    unsigned* counters = ctx->counters.as<unsigned>();
+    GPRT_TRY(keys_buf.reserve(8ull * nq));
+    GPRT_TRY(perm_buf.reserve(4ull * nq));
+    GPRT_TRY(qs4_buf.reserve(16ull * nq));
+    GPRT_TRY(counts_buf.reserve(8ull * nq + 8));
+    const int tiles = (nq + SCAN_TILE - 1) / SCAN_TILE;
+    GPRT_TRY(sums_buf.reserve(8ull * tiles + 64));
+    GPRT_TRY(ctx->ws[7].reserve(64));
+    int64_t* qoff = (int64_t*)(ctx->ws[7].as<char>() + 32);
+    int64_t h_qoff[2] = {0, nq};
+    GPRT_HIP_TRY(hipMemcpyAsync(qoff, h_qoff, 16, hipMemcpyHostToDevice, s));
+    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
+    GPRT_TRY(launch_query_prep(ctx, d_q, qoff, 1, T.cloud_bbox.as<float>(), 1, nq, keys_buf.as<uint64_t>(),
+                               perm_buf.as<uint32_t>(), qs4_buf.as<float4>(), counters, nq));
+    int2 root;
+    GPRT_HIP_TRY(hipMemcpyAsync(&root, T.roots.ptr, sizeof(int2), hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    const int grid = (nq + 255) / 256;
+    const int max_steps = T.n + 64;
+    if (!d_ids) {
+        hipLaunchKernelGGL(range_kernel<false>, dim3(grid), dim3(256), 0, s, T.nodes.as<Node>(),
+                           T.pts4.as<float4>(), root, qs4_buf.as<float4>(), nq, r, d_radii, counts_buf.as<int64_t>(),
+                           nullptr, nullptr, nullptr, (int64_t)0, max_steps, counters + CNT_ERROR);
+        int64_t* sums = sums_buf.as<int64_t>();
+        int64_t* total = sums + tiles;
+        hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(256), 0, s, counts_buf.as<int64_t>(), nq, sums);
+        hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, tiles, total);
+        hipLaunchKernelGGL(scan_tile_apply, dim3(tiles), dim3(256), 0, s, counts_buf.as<int64_t>(), nq, sums, d_off);
+        hipLaunchKernelGGL(set_last_kernel, dim3(1), dim3(1), 0, s, d_off, nq, total);
+        GPRT_HIP_TRY(hipGetLastError());
+        GPRT_HIP_TRY(hipMemcpyAsync(h_total, total, 8, hipMemcpyDeviceToHost, s));
+        GPRT_HIP_TRY(hipStreamSynchronize(s));
+        return GPRT_OK;
+    }
+    GPRT_TRY(d2_buf.reserve(8ull * std::max<int64_t>(capacity, 1)));
+    GPRT_TRY(big_buf.reserve(4ull * nq + 16));
+    hipLaunchKernelGGL(range_kernel<true>, dim3(grid), dim3(256), 0, s, T.nodes.as<Node>(), T.pts4.as<float4>(),
+                       root, qs4_buf.as<float4>(), nq, r, d_radii, nullptr, d_off, d_ids, d2_buf.as<double>(), capacity,
+                       max_steps, counters + CNT_ERROR);
+    unsigned* big_count = counters + CNT_FALLBACK;
+    hipLaunchKernelGGL(segsort_small_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, d_off, nq, d_ids,
+                       d2_buf.as<double>(), capacity, big_buf.as<int>(), big_count);
+    unsigned nbig = 0;
+    GPRT_HIP_TRY(hipMemcpyAsync(&nbig, big_count, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    if (nbig)
+        hipLaunchKernelGGL(segsort_big_kernel, dim3(nbig), dim3(256), 0, s, d_off, big_buf.as<int>(), big_count,
+                           d_ids, d2_buf.as<double>(), capacity);
+    GPRT_HIP_TRY(hipGetLastError());
+    return GPRT_OK;
+}
+
+}  // namespace gprt

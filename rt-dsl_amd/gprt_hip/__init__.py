@@ -47,6 +47,7 @@ class Config(ctypes.Structure):
         ("grid_blocks", ctypes.c_int),
         ("block_threads", ctypes.c_int),
         ("collect_counters", ctypes.c_int),
+        ("time_kernels", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -66,6 +67,8 @@ class RunStats(ctypes.Structure):
         ("ms_build", ctypes.c_double),
         ("node_visits", ctypes.c_int64),
         ("point_tests", ctypes.c_int64),
+        ("rerank_reads", ctypes.c_int64),
+        ("ms_kernel", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -317,12 +320,18 @@ class Context:
         return out
 
     def autotune(self, pts, queries, k, budget_trials=24, json_path=None):
+        """The project's autotuner (gprt_autotune! of the reference): returns
+        (Config, leaf_size) and optionally writes the Schedule-superset JSON."""
         pts, queries = _points(pts), _points(queries)
+        p, dev = _ptr(pts)
+        qp, qdev = _ptr(queries)
+        if dev != qdev:
+            raise ValueError("points and queries must both be host arrays or both device arrays")
         cfg = Config()
         leaf = ctypes.c_int()
-        _check(lib().gprt_hip_autotune(self.h, pts.ctypes.data, len(pts), queries.ctypes.data, len(queries), k,
-                                       budget_trials, ctypes.byref(cfg), ctypes.byref(leaf),
-                                       json_path.encode() if json_path else None, PTR_HOST))
+        _check(lib().gprt_hip_autotune(self.h, p, len(pts), qp, len(queries), k, budget_trials, ctypes.byref(cfg),
+                                       ctypes.byref(leaf), json_path.encode() if json_path else None,
+                                       PTR_DEVICE if dev else PTR_HOST))
         return cfg, leaf.value
 
 

@@ -1,0 +1,337 @@
+"""GPU parity of the building blocks and the other hot-path entry points:
+Morton codes, radius samplers, radix sort, LBVH invariants, range query,
+batched uneven clouds, Barnes-Hut, the reference FFI, the autotuner."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gprt_hip
+import oracle
+from gprt_hip import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+
+# ---------------------------------------------------------------- pieces
+@pytest.mark.parametrize("case", ["uniform", "flat", "negative", "road", "lidar_subset"])
+def test_morton_codes_bit_exact(ctx, case):
+    rng = np.random.default_rng(1)
+    if case == "uniform":
+        pts = rng.random((100_000, 3), dtype=np.float32)
+    elif case == "flat":
+        pts = rng.random((10_000, 3), dtype=np.float32)
+        pts[:, 2] = 7.0
+    elif case == "negative":
+        pts = (rng.normal(size=(10_000, 3)) * 50).astype(np.float32)
+    elif case == "road":
+        pts, _ = D.road_network(n=50_000, nq=10)
+    else:
+        pts, _ = D.lidar_scan(frames=2, nq=10)
+    codes, bbox = ctx.morton_codes(pts)
+    rc, rb = oracle.morton_codes(pts)
+    assert np.array_equal(bbox, rb)
+    assert np.array_equal(codes, rc)
+
+
+@pytest.mark.parametrize("sampler,heur,param,k", [
+    (gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_PERCENTILE, 0.95, 5),
+    (gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_PERCENTILE, 0.10, 5),
+    (gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_MAX, 0.0, 5),
+    (gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_FIXED, 0.25, 5),
+    (gprt_hip.SAMPLER_STRIDED, gprt_hip.RADIUS_PERCENTILE, 0.10, 5),
+    (gprt_hip.SAMPLER_STRIDED, gprt_hip.RADIUS_PERCENTILE, 0.95, 16),
+    (gprt_hip.SAMPLER_STRIDED, gprt_hip.RADIUS_FIXED, 0.5, 5),
+])
+def test_samplers_bit_exact(ctx, sampler, heur, param, k):
+    for pts, q in (D.uniform_cloud(100_000, 10_000), D.lidar_scan(frames=3, nq=20_000)):
+        got = ctx.sample_radius(pts, q, sampler, heur, param, k)
+        if sampler == gprt_hip.SAMPLER_LCG:
+            want = oracle.sampler_lcg(pts, q, heur, param)
+        else:
+            want = oracle.sampler_strided(pts, q, k, heur, param)
+        assert got == want, (got, want)
+
+
+def test_sampler_small_inputs(ctx):
+    pts = np.zeros((3, 3), np.float32)
+    q = np.ones((5, 3), np.float32)
+    assert ctx.sample_radius(pts, q, gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_PERCENTILE, 0.5) == 10.0
+    assert ctx.sample_radius(pts, q, gprt_hip.SAMPLER_LCG, gprt_hip.RADIUS_PERCENTILE, 0.5) == \
+        oracle.sampler_lcg(pts, q, oracle.HEUR_PERCENTILE, 0.5)
+
+
+@pytest.mark.parametrize("n,bits", [(1, (0, 64)), (1000, (0, 30)), (300_000, (0, 40)), (70_000, (8, 24))])
+def test_radix_sort_stable(ctx, n, bits):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 40, n, dtype=np.uint64) & np.uint64((1 << 40) - 1)
+    keys[: n // 3] = keys[0]  # many duplicates
+    vals = np.arange(n, dtype=np.uint32)
+    k2, v2 = ctx.sort_pairs(keys, vals, *bits)
+    sub = (keys >> np.uint64(bits[0])) & np.uint64((1 << (bits[1] - bits[0])) - 1) if bits[1] < 64 else keys >> np.uint64(bits[0])
+    order = np.argsort(sub, kind="stable")
+    assert np.array_equal(v2, vals[order])
+    assert np.array_equal(k2, keys[order])
+
+
+def _walk(nodes, roots, c):
+    """Host walk of the exported tree: every point of cloud c once; boxes tight."""
+    ref, cnt = roots[c]
+    seen = []
+    if cnt == 0:
+        return seen
+    if ref < 0:
+        return list(range(~ref, ~ref + cnt))
+    stack = [ref]
+    while stack:
+        i = stack.pop()
+        nd = nodes[i]
+        for lo, hi in ((0, 4), (8, 12)):
+            r = nd[lo + 3].view(np.int32)
+            m = nd[hi + 3].view(np.int32)
+            if r < 0:
+                seen.extend(range(~r, ~r + m))
+            else:
+                stack.append(int(r))
+    return seen
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8, 32])
+def test_lbvh_invariants(ctx, leaf):
+    pts, _ = D.road_network(n=40_000, nq=10)
+    t = ctx.build(pts, leaf_size=leaf)
+    keys, perm, nodes, roots = t.export()
+    assert np.array_equal(np.sort(perm), np.arange(len(pts)))
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    codes, _ = oracle.morton_codes(pts)
+    assert np.array_equal((keys & np.uint64(0xFFFFFFFF)).astype(np.uint32), codes[perm])
+    # stable order: equal codes keep input order
+    same = np.diff(keys.astype(np.int64)) == 0
+    assert np.all(np.diff(perm.astype(np.int64))[same] > 0)
+    seen = _walk(nodes, roots, 0)
+    assert sorted(seen) == list(range(len(pts)))
+    # child boxes equal the union of their points (sorted order)
+    sp = pts[perm]
+    rng = np.random.default_rng(0)
+    for i in rng.integers(0, len(nodes), 300):
+        nd = nodes[i]
+        for lo, hi in ((0, 4), (8, 12)):
+            r = nd[lo + 3].view(np.int32)
+            m = nd[hi + 3].view(np.int32)
+            if r < 0:
+                sub = sp[~r: ~r + m]
+                assert np.array_equal(nd[lo:lo + 3], sub.min(0)) and np.array_equal(nd[hi:hi + 3], sub.max(0))
+
+
+def test_lbvh_batched_roots(ctx):
+    pts, poff, q, qoff, kc = D.clustered_batch(n_clouds=12, lo=1, hi=5e3, seed=3)
+    t = ctx.build_batched(pts, poff, leaf_size=8)
+    keys, perm, nodes, roots = t.export()
+    for c in range(12):
+        got = sorted(int(perm[i]) for i in _walk(nodes, roots, c))
+        assert got == list(range(poff[c], poff[c + 1])), c
+
+
+# ---------------------------------------------------------------- range query
+def _check_range(off, ids, roff, rids):
+    assert np.array_equal(off, roff)
+    assert np.array_equal(ids, rids)
+
+
+def test_range_uniform(ctx):
+    pts, q = D.uniform_cloud(50_000, 3_000, seed=4)
+    t = ctx.build(pts)
+    for r in (0.0, 0.01, 0.05, 0.2):
+        _check_range(*t.range_query(q, r), *oracle.range_brute(pts, q, r))
+
+
+def test_range_boundary_and_lattice(ctx):
+    g = np.arange(12, dtype=np.float32)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).copy()
+    q = pts[::7].copy()
+    t = ctx.build(pts, leaf_size=4)
+    for r in (1.0, np.sqrt(2.0), 2.0, 3.0):  # many points exactly on the sphere
+        _check_range(*t.range_query(q, float(np.float32(r))), *oracle.range_brute(pts, q, float(np.float32(r))))
+
+
+def test_range_lidar_subset(ctx):
+    pts, q = D.lidar_scan(frames=4, nq=20_000)
+    t = ctx.build(pts)
+    off, ids = t.range_query(q, 0.5)
+    roff, rids = oracle.range_brute(pts, q, 0.5)
+    _check_range(off, ids, roff, rids)
+    assert off[-1] > len(q)  # non-trivial result sizes, including long lists
+
+
+# ---------------------------------------------------------------- batched uneven clouds
+def test_knn_batched_uneven(ctx):
+    pts, poff, q, qoff, kc = D.clustered_batch(n_clouds=24, lo=1e2, hi=2e4, seed=6, ks=(1, 5, 16, 32))
+    t = ctx.build_batched(pts, poff)
+    ids, out_off = t.k_nn_batched(q, qoff, kc)
+    for c in range(24):
+        P = pts[poff[c]:poff[c + 1]]
+        Q = q[qoff[c]:qoff[c + 1]]
+        if len(Q) == 0:
+            continue
+        ref = oracle.knn_brute(P, Q, int(kc[c]))
+        got = ids[out_off[c]:out_off[c + 1]].reshape(len(Q), int(kc[c]))
+        assert np.array_equal(got, ref), c
+
+
+def test_knn_batched_tiny_and_empty_clouds(ctx):
+    rng = np.random.default_rng(2)
+    sizes = [0, 1, 3, 200, 0, 5000]
+    pts = rng.random((sum(sizes), 3), dtype=np.float32)
+    poff = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    qs = [rng.random((n_, 3), dtype=np.float32) for n_ in (4, 0, 6, 50, 3, 100)]
+    qoff = np.concatenate([[0], np.cumsum([len(x) for x in qs])]).astype(np.int64)
+    kc = np.array([5, 5, 5, 8, 2, 13], np.int32)
+    t = ctx.build_batched(pts, poff)
+    ids, out_off = t.k_nn_batched(np.concatenate(qs), qoff, kc)
+    for c in range(6):
+        if len(qs[c]) == 0:
+            continue
+        ref = oracle.knn_brute(pts[poff[c]:poff[c + 1]], qs[c], int(kc[c]))
+        assert np.array_equal(ids[out_off[c]:out_off[c + 1]].reshape(len(qs[c]), -1), ref), c
+
+
+# ---------------------------------------------------------------- Barnes-Hut
+def _rel_err(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
+
+
+def test_bh_theta_zero_is_direct_sum(ctx):
+    pm = D.plummer(20_000, seed=7)
+    acc = ctx.barnes_hut(pm, theta=0.0, eps=1e-3, G=1.0)
+    sample = np.arange(0, 20_000, 97)
+    ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3)
+    err = _rel_err(acc[sample].astype(np.float64), ref)
+    # FP32 inputs and compensated FP32 sums: relative error of a few ulps
+    assert np.max(err) < 2e-5, np.max(err)
+
+
+@pytest.mark.parametrize("theta,tol_med,tol_p99", [(0.3, 2e-3, 2e-2), (0.5, 5e-3, 5e-2), (0.7, 1e-2, 1e-1)])
+def test_bh_error_vs_direct(ctx, theta, tol_med, tol_p99):
+    pm = D.plummer(100_000, seed=8)
+    acc = ctx.barnes_hut(pm, theta=theta, eps=1e-3, G=1.0)
+    sample = np.random.default_rng(1).choice(100_000, 1000, replace=False)
+    err = _rel_err(acc[sample].astype(np.float64), oracle.bh_direct(pm, sample, G=1.0, eps=1e-3))
+    print(f"theta {theta}: median {np.median(err):.2e} p99 {np.quantile(err, 0.99):.2e} max {err.max():.2e}")
+    assert np.median(err) < tol_med and np.quantile(err, 0.99) < tol_p99
+
+
+def test_bh_two_bodies_and_rejects_negative_mass(ctx):
+    pm = np.array([[0, 0, 0, 2.0], [3, 4, 0, 5.0]], np.float32)
+    acc = ctx.barnes_hut(pm, theta=0.5, eps=0.5, G=1.5)
+    ref = oracle.bh_direct(pm, G=1.5, eps=0.5)
+    assert np.allclose(acc, ref, rtol=1e-6)
+    bad = pm.copy()
+    bad[1, 3] = -1
+    with pytest.raises(gprt_hip.GprtError):
+        ctx.barnes_hut(bad)
+
+
+# ---------------------------------------------------------------- the reference FFI
+def _legacy_pipe(program):
+    L = gprt_hip.lib()
+    ir = program.encode()
+    p = L.gprt_pipeline_create(ir, len(ir), 0)
+    assert p, L.gprt_hip_last_error()
+    return L, p
+
+
+def test_legacy_register_heap_knn():
+    L, p = _legacy_pipe("gprt-hip:knn-heap")
+    try:
+        pts, q = D.uniform_cloud(20_000, 2_000, seed=12)
+        r = np.float32(0.02)
+        geom = np.concatenate([pts, np.full((len(pts), 1), r, np.float32)], 1)
+        aabbs = np.concatenate([pts - r, pts + r], 1).astype(np.float32)
+        bvh = L.gprt_bvh_build(p, geom.ctypes.data, aabbs.ctypes.data, len(pts), geom.nbytes)
+        assert bvh
+        k = 5
+        L.gprt_register_array(p, b"out", 2 * k, len(q))
+        qq = np.concatenate([q, np.full((len(q), 1), r, np.float32)], 1)
+        L.gprt_execute(p, bvh, qq.ctypes.data, len(q))
+        lens = np.empty(len(q), np.uint32)
+        flat = np.empty(len(q) * 2 * k, np.uint32)
+        L.gprt_retrieve_array_lengths(p, b"out", lens.ctypes.data, len(q))
+        L.gprt_retrieve_array_flat(p, b"out", flat.ctypes.data, flat.size)
+        flat = flat.reshape(len(q), 2 * k)
+        ids, d2 = oracle.knn_brute(pts, q, k, with_d2=True)
+        r2 = float(r) * float(r)
+        for i in range(len(q)):
+            m = int((d2[i] < r2).sum())
+            assert lens[i] == m
+            assert np.array_equal(flat[i, :m], ids[i, :m])
+            assert np.all(flat[i, m:k] == 0xFFFFFFFF)
+            assert np.array_equal(flat[i, k:k + m].view(np.float32), d2[i, :m].astype(np.float32))
+        L.gprt_bvh_destroy(bvh)
+    finally:
+        L.gprt_pipeline_destroy(p)
+
+
+def test_legacy_push_program():
+    L, p = _legacy_pipe("gprt-hip:push")
+    try:
+        pts, q = D.uniform_cloud(20_000, 1_000, seed=13)
+        r = np.float32(0.03)
+        geom = np.concatenate([pts, np.full((len(pts), 1), r, np.float32)], 1)
+        bvh = L.gprt_bvh_build(p, geom.ctypes.data, None, len(pts), geom.nbytes)
+        cap = 16
+        L.gprt_register_array(p, b"hits", cap, len(q))
+        qq = np.concatenate([q, np.full((len(q), 1), r, np.float32)], 1)
+        L.gprt_execute(p, bvh, qq.ctypes.data, len(q))
+        lens = np.empty(len(q), np.uint32)
+        flat = np.empty(len(q) * cap, np.uint32)
+        L.gprt_retrieve_array_lengths(p, b"hits", lens.ctypes.data, len(q))
+        L.gprt_retrieve_array_flat(p, b"hits", flat.ctypes.data, flat.size)
+        off, ids = oracle.range_brute(pts, q, float(r))
+        assert np.array_equal(lens.astype(np.int64), np.diff(off))
+        flat = flat.reshape(len(q), cap)
+        for i in range(len(q)):
+            m = min(cap, int(off[i + 1] - off[i]))
+            assert np.array_equal(flat[i, :m], ids[off[i]:off[i] + m])
+        L.gprt_bvh_destroy(bvh)
+    finally:
+        L.gprt_pipeline_destroy(p)
+
+
+def test_legacy_autorope_single_leaf_is_direct_sum():
+    L, p = _legacy_pipe("gprt-hip:autorope")
+    try:
+        rng = np.random.default_rng(3)
+        nb = 500
+        body = np.zeros(nb, dtype=[("pos", "<f4", 3), ("mass", "<f4"), ("vel", "<f4", 3)])
+        body["pos"] = rng.normal(size=(nb, 3))
+        body["mass"] = rng.uniform(0.5, 1.5, nb)
+        spatial = np.zeros(1, dtype=[("com", "<f4", 3), ("mass", "<f4")])
+        routing = np.zeros(1, dtype=[("next", "<u4"), ("rope", "<u4"), ("leaf", "<u4"), ("width", "<f4"),
+                                     ("start", "<u4"), ("num", "<u4"), ("pad", "<u4", 2)])
+        routing[0] = (1, 1, 1, 10.0, 0, nb, (0, 0))
+        out = np.zeros((nb, 3), np.float32)
+        L.gprt_execute_autorope_soa_ffi(p, body.ctypes.data, nb, spatial.ctypes.data, 1, routing.ctypes.data, 0.5,
+                                        out.ctypes.data)
+        x = body["pos"].astype(np.float64)
+        m = body["mass"].astype(np.float64)
+        diff = x[None] - x[:, None]
+        d2 = (diff ** 2).sum(-1)
+        w = np.where(d2 > 1e-10, 6.674e-11 * m[:, None] * m[None] / (d2 * np.sqrt(d2 + 1e-6) + (d2 <= 1e-10)), 0)
+        ref = (w[..., None] * diff).sum(1)
+        assert np.allclose(out, ref, rtol=1e-3, atol=1e-16)
+    finally:
+        L.gprt_pipeline_destroy(p)
+
+
+# ---------------------------------------------------------------- autotuner
+def test_autotune_writes_schedule(ctx, tmp_path):
+    pts, q = D.uniform_cloud(60_000, 20_000, seed=14)
+    path = str(tmp_path / "gprt_wisdom.json")
+    cfg, leaf = ctx.autotune(pts, q, 5, budget_trials=12, json_path=path)
+    assert leaf in (4, 8, 16)
+    js = json.load(open(path))
+    assert "radius_increment_mult" in js and "radius_heuristic" in js and js["gprt_hip"]["leaf_size"] == leaf
+    t = ctx.build(pts, leaf_size=leaf)
+    assert np.array_equal(t.k_nn(q, 5, cfg), oracle.knn_brute(pts, q, 5))
