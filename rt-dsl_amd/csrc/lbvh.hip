@@ -132,7 +132,8 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 // (tile, digit) pair.
 template <typename K>
 __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict__ keys, int n, int shift,
-                                                             uint32_t* __restrict__ hist, int nblocks) {
+                                                             uint32_t mask, uint32_t* __restrict__ hist,
+                                                             int nblocks) {
     __shared__ uint32_t h[RS_BINS];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
         int idx = base + i * RS_THREADS + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 0xFFu], 1u);
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & mask], 1u);
     }
     __syncthreads();
     hist[blockIdx.x * RS_BINS + threadIdx.x] = h[threadIdx.x];
@@ -196,7 +197,7 @@ template <typename K>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
                                                                 K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                                int n, int shift,
+                                                                int n, int shift, uint32_t mask,
                                                                 const uint32_t* __restrict__ hist, int nblocks) {
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
     __shared__ uint32_t boff[RS_BINS];
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
         bool valid = idx < n;
         k[i] = valid ? kin[idx] : (K)0;
         v[i] = valid ? vin[idx] : 0u;
-        uint32_t d = (uint32_t)(k[i] >> shift) & 0xFFu;
+        uint32_t d = (uint32_t)(k[i] >> shift) & mask;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
     for (int i = 0; i < RS_ITEMS; ++i) {
         int idx = base + i * 64 + lane;
         if (idx < n) {
-            uint32_t d = (uint32_t)(k[i] >> shift) & 0xFFu;
+            uint32_t d = (uint32_t)(k[i] >> shift) & mask;
             uint32_t pos = boff[d] + wcnt[w][d] + rank[i];
             kout[pos] = k[i];
             vout[pos] = v[i];
@@ -267,11 +268,13 @@ static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_b
     uint32_t* hist = ctx->sort_tmp[2].as<uint32_t>();
     int passes = 0;
     for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
-        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, n, shift, hist,
-                           nblocks);
+        const int nb = std::min(8, end_bit - shift);
+        const uint32_t mask = (1u << nb) - 1u;
+        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, n, shift, mask,
+                           hist, nblocks);
         hipLaunchKernelGGL(rs_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, hist, nblocks);
         hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, v0, k1, v1,
-                           n, shift, hist, nblocks);
+                           n, shift, mask, hist, nblocks);
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
