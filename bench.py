@@ -115,6 +115,20 @@ def main():
     alg_bytes = 16 * r0 + 32 * 2 * cnt["node_visits"] + 16 * cnt["point_tests"] + 4 * K * nq + 16 * cnt["rerank_reads"]
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
 
+    # a device-to-device copy of known size: the reference point for scaling
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes
+    copy_bytes = 64 << 20
+    cbuf_a = gprt_hip.DeviceArray(ctx, (copy_bytes // 4,), np.uint32)
+    cbuf_b = gprt_hip.DeviceArray(ctx, (copy_bytes // 4,), np.uint32)
+    gprt_hip.lib().gprt_hip_copy(ctx.h, cbuf_b.ptr, cbuf_a.ptr, copy_bytes, 2)
+    cbuf_a.free()
+    cbuf_b.free()
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "knn_round0_lidar_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
     # parity of both forms against the oracle (CPU KD-tree, pinned to brute force)
     got_b = ids_b.to_host()
     batched(True)
@@ -159,7 +173,7 @@ def main():
                   "form": "one launch, one query per lane in input order, FP64, r=inf"},
         "speedup_vs_plain": ms_p / ms_b,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "knn round 0 traversal", "kernel_ms": k_ms, "alg_bytes": int(alg_bytes),
                      "node_visits": cnt["node_visits"], "point_tests": cnt["point_tests"],
                      "rerank_reads": cnt["rerank_reads"],

@@ -601,7 +601,7 @@ __device__ __forceinline__ void scan_bucket(Q4& Q, const float4* __restrict__ pt
 // since a child box lies inside its parent's box.  A walk deeper than the
 // stack is finished by the exact FP64 kernel.
 // ---------------------------------------------------------------------------
-template <int C, int G, int STACK, bool COUNT>
+template <int C, int G, int STACK, bool COUNT, bool ROUND0>
 __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
     __shared__ int stk[STACK][256];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
         int status = 0;
         int s = 0;
         if (valid) {
-            s = a.in_list ? a.in_list[i] : (int)i;
+            s = (!ROUND0 && a.in_list) ? a.in_list[i] : (int)i;
             const float4 q = a.qs[s];
             const int c = a.qcloud ? a.qcloud[s] : 0;
             const int k = a.kc ? a.kc[c] : a.k;
@@ -930,14 +930,19 @@ int capacity_class(int kmax, int guard) {
     return 64;
 }
 
+// The first round (all queries, identity work list) gets its own kernel symbol
+// so profiles report it apart from the short later rounds.
 template <int C>
 static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, bool count) {
     if (trav == GPRT_TRAV_WAVE) {
         if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
+    } else if (a.round == 0) {
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false, true>), dim3(grid), dim3(256), 0, s, a);
     } else {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true, false>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false, false>), dim3(grid), dim3(256), 0, s, a);
     }
 }
 

@@ -32,8 +32,11 @@ __global__ void bbox_init_kernel(int* enc, int nclouds) {
     }
 }
 
-// grid: (chunks, clouds).  Each block reduces its slice of its cloud.
-__global__ void bbox_reduce_kernel(const float* __restrict__ pts, const int64_t* __restrict__ off, int* enc) {
+// grid: (chunks, clouds).  Each block reduces its slice of its cloud in LDS
+// and adds one value per bound with an atomic (six atomics per block).
+__global__ __launch_bounds__(256) void bbox_reduce_kernel(const float* __restrict__ pts,
+                                                          const int64_t* __restrict__ off, int* enc) {
+    __shared__ float red[6][4];
     int c = blockIdx.y;
     int64_t b = off[c], e = off[c + 1];
     float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
@@ -46,17 +49,24 @@ __global__ void bbox_reduce_kernel(const float* __restrict__ pts, const int64_t*
             mx[a] = fmaxf(mx[a], v);
         }
     }
-    // wave reduce
     for (int o = 32; o > 0; o >>= 1)
         for (int a = 0; a < 3; ++a) {
             mn[a] = fminf(mn[a], __shfl_xor(mn[a], o));
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], o));
         }
-    if (__lane_id() == 0 && b < e) {
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&enc[6 * c + a], f2ord(mn[a]));
-            atomicMax(&enc[6 * c + 3 + a], f2ord(mx[a]));
+            red[a][w] = mn[a];
+            red[3 + a][w] = mx[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 6 && b < e) {
+        const int a = threadIdx.x;
+        float v = red[a][0];
+        for (int ww = 1; ww < 4; ++ww) v = a < 3 ? fminf(v, red[a][ww]) : fmaxf(v, red[a][ww]);
+        if (a < 3) atomicMin(&enc[6 * c + a], f2ord(v));
+        else atomicMax(&enc[6 * c + a], f2ord(v));
     }
 }
 
@@ -416,7 +426,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     GPRT_HIP_TRY(hipMemcpyAsync(t->cloud_off.ptr, h_off, sizeof(int64_t) * (nclouds + 1), hipMemcpyHostToDevice, s));
     int64_t maxc = 0;
     for (int c = 0; c < nclouds; ++c) maxc = std::max(maxc, h_off[c + 1] - h_off[c]);
-    int chunks = std::max(1, std::min(cdiv(maxc, 256 * 8), 2048));
+    int chunks = std::max(1, std::min(cdiv(maxc, 256 * 16), 512));
 
     hipLaunchKernelGGL(bbox_init_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, enc, nclouds);
     hipLaunchKernelGGL(bbox_reduce_kernel, dim3(chunks, nclouds), dim3(256), 0, s, d_pts,
