@@ -91,7 +91,7 @@ typedef struct {
     int radius_heuristic; /* GPRT_RADIUS_*, default PERCENTILE */
     float radius_param;   /* percentile p (default 0.95) or the fixed radius */
     float radius_mult;    /* radius growth per round (default 3.0) */
-    int max_rounds;       /* bounded rounds before the final unbounded one (default 6) */
+    int max_rounds;       /* bounded rounds before the final unbounded one (default 2) */
     int traversal;        /* GPRT_TRAV_*, default THREAD */
     int stack_depth;      /* LDS traversal-stack entries per query (default 16) */
     int guard_slots;      /* spare candidate slots beyond k (default 3) */

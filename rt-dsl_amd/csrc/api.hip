@@ -89,8 +89,8 @@ static void drop_extra(Ctx* c) {
                               &e->r_off, &e->bh_xyz, &e->bh_mass, &e->bh_acc})
                 b->release();
             Tree& T = e->bh_tree;
-            for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.perm, &T.inv, &T.codes,
-                              &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
+            for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm,
+                              &T.inv, &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
                 b->release();
             delete e;
             g_extra.erase(g_extra.begin() + i);
@@ -117,9 +117,19 @@ static void launch_validate(hipStream_t s, const float* v, int64_t m, unsigned* 
     hipLaunchKernelGGL(validate_kernel, dim3(blocks), dim3(256), 0, s, v, m, flag);
 }
 
-__global__ void set_qoff_kernel(int64_t* qoff, int64_t nq) {
-    qoff[0] = 0;
-    qoff[1] = nq;
+// before each k class: its work counters restart and its round 0 holds `size`
+__global__ void class_init_kernel(unsigned* counters, unsigned size) {
+    const int t = threadIdx.x;
+    if (t <= CNT_FALLBACK) counters[t] = t == CNT_ACTIVE ? size : 0u;
+}
+
+// clears the call's counters and, for one cloud, writes its query offsets
+__global__ void init_call_kernel(unsigned* counters, int n, int64_t* qoff, int64_t nq) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0u;
+    if (qoff && threadIdx.x == 0) {
+        qoff[0] = 0;
+        qoff[1] = nq;
+    }
 }
 
 int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
@@ -146,7 +156,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->radius_heuristic = GPRT_RADIUS_PERCENTILE;
     c->radius_param = 0.95f;
     c->radius_mult = 3.0f;
-    c->max_rounds = 6;
+    c->max_rounds = 2;
     c->traversal = GPRT_TRAV_THREAD;
     c->stack_depth = 16;
     c->guard_slots = 3;
@@ -318,8 +328,8 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
     if (!t) return GPRT_OK;
     if (t->owner) (void)hipSetDevice(t->owner->c.device);
     Tree& T = t->t;
-    for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.perm, &T.inv, &T.codes,
-                      &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
+    for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.inv,
+                      &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
         b->release();
     delete t;
     return GPRT_OK;
@@ -370,12 +380,32 @@ struct KnnPlan {
     gprt_hip_config cfg;
     uint32_t* d_ids;
     double* d_d2;
+    // k classes (sorted order groups queries by class)
+    int class_begin[N_KCLASS];
+    int class_size[N_KCLASS];
+    int class_kmax[N_KCLASS];
 };
+
+// fills the k-class layout of a plan from host-side per-cloud k and query counts
+static void plan_classes(KnnPlan& p, const int* h_k, const int64_t* h_qoff, int nclouds, int k_uniform) {
+    for (int c = 0; c < N_KCLASS; ++c) p.class_size[c] = p.class_kmax[c] = 0;
+    for (int c = 0; c < nclouds; ++c) {
+        const int k = h_k ? h_k[c] : k_uniform;
+        const int cl = k_class(k);
+        p.class_size[cl] += (int)(h_qoff[c + 1] - h_qoff[c]);
+        p.class_kmax[cl] = std::max(p.class_kmax[cl], k);
+    }
+    int b = 0;
+    for (int c = 0; c < N_KCLASS; ++c) {
+        p.class_begin[c] = b;
+        b += p.class_size[c];
+    }
+}
 
 static int knn_reserve(Ctx& c, const KnnPlan& p) {
     size_t nq = (size_t)std::max<int64_t>(p.nq, 1);
     GPRT_TRY(c.ws[WS_QKEYS].reserve(8 * nq));
-    GPRT_TRY(c.ws[WS_QPERM].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_QPERM].reserve(std::max(4 * nq, query_prep_scratch_bytes(p.nclouds, p.cfg.sort_queries))));
     GPRT_TRY(c.ws[WS_QS4].reserve(16 * nq));
     GPRT_TRY(c.ws[WS_LISTA].reserve(4 * nq));
     GPRT_TRY(c.ws[WS_LISTB].reserve(4 * nq));
@@ -384,10 +414,9 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     CtxExtra* x = extra(&c);
     if (p.nclouds > 1) GPRT_TRY(x->qcloud.reserve(4 * nq));
     // radix sort temporaries for nq keys
-    size_t nblocks = (nq + 4095) / 4096;
     GPRT_TRY(c.sort_tmp[0].reserve(8 * nq));
     GPRT_TRY(c.sort_tmp[1].reserve(4 * nq));
-    GPRT_TRY(c.sort_tmp[2].reserve(4 * 256 * nblocks + 1024));
+    GPRT_TRY(c.sort_tmp[2].reserve(radix_sort_hist_bytes((int)nq)));
     return GPRT_OK;
 }
 
@@ -401,22 +430,19 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     float* radius = kth + 100 * p.nclouds;
     int nq = (int)p.nq;
     CtxExtra* x = extra(&c);
-    GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
-    if (p.nclouds == 1) hipLaunchKernelGGL(set_qoff_kernel, dim3(1), dim3(1), 0, s, (int64_t*)p.d_qoff, p.nq);
-    launch_validate(s, p.d_q, 3 * p.nq, counters + CNT_TOTAL - 1);
+    hipLaunchKernelGGL(init_call_kernel, dim3(1), dim3(64), 0, s, counters, CNT_TOTAL + 2 * CNT_STATS64,
+                       p.nclouds == 1 ? (int64_t*)p.d_qoff : nullptr, p.nq);
     GPRT_TRY(launch_sampler(s, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(), T.cloud_off.as<int64_t>(),
                             p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic, cfg.radius_param, kth, radius));
     uint64_t* keys = c.ws[WS_QKEYS].as<uint64_t>();
     uint32_t* perm = c.ws[WS_QPERM].as<uint32_t>();
     float4* qs4 = c.ws[WS_QS4].as<float4>();
-    GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.cloud_bbox.as<float>(), cfg.sort_queries, nq, keys,
-                               perm, qs4, counters, p.max_q_per_cloud));
-    int* qcloud = nullptr;
-    if (p.nclouds > 1) {
-        qcloud = x->qcloud.as<int>();
-        GPRT_TRY(launch_key_cloud(s, keys, nq, qcloud));
-    }
-    int cap = capacity_class(p.kmax, cfg.guard_slots);
+    int* qcloud = p.nclouds > 1 ? x->qcloud.as<int>() : nullptr;
+    int nclasses = 0;
+    for (int cl = 0; cl < N_KCLASS; ++cl) nclasses += p.class_size[cl] > 0;
+    GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
+                               perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
+                               p.d_kc, p.k, nclasses > 1));
     int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : c.num_cus * 4;
     int R = cfg.radius_heuristic == GPRT_RADIUS_INFINITE ? 1 : std::min(cfg.max_rounds, MAX_ROUNDS - 1) + 1;
     int* lists[2] = {c.ws[WS_LISTA].as<int>(), c.ws[WS_LISTB].as<int>()};
@@ -438,19 +464,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.out_base = p.d_outbase;
     ra.qoff = p.d_qoff;
     ra.max_steps = T.n + 64;
-    const bool time_k = cfg.time_kernels && !cfg.use_graph;
-    for (int r = 0; r < R; ++r) {
-        ra.round = r;
-        ra.last = (r == R - 1);
-        ra.in_list = r == 0 ? nullptr : lists[r & 1];
-        ra.out_list = lists[(r + 1) & 1];
-        if (r == 0 && time_k) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
-        GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.collect_counters != 0));
-        if (r == 0 && time_k) GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
-    }
-    c.kernel_timed = time_k;
     ExactArgs ea{};
-    ea.mode = 1;
     ea.nq = p.nq;
     ea.qs = qs4;
     ea.fb_list = c.ws[WS_FB].as<int>();
@@ -467,7 +481,42 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ea.out_base = p.d_outbase;
     ea.qoff = p.d_qoff;
     ea.max_steps = T.n + 64;
-    GPRT_TRY(launch_exact(s, ea, p.kmax, nq, cfg.collect_counters != 0));
+    const bool time_k = cfg.time_kernels && !cfg.use_graph;
+    bool timed = false;
+    for (int cl = 0; cl < N_KCLASS; ++cl) {
+        const int size = p.class_size[cl];
+        if (size == 0) continue;
+        if (cl == N_KCLASS - 1) {
+            // k too large for the FP32 candidate list: exact FP64 walk directly
+            ea.mode = 2;
+            ea.base = p.class_begin[cl];
+            ea.nq = size;
+            GPRT_TRY(launch_exact(s, ea, p.class_kmax[cl], size, cfg.collect_counters != 0));
+            continue;
+        }
+        if (nclasses > 1)
+            hipLaunchKernelGGL(class_init_kernel, dim3(1), dim3(64), 0, s, counters, (unsigned)size);
+        const int cap = k_class_cap(cl);
+        ra.base = p.class_begin[cl];
+        for (int r = 0; r < R; ++r) {
+            ra.round = r;
+            ra.last = (r == R - 1);
+            ra.in_list = r == 0 ? nullptr : lists[r & 1];
+            ra.out_list = lists[(r + 1) & 1];
+            const bool tk = r == 0 && time_k && !timed;
+            if (tk) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
+            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.collect_counters != 0));
+            if (tk) {
+                GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
+                timed = true;
+            }
+        }
+        ea.mode = 1;
+        ea.base = 0;
+        ea.nq = size;
+        GPRT_TRY(launch_exact(s, ea, p.class_kmax[cl], size, cfg.collect_counters != 0));
+    }
+    c.kernel_timed = timed;
     return GPRT_OK;
 }
 
@@ -484,7 +533,7 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
         // everything the captured kernels point at: a rebuilt tree or a grown
         // workspace changes a pointer and forces a new capture
         const Tree& T = *p.tree;
-        const void* ptrs[] = {T.nodes.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.cloud_bbox.ptr, T.cloud_off.ptr,
+        const void* ptrs[] = {T.nodes.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
                               c.ws[0].ptr, c.ws[1].ptr, c.ws[2].ptr, c.ws[3].ptr, c.ws[4].ptr, c.ws[5].ptr,
                               c.ws[6].ptr, c.sort_tmp[0].ptr, c.sort_tmp[1].ptr, c.sort_tmp[2].ptr,
                               c.counters.ptr, extra(&c)->qcloud.ptr};
@@ -563,7 +612,7 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters) {
 extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
                             const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int flags) {
     if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
-    if (k < 1 || k > 61) return fail(GPRT_ERR_ARG, "k must be in [1, 61]");
+    if (k < 1 || k > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
     if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "tree has several clouds: use gprt_hip_knn_batched");
@@ -571,7 +620,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
     GPRT_HIP_TRY(hipSetDevice(c.device));
     gprt_hip_config cf;
     if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
-    if (cf.guard_slots < 1 || k + cf.guard_slots > 64) return fail(GPRT_ERR_ARG, "k + guard_slots must be <= 64");
+
     if (cf.max_rounds < 0) return fail(GPRT_ERR_ARG, "max_rounds < 0");
     c.stats = gprt_hip_run_stats{};
     if (nq == 0) return GPRT_OK;
@@ -605,6 +654,10 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
     p.cfg = cf;
     p.d_ids = d_ids;
     p.d_d2 = d_d2;
+    {
+        int64_t qo[2] = {0, nq};
+        plan_classes(p, nullptr, qo, 1, k);
+    }
     GPRT_TRY(knn_run(c, p));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
@@ -681,7 +734,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     int64_t maxq = 0;
     for (int c = 0; c < nc; ++c) {
         if (query_offsets[c + 1] < query_offsets[c]) return fail(GPRT_ERR_ARG, "query_offsets must be non-decreasing");
-        if (k_per_cloud[c] < 1 || k_per_cloud[c] > 61) return fail(GPRT_ERR_ARG, "k must be in [1, 61]");
+        if (k_per_cloud[c] < 1 || k_per_cloud[c] > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
         kmax = std::max(kmax, k_per_cloud[c]);
         outbase[c + 1] = outbase[c] + (query_offsets[c + 1] - query_offsets[c]) * k_per_cloud[c];
         maxq = std::max(maxq, query_offsets[c + 1] - query_offsets[c]);
@@ -693,7 +746,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     GPRT_HIP_TRY(hipSetDevice(c.device));
     gprt_hip_config cf;
     if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
-    if (cf.guard_slots < 1 || kmax + cf.guard_slots > 64) return fail(GPRT_ERR_ARG, "k + guard_slots must be <= 64");
+
     c.stats = gprt_hip_run_stats{};
     if (nq == 0) return GPRT_OK;
     CtxExtra* x = extra(&c);
@@ -731,6 +784,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     p.cfg = cf;
     p.d_ids = d_ids;
     p.d_d2 = d_d2;
+    plan_classes(p, k_per_cloud, query_offsets, nc, 0);
     GPRT_TRY(knn_run(c, p));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nout, hipMemcpyDeviceToHost, s));

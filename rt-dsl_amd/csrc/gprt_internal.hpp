@@ -75,6 +75,7 @@ struct Tree {
     DevBuf roots;       // int2[nclouds]: (ref, count) of each cloud's root
     DevBuf cloud_off;   // int64[nclouds+1] point offsets per cloud (device)
     DevBuf cloud_bbox;  // float[6*nclouds]: per-cloud (min xyz, max xyz)
+    DevBuf key_bbox;    // float[6*nclouds]: box the sort keys are normalised to
     DevBuf perm;        // uint32[n]: sorted position -> input index
     DevBuf inv;         // uint32[n]: input index -> sorted position
     DevBuf codes;       // uint64[n]: sorted keys (cloud << 32 | morton)
@@ -152,6 +153,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int ncl
 int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
 int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
+size_t radix_sort_hist_bytes(int n);
 
 // the runtime context
 struct Ctx {

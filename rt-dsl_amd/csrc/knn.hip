@@ -71,41 +71,58 @@ __device__ __forceinline__ void ins4(float* d, float v) {
     }
 }
 
-// grid (100, nclouds), 256 threads.  kth[c*100 + s] = sqrt of the 4th smallest
-// sampled d^2 of sample slot s, or -1 when the reference would not push it.
-__global__ __launch_bounds__(256) void sampler_lcg_kernel(const float4* __restrict__ pts4,
-                                                          const uint32_t* __restrict__ inv,
-                                                          const int64_t* __restrict__ poff,
-                                                          const float* __restrict__ qs,
-                                                          const int64_t* __restrict__ qoff, float* kth) {
+// grid (100, nclouds), 1024 threads.  kth[c*100 + s] = sqrt of the 4th
+// smallest sampled d^2 of sample slot s, or -1 when the reference would not
+// push it.  Each lane jumps straight to its share of the draw sequence, issues
+// the point loads of its (at most five) draws together, then keeps its four
+// smallest values; lanes, then waves, merge their lists.
+constexpr int SAMPLER_THREADS = 1024;
+__global__ __launch_bounds__(SAMPLER_THREADS) void sampler_lcg_kernel(const float4* __restrict__ pts4,
+                                                                      const uint32_t* __restrict__ inv,
+                                                                      const int64_t* __restrict__ poff,
+                                                                      const float* __restrict__ qs,
+                                                                      const int64_t* __restrict__ qoff, float* kth) {
     const int c = blockIdx.y, s = blockIdx.x;
     const int64_t n = poff[c + 1] - poff[c], nq = qoff[c + 1] - qoff[c];
     const int64_t S = n < 5000 ? n : 5000, qss = nq < 100 ? nq : 100;
-    __shared__ float part[4][4];
+    __shared__ float part[SAMPLER_THREADS / 64][4];
     if (s >= qss || n <= 0) {
         if (threadIdx.x == 0) kth[c * 100 + s] = -1.0f;
         return;
     }
-    // step index (1-based) of this slot's query draw: 1 + s*(S+1)
+    // step index (1-based) of this slot's query draw: 1 + s*(S+1); draws are
+    // < 2^31 and n, nq < 2^31, so the modulo is exact in 32 bits
     const uint64_t qstep = 1ull + (uint64_t)s * (uint64_t)(S + 1);
     const uint64_t qstate = lcg_jump(LCG_SEED, qstep);
-    const int64_t qi = (int64_t)((qstate >> 33) % (uint64_t)nq);
+    const int64_t qi = (int64_t)((uint32_t)(qstate >> 33) % (uint32_t)nq);
     float q[3] = {qs[3 * (qoff[c] + qi)], qs[3 * (qoff[c] + qi) + 1], qs[3 * (qoff[c] + qi) + 2]};
-    const int64_t chunk = (S + 255) / 256;
-    int64_t t0 = threadIdx.x * chunk, t1 = t0 + chunk < S ? t0 + chunk : S;
+    constexpr int PER = 5;  // 1024 lanes x 5 >= 5000 draws
+    const int64_t t0 = (int64_t)threadIdx.x * PER;
     float d[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
-    if (t0 < t1) {
+    if (t0 < S) {
         uint64_t st = lcg_jump(qstate, (uint64_t)t0);
-        for (int64_t t = t0; t < t1; ++t) {
+        uint32_t idx[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
             st = st * LCG_A + LCG_C;
-            int64_t pi = (int64_t)((st >> 33) % (uint64_t)n);
-            const float4 p4 = pts4[inv[poff[c] + pi]];
-            const float p[3] = {p4.x, p4.y, p4.z};
-            float dd = d2_sampler(q, p);
-            if (dd < INFINITY) ins4(d, dd);
+            idx[u] = (uint32_t)(st >> 33) % (uint32_t)n;
         }
+        uint32_t sp[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (t0 + u < S) sp[u] = inv[poff[c] + idx[u]];
+        float4 p4[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (t0 + u < S) p4[u] = pts4[sp[u]];
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (t0 + u < S) {
+                const float p[3] = {p4[u].x, p4[u].y, p4[u].z};
+                const float dd = d2_sampler(q, p);
+                if (dd < INFINITY) ins4(d, dd);
+            }
     }
-    // wave merge, then across the 4 waves
     for (int o = 32; o > 0; o >>= 1) {
         float b[4];
 #pragma unroll
@@ -117,7 +134,7 @@ __global__ __launch_bounds__(256) void sampler_lcg_kernel(const float4* __restri
         for (int i = 0; i < 4; ++i) part[w][i] = d[i];
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int ww = 1; ww < 4; ++ww) merge4(d, part[ww]);
+        for (int ww = 1; ww < SAMPLER_THREADS / 64; ++ww) merge4(d, part[ww]);
         // the reference pushes dists[3] when it is not f32::MAX; its entries
         // are sqrt'd, and sqrt is monotone, so the 4th sqrt is sqrt(4th d^2)
         kth[c * 100 + s] = (d[3] < FLT_MAX) ? sqrtf(d[3]) : -1.0f;
@@ -240,37 +257,6 @@ __device__ __forceinline__ uint32_t quant10(float p, float mn, float ext) {
     float t = (p - mn) / ext * 1023.0f;
     uint32_t u = !(t > 0.0f) ? 0u : (t >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)t);
     return u < 1023u ? u : 1023u;
-}
-
-// grid (chunks, clouds): key = cloud << 32 | morton of the query in its cloud's
-// box (queries outside the box saturate to the boundary cells).
-__global__ void query_keys_kernel(const float* __restrict__ qs, const int64_t* __restrict__ qoff,
-                                  const float* __restrict__ bbox, int use_morton, uint64_t* __restrict__ keys,
-                                  uint32_t* __restrict__ vals) {
-    int c = blockIdx.y;
-    int64_t b = qoff[c], e = qoff[c + 1];
-    float bb[6];
-    for (int a = 0; a < 6; ++a) bb[a] = bbox[6 * c + a];
-    float ex = fmaxf(bb[3] - bb[0], 1e-6f), ey = fmaxf(bb[4] - bb[1], 1e-6f), ez = fmaxf(bb[5] - bb[2], 1e-6f);
-    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t m = 0;
-        if (use_morton)
-            m = (expand10(quant10(qs[3 * i], bb[0], ex)) << 2) | (expand10(quant10(qs[3 * i + 1], bb[1], ey)) << 1) |
-                expand10(quant10(qs[3 * i + 2], bb[2], ez));
-        keys[i] = ((uint64_t)c << 32) | m;
-        vals[i] = (uint32_t)i;
-    }
-}
-
-__global__ void query_gather_kernel(const float* __restrict__ qs, const uint32_t* __restrict__ perm, int nq,
-                                    float4* __restrict__ out, unsigned* counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nq) {
-        uint32_t g = perm[i];
-        out[i] = make_float4(qs[3 * g], qs[3 * g + 1], qs[3 * g + 2], __int_as_float((int)g));
-    }
-    if (i == 0) counters[CNT_ACTIVE] = (unsigned)nq;
 }
 
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
@@ -455,7 +441,7 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
         if (base >= n_in) break;
         const unsigned i = base + lane;
         const bool valid = i < n_in;
-        const int s = valid ? (a.in_list ? a.in_list[i] : (int)i) : 0;
+        const int s = valid ? (a.in_list ? a.in_list[i] : a.base + (int)i) : 0;
         const float4 q = valid ? a.qs[s] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int c = valid && a.qcloud ? a.qcloud[s] : 0;
         const int k = a.kc ? a.kc[c] : a.k;
@@ -620,7 +606,7 @@ __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
         int status = 0;
         int s = 0;
         if (valid) {
-            s = (!ROUND0 && a.in_list) ? a.in_list[i] : (int)i;
+            s = (!ROUND0 && a.in_list) ? a.in_list[i] : a.base + (int)i;
             const float4 q = a.qs[s];
             const int c = a.qcloud ? a.qcloud[s] : 0;
             const int k = a.kc ? a.kc[c] : a.k;
@@ -826,10 +812,11 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
 
 // mode 0: plain form, query i of the caller's xyz array (input order)
 // mode 1: fallback, sorted position fb_list[i]
+// mode 2: sorted positions base + i (a k class too large for the FP32 list)
 template <int KMAX, bool COUNT>
 __global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned n = a.mode == 0 ? (unsigned)a.nq : a.counters[CNT_FALLBACK];
+    const unsigned n = a.mode == 1 ? a.counters[CNT_FALLBACK] : (unsigned)a.nq;
     if ((unsigned)i >= n) return;
     float qx, qy, qz;
     int g, c = 0;
@@ -837,7 +824,7 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
         qx = a.q_xyz[3 * (int64_t)i]; qy = a.q_xyz[3 * (int64_t)i + 1]; qz = a.q_xyz[3 * (int64_t)i + 2];
         g = i;
     } else {
-        const int s = a.fb_list[i];
+        const int s = a.mode == 1 ? a.fb_list[i] : a.base + i;
         const float4 q = a.qs[s];
         qx = q.x; qy = q.y; qz = q.z;
         g = __float_as_int(q.w);
@@ -871,8 +858,8 @@ int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_
                    float* radius) {
     if (heuristic != GPRT_RADIUS_INFINITE && !(sampler == GPRT_SAMPLER_STRIDED && heuristic != GPRT_RADIUS_PERCENTILE)) {
         if (sampler == GPRT_SAMPLER_LCG)
-            hipLaunchKernelGGL(sampler_lcg_kernel, dim3(100, nclouds), dim3(256), 0, s, pts4, inv, poff, qs, qoff,
-                               kth);
+            hipLaunchKernelGGL(sampler_lcg_kernel, dim3(100, nclouds), dim3(SAMPLER_THREADS), 0, s, pts4, inv, poff,
+                               qs, qoff, kth);
         else
             hipLaunchKernelGGL(sampler_strided_kernel, dim3(100, nclouds), dim3(256), 0, s, pts4, inv, poff, qs, qoff,
                                k, kth);
@@ -889,18 +876,83 @@ static int g_query_sort_bits = 24;
 int query_sort_bits() { return g_query_sort_bits; }
 extern "C" GPRT_HIP_API void gprt_hip_debug_set_query_sort_bits(int b) { g_query_sort_bits = b < 8 ? 8 : (b > 30 ? 30 : b); }
 
+// ---------------------------------------------------------------------------
+// Query ordering: a 32-bit key = k class (3 bits) | cloud (cb bits) | the top
+// 29 - cb bits of the 30-bit Morton code; stable radix sort of the class, the
+// cloud and query_sort_bits() Morton bits, then a gather into float4 (input
+// index in .w).  Queries end up grouped by k class, then by cloud, then in
+// Morton order.  Input validation is folded into the key pass.
+// ---------------------------------------------------------------------------
+__global__ void query_key32_kernel(const float* __restrict__ qs, const int64_t* __restrict__ qoff,
+                                   const float* __restrict__ bbox, int use_morton, int cb,
+                                   const int* __restrict__ kc, int k_uniform, uint32_t* __restrict__ keys,
+                                   uint32_t* __restrict__ vals, unsigned* flag) {
+    const int c = blockIdx.y;
+    const int64_t b = qoff[c], e = qoff[c + 1];
+    float bb[6];
+    for (int a = 0; a < 6; ++a) bb[a] = bbox[6 * c + a];
+    const float ex = fmaxf(bb[3] - bb[0], 1e-6f), ey = fmaxf(bb[4] - bb[1], 1e-6f), ez = fmaxf(bb[5] - bb[2], 1e-6f);
+    const uint32_t cls = (uint32_t)k_class(kc ? kc[c] : k_uniform);
+    const int mbits = 29 - cb;
+    const uint32_t head = (cls << 29) | ((uint32_t)c << mbits);
+    bool bad = false;
+    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float x = qs[3 * i], y = qs[3 * i + 1], z = qs[3 * i + 2];
+        bad |= !(fabsf(x) <= 1e18f && fabsf(y) <= 1e18f && fabsf(z) <= 1e18f);
+        uint32_t m = 0;
+        if (use_morton)
+            m = (expand10(quant10(x, bb[0], ex)) << 2) | (expand10(quant10(y, bb[1], ey)) << 1) |
+                expand10(quant10(z, bb[2], ez));
+        keys[i] = head | (m >> (30 - mbits));
+        vals[i] = (uint32_t)i;
+    }
+    if (__ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
+}
+
+__global__ void query_gather_kernel(const float* __restrict__ qs, const uint32_t* __restrict__ perm,
+                                    const uint32_t* __restrict__ keys, int cb, int nq, float4* __restrict__ out,
+                                    int* __restrict__ qcloud, unsigned* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        uint32_t g = perm[i];
+        out[i] = make_float4(qs[3 * g], qs[3 * g + 1], qs[3 * g + 2], __int_as_float((int)g));
+        if (qcloud) qcloud[i] = (int)((keys[i] >> (29 - cb)) & ((1u << cb) - 1u));
+    }
+    if (i == 0) counters[CNT_ACTIVE] = (unsigned)nq;
+}
+
+static int cloud_bits_of(int nclouds) {
+    int cb = 0;
+    while ((1 << cb) < nclouds) ++cb;
+    return cb;
+}
+
+size_t query_prep_scratch_bytes(int nclouds, int use_morton) {
+    (void)nclouds;
+    (void)use_morton;
+    return 0;
+}
+
 int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
-                      int64_t max_per_cloud) {
+                      int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc,
+                      int k_uniform, int sort_classes) {
+    (void)scratch;
     hipStream_t s = ctx->stream;
-    int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
-    hipLaunchKernelGGL(query_keys_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, keys,
-                       perm);
-    int cloud_bits = 0;
-    while ((1 << cloud_bits) < nclouds) ++cloud_bits;
-    if (use_morton) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 30 - query_sort_bits(), 30));
-    if (cloud_bits > 0) GPRT_TRY(radix_sort_pairs_u64(ctx, keys, perm, nq, 32, 32 + cloud_bits));
-    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, nq, qs4, counters);
+    const int cb = cloud_bits_of(nclouds);
+    if (cb > 20) return fail(GPRT_ERR_ARG, "too many clouds (max 2^20)");
+    const int mbits = 29 - cb;
+    const int qbits = use_morton ? std::min(query_sort_bits(), mbits) : 0;
+    const int begin = mbits - qbits;
+    uint32_t* k32 = (uint32_t*)keys;
+    const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
+    hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb, kc,
+                       k_uniform, k32, perm, flag);
+    // the class bits (29..31) only need sorting when several k classes are present
+    GPRT_TRY(radix_sort_pairs_u32(ctx, k32, perm, nq, begin, sort_classes ? 32 : 29));
+    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, k32, cb, nq, qs4,
+                       qcloud, counters);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
 }
 

@@ -41,10 +41,13 @@ struct RoundArgs {
     const int64_t* out_base; // per cloud output offset (nullptr: g * k)
     const int64_t* qoff;     // per cloud first global query index
     int max_steps;           // node-visit guard (> number of internal nodes)
+    int base;                // first sorted position of this k class (identity work list)
 };
 
 struct ExactArgs {
-    int mode;                // 0 plain (caller's xyz array, input order), 1 fallback list
+    int mode;                // 0 plain (caller's xyz array, input order), 1 fallback list,
+                             // 2 sorted positions [base, base + nq)
+    int base;
     int64_t nq;
     const float* q_xyz;
     const float4* qs;
@@ -67,9 +70,20 @@ struct ExactArgs {
 int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_t* inv, const int64_t* poff,
                    const float* qs,
                    const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth, float* radius);
+// queries -> Morton-bucket order (float4 with the input index in .w); keys is
+// 8 B of scratch per query, scratch holds query_prep_scratch_bytes(); flag
+// receives the input-validation result
 int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
-                      int64_t max_per_cloud);
+                      int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc = nullptr,
+                      int k_uniform = 1, int sort_classes = 0);
+// k classes: queries of one class share a candidate-list capacity.  Classes
+// 0..3 run the FP32 rounds with capacity 8/16/32/64 (k + 3 guard slots);
+// class 4 (k > 61) goes straight to the exact FP64 kernel.
+constexpr int N_KCLASS = 5;
+__host__ __device__ inline int k_class(int k) { return k <= 5 ? 0 : (k <= 13 ? 1 : (k <= 29 ? 2 : (k <= 61 ? 3 : 4))); }
+__host__ __device__ inline int k_class_cap(int cls) { return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 32 : 64)); }
+size_t query_prep_scratch_bytes(int nclouds, int use_morton);
 int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);

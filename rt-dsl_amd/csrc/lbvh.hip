@@ -70,6 +70,22 @@ __global__ __launch_bounds__(256) void bbox_reduce_kernel(const float* __restric
     }
 }
 
+// Box used to normalise the sort keys.  mode 0: the cloud's own box (per-axis
+// normalisation, morton.rs:37-47); mode 1: a cube of the largest extent, so a
+// flat cloud (a LiDAR scene is ~240 x 240 x 6 m) is not cut along its thin axis
+// as often as along its wide ones.  Keys only order the points: the search
+// results do not depend on this choice.
+__global__ void key_box_kernel(const float* __restrict__ bbox, int nclouds, int mode, float* __restrict__ kb) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclouds) return;
+    const float* b = bbox + 6 * c;
+    float e = fmaxf(b[3] - b[0], fmaxf(b[4] - b[1], b[5] - b[2]));
+    for (int a = 0; a < 3; ++a) {
+        kb[6 * c + a] = b[a];
+        kb[6 * c + 3 + a] = mode == 1 ? b[a] + e : b[3 + a];
+    }
+}
+
 __global__ void bbox_decode_kernel(const int* enc, float* bbox, int nclouds) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < 6 * nclouds) bbox[i] = ord2f(enc[i]);
@@ -132,7 +148,7 @@ __global__ void morton_codes_kernel(const float* __restrict__ pts, int n, const 
 // stable LSD radix sort, 8-bit digits, 4096-key tiles (256 lanes x 16 keys)
 // ---------------------------------------------------------------------------
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
@@ -186,14 +202,15 @@ __global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nb
         dtot[d] = t;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (int dd = 0; dd < RS_BINS; ++dd) {
-            uint32_t v = dtot[dd];
-            dtot[dd] = t;
-            t += v;
-        }
+    // exclusive scan of the 256 digit totals (Hillis-Steele, first 256 lanes)
+    uint32_t tot = g == 0 ? dtot[d] : 0u;
+    for (int o = 1; o < RS_BINS; o <<= 1) {
+        uint32_t v = (g == 0 && d >= o) ? dtot[d - o] : 0u;
+        __syncthreads();
+        if (g == 0) dtot[d] += v;
+        __syncthreads();
     }
+    if (g == 0) dtot[d] -= tot;
     __syncthreads();
     run = dtot[d] + gsum[g][d];
     for (b = b0; b < b1; ++b) {
@@ -296,6 +313,8 @@ static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_b
     return GPRT_OK;
 }
 
+size_t radix_sort_hist_bytes(int n) { return sizeof(uint32_t) * (size_t)RS_BINS * ((n + RS_TILE - 1) / RS_TILE) + 1024; }
+
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
     return radix_sort_impl<uint64_t>(ctx, keys, vals, n, begin_bit, end_bit);
 }
@@ -394,6 +413,9 @@ __global__ void fill_int_kernel(int* p, int n, int v) {
 // ---------------------------------------------------------------------------
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
+static int g_key_mode = 1;
+extern "C" GPRT_HIP_API void gprt_hip_debug_set_key_mode(int m) { g_key_mode = m ? 1 : 0; }
+
 int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, int leaf_size, Tree* t) {
     int64_t n64 = h_off[nclouds];
     if (n64 > (int64_t)0x7FFFFFF0) return fail(GPRT_ERR_ARG, "too many points for one tree");
@@ -410,6 +432,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
 
     GPRT_TRY(t->cloud_off.reserve(sizeof(int64_t) * (nclouds + 1)));
     GPRT_TRY(t->cloud_bbox.reserve(sizeof(float) * 6 * nclouds));
+    GPRT_TRY(t->key_bbox.reserve(sizeof(float) * 6 * nclouds));
     GPRT_TRY(t->roots.reserve(sizeof(int2) * nclouds));
     GPRT_TRY(t->pts4.reserve(sizeof(float4) * std::max(n, 1)));
     GPRT_TRY(t->nodes.reserve(sizeof(Node) * std::max(n - 1, 1)));
@@ -433,11 +456,13 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
                        t->cloud_off.as<int64_t>(), enc);
     hipLaunchKernelGGL(bbox_decode_kernel, dim3(cdiv(6 * nclouds, 256)), dim3(256), 0, s, enc,
                        t->cloud_bbox.as<float>(), nclouds);
+    hipLaunchKernelGGL(key_box_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, t->cloud_bbox.as<float>(),
+                       nclouds, g_key_mode, t->key_bbox.as<float>());
     hipLaunchKernelGGL(roots_small_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, t->cloud_off.as<int64_t>(),
                        nclouds, leaf_size, t->roots.as<int2>());
     if (n > 0) {
         hipLaunchKernelGGL(morton_keys_kernel, dim3(std::max(1, std::min(cdiv(maxc, 256), 4096)), nclouds),
-                           dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->cloud_bbox.as<float>(),
+                           dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->key_bbox.as<float>(),
                            t->codes.as<uint64_t>(), t->perm.as<uint32_t>());
         GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 0, 30));
         if (cloud_bits > 0)

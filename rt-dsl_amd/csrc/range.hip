@@ -252,7 +252,7 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     hipStream_t s = ctx->stream;
     unsigned* counters = ctx->counters.as<unsigned>();
     GPRT_TRY(keys_buf.reserve(8ull * nq));
-    GPRT_TRY(perm_buf.reserve(4ull * nq));
+    GPRT_TRY(perm_buf.reserve(std::max<size_t>(4ull * nq, query_prep_scratch_bytes(1, 1))));
     GPRT_TRY(qs4_buf.reserve(16ull * nq));
     GPRT_TRY(counts_buf.reserve(8ull * nq + 8));
     const int tiles = (nq + SCAN_TILE - 1) / SCAN_TILE;
@@ -262,8 +262,9 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     int64_t h_qoff[2] = {0, nq};
     GPRT_HIP_TRY(hipMemcpyAsync(qoff, h_qoff, 16, hipMemcpyHostToDevice, s));
     GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
-    GPRT_TRY(launch_query_prep(ctx, d_q, qoff, 1, T.cloud_bbox.as<float>(), 1, nq, keys_buf.as<uint64_t>(),
-                               perm_buf.as<uint32_t>(), qs4_buf.as<float4>(), counters, nq));
+    GPRT_TRY(launch_query_prep(ctx, d_q, qoff, 1, T.key_bbox.as<float>(), 1, nq, keys_buf.as<uint64_t>(),
+                               perm_buf.as<uint32_t>(), qs4_buf.as<float4>(), counters, nq, nullptr,
+                               perm_buf.as<unsigned>(), counters + CNT_TOTAL - 1));
     int2 root;
     GPRT_HIP_TRY(hipMemcpyAsync(&root, T.roots.ptr, sizeof(int2), hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipStreamSynchronize(s));

@@ -105,7 +105,11 @@ def test_lbvh_invariants(ctx, leaf):
     keys, perm, nodes, roots = t.export()
     assert np.array_equal(np.sort(perm), np.arange(len(pts)))
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
-    codes, _ = oracle.morton_codes(pts)
+    # the tree keys are Morton codes over a cube of the largest extent
+    mn, mx = pts.min(0), pts.max(0)
+    e = np.float32((mx - mn).max())
+    kbox = np.concatenate([mn, (mn + e).astype(np.float32)]).astype(np.float32)
+    codes = oracle.morton_codes_bbox(pts, kbox)
     assert np.array_equal((keys & np.uint64(0xFFFFFFFF)).astype(np.uint32), codes[perm])
     # stable order: equal codes keep input order
     same = np.diff(keys.astype(np.int64)) == 0
@@ -167,7 +171,7 @@ def test_range_lidar_subset(ctx):
 
 # ---------------------------------------------------------------- batched uneven clouds
 def test_knn_batched_uneven(ctx):
-    pts, poff, q, qoff, kc = D.clustered_batch(n_clouds=24, lo=1e2, hi=2e4, seed=6, ks=(1, 5, 16, 32))
+    pts, poff, q, qoff, kc = D.clustered_batch(n_clouds=24, lo=1e2, hi=2e4, seed=6, ks=(1, 5, 16, 32, 64))
     t = ctx.build_batched(pts, poff)
     ids, out_off = t.k_nn_batched(q, qoff, kc)
     for c in range(24):
