@@ -158,7 +158,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->radius_mult = 3.0f;
     c->max_rounds = 2;
     c->traversal = GPRT_TRAV_THREAD;
-    c->stack_depth = 16;
+    c->stack_depth = 32;
     c->guard_slots = 3;
     c->sort_queries = 1;
     c->use_graph = 1;
@@ -443,7 +443,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
                                perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
                                p.d_kc, p.k, nclasses > 1));
-    int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : c.num_cus * 4;
+    int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : 0;
     int R = cfg.radius_heuristic == GPRT_RADIUS_INFINITE ? 1 : std::min(cfg.max_rounds, MAX_ROUNDS - 1) + 1;
     int* lists[2] = {c.ws[WS_LISTA].as<int>(), c.ws[WS_LISTB].as<int>()};
     RoundArgs ra{};
@@ -505,7 +505,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
             ra.out_list = lists[(r + 1) & 1];
             const bool tk = r == 0 && time_k && !timed;
             if (tk) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
-            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.collect_counters != 0));
+            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.stack_depth, cfg.collect_counters != 0));
             if (tk) {
                 GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
                 timed = true;

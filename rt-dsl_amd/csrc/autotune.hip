@@ -5,7 +5,7 @@
 // change when every query got k ids and the run was faster) and the funnel
 // profiler (gprt-profiler/src/main.rs:37-133: grid over percentile x multiplier
 // x cap on growing subsets, saturation gate).  Here the knobs are the ones that
-// matter on the GPU -- traversal mapping, leaf size, initial-radius rule
+// matter on the GPU -- traversal mapping, leaf size, LDS stack depth, initial-radius rule
 // (percentile / unbounded) and growth multiplier -- and the gate is stronger:
 // every candidate's ids must equal those of the reference schedule bit for bit.
 // Timing uses hipEvents around the query phase (median of the timed repeats).
@@ -68,7 +68,7 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
                                  int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                  const char* json_path, int flags) {
     if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
-    if (n <= 0 || nq <= 0 || k < 1 || k > 61) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 61");
+    if (n <= 0 || nq <= 0 || k < 1 || k > 64) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 64");
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));
     DevBuf dq, dids, dref;
@@ -121,6 +121,12 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
             if (leaf == start.leaf || rc != GPRT_OK) continue;
             Cand cd = best;
             cd.leaf = leaf;
+            rc = try_cand(cd);
+        }
+        for (int depth : {16, 24, 32}) {
+            if (depth == best.cfg.stack_depth || rc != GPRT_OK) continue;
+            Cand cd = best;
+            cd.cfg.stack_depth = depth;
             rc = try_cand(cd);
         }
         for (int trav : {GPRT_TRAV_THREAD, GPRT_TRAV_WAVE}) {

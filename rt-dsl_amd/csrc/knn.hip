@@ -984,26 +984,49 @@ int capacity_class(int kmax, int guard) {
 
 // The first round (all queries, identity work list) gets its own kernel symbol
 // so profiles report it apart from the short later rounds.
-template <int C>
-static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, bool count) {
-    if (trav == GPRT_TRAV_WAVE) {
-        if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
-    } else if (a.round == 0) {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false, true>), dim3(grid), dim3(256), 0, s, a);
+template <int C, int STACK>
+static void launch_lane(hipStream_t s, const RoundArgs& a, int grid, bool count) {
+    if (grid <= 0) {
+        // persistent grid: as many blocks as the device holds at once
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, knn_lane_kernel<C, GUARD, STACK, false, true>, 256,
+                                                         0) != hipSuccess || per_cu <= 0)
+            per_cu = 4;
+        int dev = 0, cus = 256;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = per_cu * cus;
+    }
+    if (a.round == 0) {
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, true, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, false, true>), dim3(grid), dim3(256), 0, s, a);
     } else {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, true, false>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, 32, false, false>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, true, false>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, false, false>), dim3(grid), dim3(256), 0, s, a);
     }
 }
 
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, bool count) {
+template <int C>
+static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, int stack, bool count) {
+    if (trav == GPRT_TRAV_WAVE) {
+        if (grid <= 0) grid = 1024;
+        if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
+    } else if (stack <= 16) {
+        launch_lane<C, 16>(s, a, grid, count);
+    } else if (stack <= 24) {
+        launch_lane<C, 24>(s, a, grid, count);
+    } else {
+        launch_lane<C, 32>(s, a, grid, count);
+    }
+}
+
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count) {
     switch (cap) {
-        case 8: launch_round_c<8>(s, a, grid, trav, count); break;
-        case 16: launch_round_c<16>(s, a, grid, trav, count); break;
-        case 32: launch_round_c<32>(s, a, grid, trav, count); break;
-        default: launch_round_c<64>(s, a, grid, trav, count); break;
+        case 8: launch_round_c<8>(s, a, grid, trav, stack, count); break;
+        case 16: launch_round_c<16>(s, a, grid, trav, stack, count); break;
+        case 32: launch_round_c<32>(s, a, grid, trav, stack, count); break;
+        default: launch_round_c<64>(s, a, grid, trav, stack, count); break;
     }
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }

@@ -87,7 +87,7 @@ size_t query_prep_scratch_bytes(int nclouds, int use_morton);
 int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, bool count);
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
 
 }  // namespace gprt
