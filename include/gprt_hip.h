@@ -60,6 +60,10 @@ extern "C" {
 #define GPRT_NO_SYNC 2
 /* gprt_hip_knn_plain: count node visits and point tests into the statistics */
 #define GPRT_COLLECT_COUNTERS 4
+/* tree build: order the points by the reference's 30-bit Morton codes (10 bits
+ * per axis, gprt-core/src/morton.rs:4-48) instead of the default 21 bits per
+ * axis.  Results are the same either way; only the tree shape changes. */
+#define GPRT_KEYS_REF30 8
 
 #define GPRT_ID_NONE 0xFFFFFFFFu
 
@@ -124,7 +128,8 @@ typedef struct {
     int nclouds;
     int leaf_size;
     float bbox[6];  /* cloud 0: min xyz, max xyz */
-    int key_bits;
+    int key_bits;   /* significant bits of the sort key: 3 * axis_bits + cloud bits */
+    int axis_bits;  /* Morton bits per axis; key = cloud << (3 * axis_bits) | morton */
 } gprt_hip_tree_desc;
 
 /* ---------------- library / context ---------------- */

@@ -225,6 +225,31 @@ EXPORT int oracle_morton_codes_bbox(const float *pts, int64_t n, const float *bb
     return 0;
 }
 
+/* Wide keys of the tree build: the same quantisation with b bits per axis,
+ * q = min(sat((p - min) / ext * (2^b - 1)), 2^b - 1), interleaved bit by bit
+ * (x above y above z at each level).  At b = 10 this is morton.rs:42-47
+ * exactly, which is how the tests pin it to the reference's codes. */
+EXPORT int oracle_morton_codes_wide(const float *pts, int64_t n, const float *bbox6, int bits, uint64_t *codes) {
+    if (bits < 1 || bits > 21) return 1;
+    const uint32_t qmax = (1u << bits) - 1u;
+    const float scale = (float)qmax;
+    float ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = fmaxf(bbox6[3 + a] - bbox6[a], 1e-6f);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) {
+            float t = (pts[3 * i + a] - bbox6[a]) / ext[a] * scale;
+            uint32_t u = sat_u32(t);
+            c[a] = u < qmax ? u : qmax;
+        }
+        uint64_t m = 0;
+        for (int l = bits - 1; l >= 0; --l)
+            for (int a = 0; a < 3; ++a) m = (m << 1) | ((c[a] >> l) & 1u);
+        codes[i] = m;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* radius samplers                                                           */
 /* ------------------------------------------------------------------------ */

@@ -42,6 +42,7 @@ def lib():
         L.oracle_morton3.restype = ctypes.c_uint32
         L.oracle_morton_codes.argtypes = [_f32p, _i64, _u32p, ctypes.c_void_p]
         L.oracle_morton_codes_bbox.argtypes = [_f32p, _i64, _f32p, _u32p]
+        L.oracle_morton_codes_wide.argtypes = [_f32p, _i64, _f32p, _int, _u64p]
         L.oracle_lcg_draws.argtypes = [ctypes.c_uint64, _i64, _u64p]
         L.oracle_lcg_draws.restype = ctypes.c_uint64
         L.oracle_sampler_lcg.argtypes = [_f32p, _i64, _f32p, _i64, _int, ctypes.c_float]
@@ -109,6 +110,15 @@ def morton_codes_bbox(pts, bbox6):
     pts = _pts(pts)
     codes = np.empty(len(pts), np.uint32)
     lib().oracle_morton_codes_bbox(pts, len(pts), np.ascontiguousarray(bbox6, np.float32), codes)
+    return codes
+
+
+def morton_codes_wide(pts, bbox6, bits):
+    """Tree sort keys with `bits` per axis (bits = 10: the reference's codes)."""
+    pts = _pts(pts)
+    codes = np.empty(len(pts), np.uint64)
+    if lib().oracle_morton_codes_wide(pts, len(pts), np.ascontiguousarray(bbox6, np.float32), bits, codes):
+        raise ValueError("bits must be in [1, 21]")
     return codes
 
 

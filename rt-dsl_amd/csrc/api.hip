@@ -297,7 +297,7 @@ static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, i
     auto* t = new gprt_hip_tree();
     t->owner = h;
     GPRT_HIP_TRY(hipEventRecord(c.ev2, c.stream));
-    int rc = tree_build(&c, d_pts, off, nclouds, leaf_size, &t->t);
+    int rc = tree_build(&c, d_pts, off, nclouds, leaf_size, &t->t, (flags & GPRT_KEYS_REF30) ? 10 : 0);
     if (rc != GPRT_OK) {
         gprt_hip_tree_destroy(t);
         return rc;
@@ -342,6 +342,7 @@ extern "C" int gprt_hip_tree_info(const gprt_hip_tree* t, gprt_hip_tree_desc* ou
     out->leaf_size = t->t.leaf_size;
     for (int a = 0; a < 6; ++a) out->bbox[a] = t->t.h_bbox[a];
     out->key_bits = t->t.key_bits;
+    out->axis_bits = t->t.axis_bits;
     return GPRT_OK;
 }
 

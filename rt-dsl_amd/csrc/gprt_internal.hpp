@@ -78,7 +78,7 @@ struct Tree {
     DevBuf key_bbox;    // float[6*nclouds]: box the sort keys are normalised to
     DevBuf perm;        // uint32[n]: sorted position -> input index
     DevBuf inv;         // uint32[n]: input index -> sorted position
-    DevBuf codes;       // uint64[n]: sorted keys (cloud << 32 | morton)
+    DevBuf codes;       // uint64[n]: sorted keys (cloud << 3*axis_bits | morton)
     DevBuf parent;      // int[n-1 + n]: parent of internal nodes, then of points
     DevBuf flags;       // int[n-1]: refit arrival counters
     DevBuf ranges;      // int2[n-1]: sorted-point range [lo, hi] of each internal node
@@ -87,6 +87,7 @@ struct Tree {
     std::vector<int64_t> h_cloud_off;
     float h_bbox[6] = {0, 0, 0, 0, 0, 0};
     int key_bits = 30;  // significant bits of the sort key
+    int axis_bits = 10; // Morton bits per axis
     int max_depth_bound = 64;
 };
 
@@ -149,7 +150,9 @@ __device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
 // ---------------------------------------------------------------------------
 // internal entry points between translation units
 // ---------------------------------------------------------------------------
-int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int nclouds, int leaf_size, Tree* t);
+// axis_bits: Morton bits per axis of the sort keys (0: the widest that fits)
+int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int nclouds, int leaf_size, Tree* t,
+               int axis_bits = 0);
 int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
 int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);

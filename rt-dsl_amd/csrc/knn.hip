@@ -956,16 +956,6 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
 }
 
-__global__ void key_cloud_kernel(const uint64_t* keys, int n, int* cloud) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cloud[i] = (int)(keys[i] >> 32);
-}
-
-int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud) {
-    hipLaunchKernelGGL(key_cloud_kernel, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, cloud);
-    return GPRT_OK;
-}
-
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
     if (n > 0) hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
     return GPRT_OK;

@@ -84,7 +84,6 @@ constexpr int N_KCLASS = 5;
 __host__ __device__ inline int k_class(int k) { return k <= 5 ? 0 : (k <= 13 ? 1 : (k <= 29 ? 2 : (k <= 61 ? 3 : 4))); }
 __host__ __device__ inline int k_class_cap(int cls) { return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 32 : 64)); }
 size_t query_prep_scratch_bytes(int nclouds, int use_morton);
-int launch_key_cloud(hipStream_t s, const uint64_t* keys, int n, int* cloud);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);
 int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count);

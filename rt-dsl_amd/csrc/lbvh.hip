@@ -118,18 +118,43 @@ __device__ __forceinline__ uint32_t morton_of(float x, float y, float z, const f
            expand_bits(quantize(z, bb[2], ez));
 }
 
-// grid: (chunks, clouds).  key = cloud << 32 | morton; vals = input index.
+// 21-bit spread for the wide keys: bit i of x moves to bit 3i
+__device__ __forceinline__ uint64_t expand_bits21(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x001F00000000FFFFull;
+    x = (x | (x << 16)) & 0x001F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// b bits per axis: q = min(sat((p - mn) / ext * (2^b - 1)), 2^b - 1), the
+// reference's quantisation (morton.rs:42-47) with 1023 generalised; at b = 10
+// the key is the reference's 30-bit code.
+__device__ __forceinline__ uint32_t quantize_b(float p, float mn, float ext, float scale, uint32_t qmax) {
+    float t = (p - mn) / ext * scale;
+    uint32_t u = !(t > 0.0f) ? 0u : (t >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)t);
+    return u < qmax ? u : qmax;
+}
+
+// grid: (chunks, clouds).  key = cloud << 3b | morton(b bits per axis);
+// vals = input index.
 __global__ void morton_keys_kernel(const float* __restrict__ pts, const int64_t* __restrict__ off,
-                                   const float* __restrict__ bbox, uint64_t* __restrict__ keys,
+                                   const float* __restrict__ bbox, int b, uint64_t* __restrict__ keys,
                                    uint32_t* __restrict__ vals) {
     int c = blockIdx.y;
-    int64_t b = off[c], e = off[c + 1];
-    float bb[6];
-    for (int a = 0; a < 6; ++a) bb[a] = bbox[6 * c + a];
-    for (int64_t i = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e;
+    int64_t lo = off[c], hi = off[c + 1];
+    const float* bb = bbox + 6 * c;
+    const float mn[3] = {bb[0], bb[1], bb[2]};
+    const float ex[3] = {fmaxf(bb[3] - bb[0], 1e-6f), fmaxf(bb[4] - bb[1], 1e-6f), fmaxf(bb[5] - bb[2], 1e-6f)};
+    const uint32_t qmax = (1u << b) - 1u;
+    const float scale = (float)qmax;
+    for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi;
          i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t m = morton_of(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], bb);
-        keys[i] = ((uint64_t)c << 32) | m;
+        uint64_t m = 0;
+        for (int a = 0; a < 3; ++a) m = (m << 1) | expand_bits21(quantize_b(pts[3 * i + a], mn[a], ex[a], scale, qmax));
+        keys[i] = ((uint64_t)c << (3 * b)) | m;
         vals[i] = (uint32_t)i;
     }
 }
@@ -326,12 +351,12 @@ int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int be
 // gather sorted points: pts4[i] = (xyz of input perm[i], id local to its cloud)
 // ---------------------------------------------------------------------------
 __global__ void gather_points_kernel(const float* __restrict__ pts, const uint32_t* __restrict__ perm,
-                                     const uint64_t* __restrict__ keys, const int64_t* __restrict__ off,
+                                     const uint64_t* __restrict__ keys, int cshift, const int64_t* __restrict__ off,
                                      float4* __restrict__ out, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         uint32_t src = perm[i];
-        int c = (int)(keys[i] >> 32);
+        int c = (int)(keys[i] >> cshift);
         int id = (int)((int64_t)src - off[c]);
         out[i] = make_float4(pts[3 * src], pts[3 * src + 1], pts[3 * src + 2], __int_as_float(id));
     }
@@ -347,7 +372,7 @@ __device__ __forceinline__ int delta_fn(const uint64_t* __restrict__ keys, int n
     return __clzll((long long)(a ^ b));
 }
 
-__global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, int leaf_size,
+__global__ void karras_kernel(const uint64_t* __restrict__ keys, int cshift, int n, int leaf_size,
                               const int64_t* __restrict__ off, Node* __restrict__ nodes, int* __restrict__ parent,
                               int2* __restrict__ roots, int2* __restrict__ ranges) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,7 +411,7 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int n, int leaf
     parent[(hi == gamma + 1 ? (n - 1) : 0) + gamma + 1] = (i << 1) | 1;
     if (i == 0) parent[0] = -1;
     // cloud roots: the node whose range is exactly one cloud
-    int c = (int)(keys[lo] >> 32);
+    int c = (int)(keys[lo] >> cshift);
     if ((int64_t)lo == off[c] && (int64_t)hi == off[c + 1] - 1 && (hi - lo + 1) > leaf_size) roots[c] = make_int2(i, hi - lo + 1);
 }
 
@@ -416,7 +441,8 @@ static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 static int g_key_mode = 1;
 extern "C" GPRT_HIP_API void gprt_hip_debug_set_key_mode(int m) { g_key_mode = m ? 1 : 0; }
 
-int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, int leaf_size, Tree* t) {
+int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, int leaf_size, Tree* t,
+               int axis_bits) {
     int64_t n64 = h_off[nclouds];
     if (n64 > (int64_t)0x7FFFFFF0) return fail(GPRT_ERR_ARG, "too many points for one tree");
     int n = (int)n64;
@@ -426,8 +452,13 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     t->h_cloud_off.assign(h_off, h_off + nclouds + 1);
     int cloud_bits = 0;
     while ((1 << cloud_bits) < nclouds) ++cloud_bits;
-    t->key_bits = 30 + cloud_bits;
-    t->max_depth_bound = 30 + cloud_bits + 32;
+    // bits per axis: as many as fit beside the cloud index in 64 bits (21 for
+    // one or two clouds), or the reference's 10 when asked for
+    const int b = axis_bits > 0 ? std::min(axis_bits, (64 - cloud_bits) / 3) : std::min(21, (64 - cloud_bits) / 3);
+    if (b < 1) return fail(GPRT_ERR_ARG, "too many clouds for 64-bit keys");
+    t->axis_bits = b;
+    t->key_bits = 3 * b + cloud_bits;
+    t->max_depth_bound = t->key_bits + 32;
     hipStream_t s = ctx->stream;
 
     GPRT_TRY(t->cloud_off.reserve(sizeof(int64_t) * (nclouds + 1)));
@@ -462,14 +493,11 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
                        nclouds, leaf_size, t->roots.as<int2>());
     if (n > 0) {
         hipLaunchKernelGGL(morton_keys_kernel, dim3(std::max(1, std::min(cdiv(maxc, 256), 4096)), nclouds),
-                           dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->key_bbox.as<float>(),
+                           dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->key_bbox.as<float>(), b,
                            t->codes.as<uint64_t>(), t->perm.as<uint32_t>());
-        GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 0, 30));
-        if (cloud_bits > 0)
-            GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 32,
-                                          32 + cloud_bits));
+        GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 0, t->key_bits));
         hipLaunchKernelGGL(gather_points_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts,
-                           t->perm.as<uint32_t>(), t->codes.as<uint64_t>(), t->cloud_off.as<int64_t>(),
+                           t->perm.as<uint32_t>(), t->codes.as<uint64_t>(), 3 * b, t->cloud_off.as<int64_t>(),
                            t->pts4.as<float4>(), n);
         hipLaunchKernelGGL(invert_perm_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, t->perm.as<uint32_t>(),
                            t->inv.as<uint32_t>(), n);
@@ -478,7 +506,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
         hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->flags.as<int>(), n - 1, 0);
         int* ready = t->ready.as<int>();
         hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ready + n, 1, 0);
-        hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), n,
+        hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), 3 * b, n,
                            leaf_size, t->cloud_off.as<int64_t>(), t->nodes.as<Node>(), t->parent.as<int>(),
                            t->roots.as<int2>(), t->ranges.as<int2>());
         BoxSlots slots{t->nodes.as<Node>(), t->pts4.as<float4>()};

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libgprt_hip.so")
 
 OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
-PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS = 0, 1, 2, 4
+PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30 = 0, 1, 2, 4, 8
 ID_NONE = 0xFFFFFFFF
 RADIUS_MAX, RADIUS_PERCENTILE, RADIUS_FIXED, RADIUS_INFINITE = 0, 1, 2, 3
 SAMPLER_LCG, SAMPLER_STRIDED = 0, 1
@@ -79,7 +79,8 @@ class RunStats(ctypes.Structure):
 
 class TreeDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("nclouds", ctypes.c_int), ("leaf_size", ctypes.c_int),
-                ("bbox", ctypes.c_float * 6), ("key_bits", ctypes.c_int)]
+                ("bbox", ctypes.c_float * 6), ("key_bits", ctypes.c_int),
+                ("axis_bits", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -265,21 +266,23 @@ class Context:
         return s.as_dict()
 
     # ---- trees ----
-    def build(self, pts, leaf_size=8):
+    def build(self, pts, leaf_size=8, ref_keys=False):
+        """LBVH over pts; ref_keys sorts by the reference's 30-bit Morton codes."""
         pts = _points(pts)
         p, dev = _ptr(pts)
         t = ctypes.c_void_p()
-        _check(lib().gprt_hip_tree_build(self.h, p, len(pts), leaf_size, PTR_DEVICE if dev else PTR_HOST,
-                                         ctypes.byref(t)))
+        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0)
+        _check(lib().gprt_hip_tree_build(self.h, p, len(pts), leaf_size, flags, ctypes.byref(t)))
         return Tree(self, t, pts)
 
-    def build_batched(self, pts, cloud_offsets, leaf_size=8):
+    def build_batched(self, pts, cloud_offsets, leaf_size=8, ref_keys=False):
         pts = _points(pts)
         off = np.ascontiguousarray(cloud_offsets, np.int64)
         p, dev = _ptr(pts)
         t = ctypes.c_void_p()
-        _check(lib().gprt_hip_tree_build_batched(self.h, p, off.ctypes.data, len(off) - 1, leaf_size,
-                                                 PTR_DEVICE if dev else PTR_HOST, ctypes.byref(t)))
+        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0)
+        _check(lib().gprt_hip_tree_build_batched(self.h, p, off.ctypes.data, len(off) - 1, leaf_size, flags,
+                                                 ctypes.byref(t)))
         return Tree(self, t, pts)
 
     # ---- pieces ----
@@ -348,6 +351,12 @@ class Tree:
         self.nclouds = d.nclouds
         self.leaf_size = d.leaf_size
         self.bbox = np.array(list(d.bbox), np.float32)
+        self.key_bits = d.key_bits
+        self.axis_bits = d.axis_bits
+
+    def info(self):
+        return {"n": self.n, "nclouds": self.nclouds, "leaf_size": self.leaf_size, "bbox": self.bbox,
+                "key_bits": self.key_bits, "axis_bits": self.axis_bits}
 
     def close(self):
         if self.h:

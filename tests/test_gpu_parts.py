@@ -98,21 +98,26 @@ def _walk(nodes, roots, c):
     return seen
 
 
-@pytest.mark.parametrize("leaf", [1, 4, 8, 32])
-def test_lbvh_invariants(ctx, leaf):
+@pytest.mark.parametrize("leaf,ref_keys", [(1, False), (4, False), (8, False), (32, False), (4, True), (8, True)])
+def test_lbvh_invariants(ctx, leaf, ref_keys):
     pts, _ = D.road_network(n=40_000, nq=10)
-    t = ctx.build(pts, leaf_size=leaf)
+    t = ctx.build(pts, leaf_size=leaf, ref_keys=ref_keys)
+    info = t.info()
+    bits = 10 if ref_keys else 21
+    assert info["axis_bits"] == bits and info["key_bits"] == 3 * bits
     keys, perm, nodes, roots = t.export()
     assert np.array_equal(np.sort(perm), np.arange(len(pts)))
-    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    assert np.all(keys[1:] >= keys[:-1])
     # the tree keys are Morton codes over a cube of the largest extent
     mn, mx = pts.min(0), pts.max(0)
     e = np.float32((mx - mn).max())
     kbox = np.concatenate([mn, (mn + e).astype(np.float32)]).astype(np.float32)
-    codes = oracle.morton_codes_bbox(pts, kbox)
-    assert np.array_equal((keys & np.uint64(0xFFFFFFFF)).astype(np.uint32), codes[perm])
+    codes = oracle.morton_codes_wide(pts, kbox, bits)
+    assert np.array_equal(keys, codes[perm])
+    if ref_keys:
+        assert np.array_equal(keys.astype(np.uint32), oracle.morton_codes_bbox(pts, kbox)[perm])
     # stable order: equal codes keep input order
-    same = np.diff(keys.astype(np.int64)) == 0
+    same = keys[1:] == keys[:-1]
     assert np.all(np.diff(perm.astype(np.int64))[same] > 0)
     seen = _walk(nodes, roots, 0)
     assert sorted(seen) == list(range(len(pts)))

@@ -80,6 +80,30 @@ def test_morton_query_outside_box_saturates():
     assert codes[2] == _py_morton(int(np.float32(0.5) * np.float32(1023)), 1023, 0)
 
 
+def test_morton_wide_keys():
+    """The tree's sort keys: at 10 bits per axis they are the reference's
+    codes; at other widths each axis de-interleaves to the float32 cell index."""
+    rng = np.random.default_rng(8)
+    pts = (rng.random((400, 3), dtype=np.float32) * 300 - 100).astype(np.float32)
+    pts[0] = [-100, -100, -100]
+    pts[1] = [500, 500, 500]  # outside the box: saturates
+    bbox = np.array([-100, -100, -100, 200, 200, 200], np.float32)
+    assert np.array_equal(oracle.morton_codes_wide(pts, bbox, 10), oracle.morton_codes_bbox(pts, bbox).astype(np.uint64))
+    for bits in (1, 7, 16, 21):
+        w = oracle.morton_codes_wide(pts, bbox, bits)
+        qmax = (1 << bits) - 1
+        t = (pts - bbox[:3]) / (bbox[3:] - bbox[:3]) * np.float32(qmax)
+        cell = np.minimum(np.where(t > 0, t, 0).astype(np.uint64), qmax)
+        for a in range(3):
+            got = np.zeros(len(pts), np.uint64)
+            for lvl in range(bits):
+                got |= ((w >> np.uint64(3 * lvl + 2 - a)) & np.uint64(1)) << np.uint64(lvl)
+            assert np.array_equal(got, cell[:, a]), (bits, a)
+        assert w[1] == (1 << (3 * bits)) - 1 and w[0] == 0
+    with pytest.raises(ValueError):
+        oracle.morton_codes_wide(pts, bbox, 22)
+
+
 # ---------------------------------------------------------------- LCG sampler
 def test_lcg_draws_golden():
     with open(os.path.join(GOLD, "lcg_draws.json")) as f:
