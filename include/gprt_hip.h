@@ -80,7 +80,7 @@ extern "C" {
 
 /* traversal mapping */
 #define GPRT_TRAV_THREAD 0 /* one query per lane */
-#define GPRT_TRAV_WAVE 1   /* one query per wavefront, lanes test a leaf together */
+#define GPRT_TRAV_WAVE 1   /* 64 Morton-adjacent queries walk the tree together (packet) */
 
 typedef struct gprt_hip_ctx gprt_hip_ctx;
 typedef struct gprt_hip_tree gprt_hip_tree;
@@ -97,7 +97,8 @@ typedef struct {
     float radius_mult;    /* radius growth per round (default 3.0) */
     int max_rounds;       /* bounded rounds before the final unbounded one (default 2) */
     int traversal;        /* GPRT_TRAV_*, default THREAD */
-    int stack_depth;      /* LDS traversal-stack entries per query (default 16) */
+    int stack_depth;      /* traversal-stack entries per query, LDS + HBM spill (default 64);
+                             a deeper walk is finished by the exact FP64 kernel */
     int guard_slots;      /* spare candidate slots beyond k (default 3) */
     int sort_queries;     /* process queries in Morton order (default 1) */
     int use_graph;        /* replay the pipeline as a captured hipGraph (default 1) */
@@ -105,6 +106,8 @@ typedef struct {
     int block_threads;    /* 0 = automatic (256) */
     int collect_counters; /* count node visits / point tests (slower; for the roofline) */
     int time_kernels;     /* with use_graph == 0: time the first traversal round on its own */
+    int lds_stack;        /* most recent stack entries kept in LDS per query: 8, 16 or 32
+                             (default 8); older ones spill to an HBM workspace */
 } gprt_hip_config;
 
 typedef struct {

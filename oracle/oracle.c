@@ -370,8 +370,8 @@ EXPORT float oracle_sampler_strided(const float *pts, int64_t n, const float *qs
 /* ------------------------------------------------------------------------ */
 /* pos_mass: float4 (x, y, z, m) per body.  For each sampled body i:
  * acc[i] = sum_{j} G m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^{3/2}
- * (the j == i term is zero).  This is the force law of the north-star
- * workload; the reference's shader uses G*m_i*m_j/(d^2 sqrt(d^2+1e-6))
+ * (the j == i term is zero).  This is the Plummer-softened force law of the
+ * Barnes-Hut workload; the reference's shader uses G*m_i*m_j/(d^2 sqrt(d^2+1e-6))
  * (gprt-macros/src/lib.rs:785) and is deliberately not followed. */
 EXPORT int oracle_bh_direct(const float *pos_mass, int64_t n, const int64_t *sample, int64_t ns, double G,
                             double eps, double *acc_out, int nthreads) {

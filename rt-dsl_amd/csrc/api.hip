@@ -63,6 +63,7 @@ enum {
 // extra per-context buffers beyond the 8 generic slots
 struct CtxExtra {
     DevBuf q_in, ids_out, d2_out, qcloud, kc, qoff, outbase, flag;
+    DevBuf spill;  // traversal-stack entries beyond the LDS ring, int2[depth][lanes]
     // range query
     DevBuf r_counts, r_d2, r_big, r_keys, r_perm, r_qs4, r_sums, r_off;
     // Barnes-Hut
@@ -85,6 +86,7 @@ static void drop_extra(Ctx* c) {
         if (g_extra[i].first == c) {
             CtxExtra* e = g_extra[i].second;
             for (DevBuf* b : {&e->q_in, &e->ids_out, &e->d2_out, &e->qcloud, &e->kc, &e->qoff, &e->outbase, &e->flag,
+                              &e->spill,
                               &e->r_counts, &e->r_d2, &e->r_big, &e->r_keys, &e->r_perm, &e->r_qs4, &e->r_sums,
                               &e->r_off, &e->bh_xyz, &e->bh_mass, &e->bh_acc})
                 b->release();
@@ -158,7 +160,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->radius_mult = 3.0f;
     c->max_rounds = 2;
     c->traversal = GPRT_TRAV_THREAD;
-    c->stack_depth = 32;
+    c->stack_depth = 64;
     c->guard_slots = 3;
     c->sort_queries = 1;
     c->use_graph = 1;
@@ -166,6 +168,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->block_threads = 0;
     c->collect_counters = 0;
     c->time_kernels = 0;
+    c->lds_stack = 8;
 }
 
 extern "C" int gprt_hip_num_devices(int* out) {
@@ -196,12 +199,15 @@ extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c.num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c.ev0) != hipSuccess || hipEventCreate(&c.ev1) != hipSuccess ||
-        hipEventCreate(&c.ev2) != hipSuccess || hipEventCreate(&c.ev3) != hipSuccess) {
-        delete h;
+        hipEventCreate(&c.ev2) != hipSuccess || hipEventCreate(&c.ev3) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) != hipSuccess) {
+        gprt_hip_ctx_destroy(h);
         return fail(GPRT_ERR_HIP, "stream/event creation failed");
     }
-    if (c.counters.reserve(sizeof(unsigned) * CNT_TOTAL + 64) != GPRT_OK) {
+    if (c.counters.reserve(sizeof(unsigned) * CNT_ALL) != GPRT_OK) {
         delete h;
         return GPRT_ERR_ALLOC;
     }
@@ -220,9 +226,11 @@ extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
     for (auto& b : c.sort_tmp) b.release();
     c.counters.release();
     drop_extra(&c);
-    for (hipEvent_t e : {c.ev0, c.ev1, c.ev2, c.ev3})
+    if (c.side) (void)hipStreamSynchronize(c.side);
+    for (hipEvent_t e : {c.ev0, c.ev1, c.ev2, c.ev3, c.ev_fork, c.ev_join})
         if (e) (void)hipEventDestroy(e);
     if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.side) (void)hipStreamDestroy(c.side);
     delete h;
     return GPRT_OK;
 }
@@ -414,12 +422,36 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     GPRT_TRY(c.ws[WS_MISC].reserve(sizeof(float) * 101 * std::max(p.nclouds, 1) + 1024));
     CtxExtra* x = extra(&c);
     if (p.nclouds > 1) GPRT_TRY(x->qcloud.reserve(4 * nq));
+    // HBM part of the per-lane traversal stacks of the persistent rounds
+    if (p.cfg.traversal != GPRT_TRAV_WAVE) {
+        size_t lanes = 0;
+        for (int cl = 0; cl + 1 < N_KCLASS; ++cl)
+            if (p.class_size[cl] > 0)
+                lanes = std::max(lanes, (size_t)256 * lane_round_grid(k_class_cap(cl), p.cfg.lds_stack,
+                                                                      p.cfg.grid_blocks));
+        const int depth = lane_spill_depth(p.cfg.stack_depth, p.cfg.lds_stack);
+        if (lanes && depth > 0) GPRT_TRY(x->spill.reserve(8 * lanes * depth));
+    }
     // radix sort temporaries for nq keys
     GPRT_TRY(c.sort_tmp[0].reserve(8 * nq));
     GPRT_TRY(c.sort_tmp[1].reserve(4 * nq));
     GPRT_TRY(c.sort_tmp[2].reserve(radix_sort_hist_bytes((int)nq)));
     return GPRT_OK;
 }
+
+static int check_config(const gprt_hip_config& cf) {
+    if (cf.max_rounds < 0) return fail(GPRT_ERR_ARG, "max_rounds < 0");
+    if (cf.traversal < GPRT_TRAV_THREAD || cf.traversal > GPRT_TRAV_WAVE) return fail(GPRT_ERR_ARG, "bad traversal");
+    if (cf.stack_depth < 1 || cf.stack_depth > 4096) return fail(GPRT_ERR_ARG, "stack_depth must be in [1, 4096]");
+    if (cf.lds_stack < 1 || cf.lds_stack > 32) return fail(GPRT_ERR_ARG, "lds_stack must be in [1, 32]");
+    if (cf.grid_blocks < 0 || cf.grid_blocks > (1 << 20)) return fail(GPRT_ERR_ARG, "bad grid_blocks");
+    if (cf.sampler != GPRT_SAMPLER_LCG && cf.sampler != GPRT_SAMPLER_STRIDED) return fail(GPRT_ERR_ARG, "bad sampler");
+    if (cf.radius_heuristic < GPRT_RADIUS_MAX || cf.radius_heuristic > GPRT_RADIUS_INFINITE)
+        return fail(GPRT_ERR_ARG, "bad radius_heuristic");
+    if (!(cf.radius_mult > 1.0f) || !(cf.radius_mult < 1e30f)) return fail(GPRT_ERR_ARG, "radius_mult must be > 1");
+    return GPRT_OK;
+}
+
 
 static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     hipStream_t s = c.stream;
@@ -431,10 +463,16 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     float* radius = kth + 100 * p.nclouds;
     int nq = (int)p.nq;
     CtxExtra* x = extra(&c);
-    hipLaunchKernelGGL(init_call_kernel, dim3(1), dim3(64), 0, s, counters, CNT_TOTAL + 2 * CNT_STATS64,
+    hipLaunchKernelGGL(init_call_kernel, dim3(1), dim3(64), 0, s, counters, CNT_ALL,
                        p.nclouds == 1 ? (int64_t*)p.d_qoff : nullptr, p.nq);
-    GPRT_TRY(launch_sampler(s, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(), T.cloud_off.as<int64_t>(),
-                            p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic, cfg.radius_param, kth, radius));
+    // the radius sampler runs on the side stream, overlapping the query sort;
+    // the rounds wait for both
+    GPRT_HIP_TRY(hipEventRecord(c.ev_fork, s));
+    GPRT_HIP_TRY(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    GPRT_TRY(launch_sampler(c.side, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(),
+                            T.cloud_off.as<int64_t>(), p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic,
+                            cfg.radius_param, kth, radius));
+    GPRT_HIP_TRY(hipEventRecord(c.ev_join, c.side));
     uint64_t* keys = c.ws[WS_QKEYS].as<uint64_t>();
     uint32_t* perm = c.ws[WS_QPERM].as<uint32_t>();
     float4* qs4 = c.ws[WS_QS4].as<float4>();
@@ -444,6 +482,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
                                perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
                                p.d_kc, p.k, nclasses > 1));
+    GPRT_HIP_TRY(hipStreamWaitEvent(s, c.ev_join, 0));
     int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : 0;
     int R = cfg.radius_heuristic == GPRT_RADIUS_INFINITE ? 1 : std::min(cfg.max_rounds, MAX_ROUNDS - 1) + 1;
     int* lists[2] = {c.ws[WS_LISTA].as<int>(), c.ws[WS_LISTB].as<int>()};
@@ -465,6 +504,8 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.out_base = p.d_outbase;
     ra.qoff = p.d_qoff;
     ra.max_steps = T.n + 64;
+    ra.spill = x->spill.as<int2>();
+    ra.spill_depth = lane_spill_depth(cfg.stack_depth, cfg.lds_stack);
     ExactArgs ea{};
     ea.nq = p.nq;
     ea.qs = qs4;
@@ -506,7 +547,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
             ra.out_list = lists[(r + 1) & 1];
             const bool tk = r == 0 && time_k && !timed;
             if (tk) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
-            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.stack_depth, cfg.collect_counters != 0));
+            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.lds_stack, cfg.collect_counters != 0));
             if (tk) {
                 GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
                 timed = true;
@@ -537,7 +578,7 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
         const void* ptrs[] = {T.nodes.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
                               c.ws[0].ptr, c.ws[1].ptr, c.ws[2].ptr, c.ws[3].ptr, c.ws[4].ptr, c.ws[5].ptr,
                               c.ws[6].ptr, c.sort_tmp[0].ptr, c.sort_tmp[1].ptr, c.sort_tmp[2].ptr,
-                              c.counters.ptr, extra(&c)->qcloud.ptr};
+                              c.counters.ptr, extra(&c)->qcloud.ptr, extra(&c)->spill.ptr};
         uint64_t key = 1469598103934665603ull;
         key = fnv(key, &p, sizeof(KnnPlan));
         key = fnv(key, ptrs, sizeof(ptrs));
@@ -621,8 +662,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
     GPRT_HIP_TRY(hipSetDevice(c.device));
     gprt_hip_config cf;
     if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
-
-    if (cf.max_rounds < 0) return fail(GPRT_ERR_ARG, "max_rounds < 0");
+    GPRT_TRY(check_config(cf));
     c.stats = gprt_hip_run_stats{};
     if (nq == 0) return GPRT_OK;
     CtxExtra* x = extra(&c);
@@ -747,7 +787,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     GPRT_HIP_TRY(hipSetDevice(c.device));
     gprt_hip_config cf;
     if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
-
+    GPRT_TRY(check_config(cf));
     c.stats = gprt_hip_run_stats{};
     if (nq == 0) return GPRT_OK;
     CtxExtra* x = extra(&c);

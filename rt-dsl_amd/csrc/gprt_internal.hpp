@@ -108,7 +108,9 @@ __device__ __forceinline__ double d2d(float qx, float qy, float qz, float px, fl
     return (dx * dx + dy * dy) + dz * dz;
 }
 
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+// clamp to [lo, hi] for lo <= hi (every box here is built from points): the
+// median of the three, one v_med3_f32
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 
 __device__ __forceinline__ float box2f(float qx, float qy, float qz, const float4& lo, const float4& hi) {
     return d2f(qx, qy, qz, clampf(qx, lo.x, hi.x), clampf(qy, lo.y, hi.y), clampf(qz, lo.z, hi.z));
@@ -155,7 +157,10 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int ncl
                int axis_bits = 0);
 int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
-int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
+// with res_keys/res_vals the result may stay in the context's sort temporaries
+// (see radix_sort_impl); without them it is copied back into keys/vals
+int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit,
+                         uint32_t** res_keys = nullptr, uint32_t** res_vals = nullptr);
 size_t radix_sort_hist_bytes(int n);
 
 // the runtime context
@@ -167,6 +172,10 @@ struct Ctx {
     DevBuf sort_tmp[4];    // radix sort ping-pong + histograms
     DevBuf counters;       // small device counters
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    // side stream (default priority) for work that overlaps the main stream:
+    // fork/join events without timing
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     gprt_hip_run_stats stats{};
     // cached graph of the batched k-NN pipeline
     hipGraphExec_t graph_exec = nullptr;

@@ -11,6 +11,7 @@
 //   * knn_exact_kernel: FP64 traversal with an exact (d64, id) list, used as the
 //     plain single-launch form and as the fallback for queries whose guard band
 //     overflowed in the FP32 pass.
+#include <atomic>
 #include <algorithm>
 #include <cfloat>
 
@@ -286,18 +287,20 @@ struct CandList {
     }
     // insert (dd, ss), dd >= 0; returns the value pushed out of the last slot
     // (INF if that slot was empty).  Caller guarantees dd < d[C-1].
+    // The new slot j value of a sorted list is the median of the old d[j-1],
+    // d[j] and dd (one v_med3_f32); ids follow with two selects.  An entry
+    // equal to existing ones goes after them.
     __device__ __forceinline__ float insert(float dd, int ss) {
         const float ev = d[C - 1];
 #pragma unroll
         for (int j = C - 1; j >= 1; --j) {
             const bool shift = d[j - 1] > dd;
-            const bool here = !shift && d[j] > dd;
-            const float nd = shift ? d[j - 1] : (here ? dd : d[j]);
-            const int ns = shift ? s[j - 1] : (here ? ss : s[j]);
-            d[j] = nd;
-            s[j] = ns;
+            const bool here = d[j] > dd;
+            s[j] = shift ? s[j - 1] : (here ? ss : s[j]);
+            d[j] = __builtin_amdgcn_fmed3f(d[j - 1], d[j], dd);
         }
-        if (d[0] > dd) { d[0] = dd; s[0] = ss; }
+        s[0] = d[0] > dd ? ss : s[0];
+        d[0] = fminf(d[0], dd);
         return ev;
     }
 };
@@ -581,20 +584,29 @@ __device__ __forceinline__ void scan_bucket(Q4& Q, const float4* __restrict__ pt
 
 // ---------------------------------------------------------------------------
 // The same round with one independent walk per lane (closest child first).
-// The stack holds node references only (4 bytes, STACK per lane in LDS,
-// [depth][lane] so lanes hit distinct banks); a popped node that no longer
-// intersects the threshold costs one node load and prunes both children,
-// since a child box lies inside its parent's box.  A walk deeper than the
-// stack is finished by the exact FP64 kernel.
+// The stack holds (node ref, d32 of its box) pairs; a popped entry whose box
+// is already beyond the threshold is dropped without loading the node.  The S
+// most recent entries of each lane live in an LDS ring ([slot][lane], so lanes
+// hit distinct banks); when the ring is full its oldest entry moves to the
+// lane's HBM spill column, and is read back when the ring runs empty.  The
+// bottom of a closest-first stack holds the far siblings of the first
+// descent, which are mostly pruned by the time they are reached, so spills
+// cost little while a small ring keeps occupancy high.  A walk deeper than
+// S + spill_depth is finished by the exact FP64 kernel.
 // ---------------------------------------------------------------------------
-template <int C, int G, int STACK, bool COUNT, bool ROUND0>
+template <int C, int G, int S, bool COUNT, bool ROUND0>
 __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
-    __shared__ int stk[STACK][256];
+    static_assert((S & (S - 1)) == 0, "LDS ring size must be a power of two");
+    __shared__ int sref[S][256];
+    __shared__ float sdist[S][256];
     const int tid = threadIdx.x, lane = tid & 63;
     const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
     if (n_in == 0) return;
     const Node* __restrict__ nodes = a.nodes;
     const float4* __restrict__ pts = a.pts;
+    const size_t gstride = (size_t)gridDim.x * 256;
+    int2* __restrict__ spill = a.spill + (size_t)blockIdx.x * 256 + tid;
+    const int max_depth = S + a.spill_depth;
     uint64_t visits = 0, tests = 0, reranks = 0;
     while (true) {
         unsigned base = 0;
@@ -622,7 +634,8 @@ __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
                 scan_bucket(Q, pts, ~root.x, root.y);
                 if (COUNT) tests += root.y;
             } else if (root.y > 0) {
-                int sp = 0, node = root.x, steps = 0;
+                int sp = 0, lo = 0;  // stack depth; entries [0, lo) are in HBM
+                int node = root.x, steps = 0;
                 while (true) {
                     if (++steps > a.max_steps) { overflow = true; break; }
                     const Node nd = nodes[node];
@@ -632,42 +645,53 @@ __global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
                     const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
                     const bool lA = rA < 0, lB = rB < 0;
                     const bool bfirst = dB < dA;
-                    // leaf buckets are scanned at once, nearer first
-                    if (lA && lB && bfirst) {
-                        if (dB <= Q.tau) {
-                            const int f = ~rB, m = __float_as_int(nd.bhi.w);
-                            scan_bucket(Q, pts, f, m);
-                            if (COUNT) tests += m;
-                        }
-                        if (dA <= Q.tau) {
-                            const int f = ~rA, m = __float_as_int(nd.ahi.w);
-                            scan_bucket(Q, pts, f, m);
-                            if (COUNT) tests += m;
-                        }
-                    } else {
-                        if (lA && dA <= Q.tau) {
-                            const int f = ~rA, m = __float_as_int(nd.ahi.w);
-                            scan_bucket(Q, pts, f, m);
-                            if (COUNT) tests += m;
-                        }
-                        if (lB && dB <= Q.tau) {
-                            const int f = ~rB, m = __float_as_int(nd.bhi.w);
+                    // leaf buckets in range are scanned at once, from one code
+                    // site: the two buckets of a node are adjacent ranges of
+                    // the sorted points (A's then B's), so both are one range
+                    {
+                        const bool sA = lA && dA <= Q.tau, sB = lB && dB <= Q.tau;
+                        const int mA = __float_as_int(nd.ahi.w), mB = __float_as_int(nd.bhi.w);
+                        const int f = sA ? ~rA : ~rB;
+                        const int m = (sA ? mA : 0) + (sB ? mB : 0);
+                        if (m > 0) {
                             scan_bucket(Q, pts, f, m);
                             if (COUNT) tests += m;
                         }
                     }
                     const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
                     if (gA && gB) {
-                        if (sp >= STACK) { overflow = true; break; }
-                        stk[sp][tid] = bfirst ? rA : rB;
+                        if (sp >= max_depth) { overflow = true; break; }
+                        if (sp - lo == S) {  // ring full: its oldest entry goes to HBM
+                            const int j = lo & (S - 1);
+                            spill[lo * gstride] = make_int2(sref[j][tid], __float_as_int(sdist[j][tid]));
+                            ++lo;
+                        }
+                        sref[sp & (S - 1)][tid] = bfirst ? rA : rB;
+                        sdist[sp & (S - 1)][tid] = bfirst ? dA : dB;
                         ++sp;
                         node = bfirst ? rB : rA;
                         continue;
                     }
                     if (gA) { node = rA; continue; }
                     if (gB) { node = rB; continue; }
-                    if (sp == 0) break;
-                    node = stk[--sp][tid];
+                    // pop the nearest-pushed entry still inside the threshold
+                    node = -1;
+                    while (sp > 0) {
+                        --sp;
+                        int r;
+                        float d;
+                        if (sp >= lo) {
+                            r = sref[sp & (S - 1)][tid];
+                            d = sdist[sp & (S - 1)][tid];
+                        } else {
+                            const int2 e = spill[sp * gstride];
+                            lo = sp;
+                            r = e.x;
+                            d = __int_as_float(e.y);
+                        }
+                        if (d <= Q.tau) { node = r; break; }
+                    }
+                    if (node < 0) break;
                 }
             }
             const float kth = Q.kth();
@@ -814,10 +838,7 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
 // mode 1: fallback, sorted position fb_list[i]
 // mode 2: sorted positions base + i (a k class too large for the FP32 list)
 template <int KMAX, bool COUNT>
-__global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned n = a.mode == 1 ? a.counters[CNT_FALLBACK] : (unsigned)a.nq;
-    if ((unsigned)i >= n) return;
+__device__ __forceinline__ void exact_one(const ExactArgs& a, int i) {
     float qx, qy, qz;
     int g, c = 0;
     if (a.mode == 0) {
@@ -847,6 +868,13 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
         atomicAdd(&a.stats64[0], (unsigned long long)visits);
         atomicAdd(&a.stats64[1], (unsigned long long)tests);
     }
+}
+
+template <int KMAX, bool COUNT>
+__global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
+    const unsigned n = a.mode == 1 ? a.counters[CNT_FALLBACK] : (unsigned)a.nq;
+    for (unsigned ui = blockIdx.x * blockDim.x + threadIdx.x; ui < n; ui += gridDim.x * blockDim.x)
+        exact_one<KMAX, COUNT>(a, (int)ui);
 }
 
 // ---------------------------------------------------------------------------
@@ -950,8 +978,9 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
     hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb, kc,
                        k_uniform, k32, perm, flag);
     // the class bits (29..31) only need sorting when several k classes are present
-    GPRT_TRY(radix_sort_pairs_u32(ctx, k32, perm, nq, begin, sort_classes ? 32 : 29));
-    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, perm, k32, cb, nq, qs4,
+    uint32_t *sk = k32, *sv = perm;
+    GPRT_TRY(radix_sort_pairs_u32(ctx, k32, perm, nq, begin, sort_classes ? 32 : 29, &sk, &sv));
+    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, sv, sk, cb, nq, qs4,
                        qcloud, counters);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
 }
@@ -972,57 +1001,95 @@ int capacity_class(int kmax, int guard) {
     return 64;
 }
 
+static int ring_size(int ring) { return ring <= 8 ? 8 : (ring <= 16 ? 16 : 32); }
+
+int lane_spill_depth(int stack_depth, int ring) { return std::max(0, stack_depth - ring_size(ring)); }
+
+template <int C, int S>
+static int lane_grid_cs(int grid_req) {
+    if (grid_req > 0) return grid_req;
+    // persistent grid: as many blocks as the device holds at once (cached per device)
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const int slot = dev & 63;
+    int g = cache[slot].load(std::memory_order_relaxed);
+    if (g > 0) return g;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, knn_lane_kernel<C, GUARD, S, false, true>, 256, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    g = per_cu * cus;
+    cache[slot].store(g, std::memory_order_relaxed);
+    return g;
+}
+
+template <int C>
+static int lane_grid_c(int ring, int grid_req) {
+    switch (ring_size(ring)) {
+        case 8: return lane_grid_cs<C, 8>(grid_req);
+        case 16: return lane_grid_cs<C, 16>(grid_req);
+        default: return lane_grid_cs<C, 32>(grid_req);
+    }
+}
+
+int lane_round_grid(int cap, int ring, int grid_req) {
+    switch (cap) {
+        case 8: return lane_grid_c<8>(ring, grid_req);
+        case 16: return lane_grid_c<16>(ring, grid_req);
+        case 32: return lane_grid_c<32>(ring, grid_req);
+        default: return lane_grid_c<64>(ring, grid_req);
+    }
+}
+
 // The first round (all queries, identity work list) gets its own kernel symbol
 // so profiles report it apart from the short later rounds.
-template <int C, int STACK>
-static void launch_lane(hipStream_t s, const RoundArgs& a, int grid, bool count) {
-    if (grid <= 0) {
-        // persistent grid: as many blocks as the device holds at once
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, knn_lane_kernel<C, GUARD, STACK, false, true>, 256,
-                                                         0) != hipSuccess || per_cu <= 0)
-            per_cu = 4;
-        int dev = 0, cus = 256;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        grid = per_cu * cus;
-    }
+template <int C, int S>
+static void launch_lane(hipStream_t s, const RoundArgs& a, int grid_req, bool count) {
+    // later rounds hold the few queries round 0 left unresolved (often none):
+    // a quarter of the resident grid keeps their launches cheap
+    const int full = lane_grid_cs<C, S>(grid_req);
+    const int grid = (a.round == 0 || grid_req > 0) ? full : std::max(1, full / 4);
     if (a.round == 0) {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, true, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, false, true>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, true>), dim3(grid), dim3(256), 0, s, a);
     } else {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, true, false>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, STACK, false, false>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, false>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, false>), dim3(grid), dim3(256), 0, s, a);
     }
 }
 
 template <int C>
-static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, int stack, bool count) {
+static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, int ring, bool count) {
     if (trav == GPRT_TRAV_WAVE) {
         if (grid <= 0) grid = 1024;
         if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
-    } else if (stack <= 16) {
-        launch_lane<C, 16>(s, a, grid, count);
-    } else if (stack <= 24) {
-        launch_lane<C, 24>(s, a, grid, count);
     } else {
-        launch_lane<C, 32>(s, a, grid, count);
+        switch (ring_size(ring)) {
+            case 8: launch_lane<C, 8>(s, a, grid, count); break;
+            case 16: launch_lane<C, 16>(s, a, grid, count); break;
+            default: launch_lane<C, 32>(s, a, grid, count); break;
+        }
     }
 }
 
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count) {
+int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int ring, bool count) {
     switch (cap) {
-        case 8: launch_round_c<8>(s, a, grid, trav, stack, count); break;
-        case 16: launch_round_c<16>(s, a, grid, trav, stack, count); break;
-        case 32: launch_round_c<32>(s, a, grid, trav, stack, count); break;
-        default: launch_round_c<64>(s, a, grid, trav, stack, count); break;
+        case 8: launch_round_c<8>(s, a, grid, trav, ring, count); break;
+        case 16: launch_round_c<16>(s, a, grid, trav, ring, count); break;
+        case 32: launch_round_c<32>(s, a, grid, trav, ring, count); break;
+        default: launch_round_c<64>(s, a, grid, trav, ring, count); break;
     }
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }
 
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count) {
-    dim3 grid((grid_threads + 255) / 256);
+    // the fallback list (mode 1) is short or empty: a small grid strides over it
+    dim3 grid(a.mode == 1 ? std::min((grid_threads + 255) / 256, 256) : (grid_threads + 255) / 256);
     if (grid.x == 0) return GPRT_OK;
 #define LX(KM)                                                                          \
     if (count) hipLaunchKernelGGL((knn_exact_kernel<KM, true>), grid, dim3(256), 0, s, a); \

@@ -13,6 +13,7 @@ constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
 constexpr int CNT_TOTAL = 48;
 constexpr int CNT_STATS64 = 4;   // u64 statistics after the counters: visits, tests, re-rank reads
+constexpr int CNT_ALL = CNT_TOTAL + 2 * CNT_STATS64;  // words of the counter buffer
 
 // Karras depth bound: 30 Morton bits + up to 32 cloud bits... a path of the
 // radix tree has strictly increasing common-prefix lengths, so its length is
@@ -42,6 +43,8 @@ struct RoundArgs {
     const int64_t* qoff;     // per cloud first global query index
     int max_steps;           // node-visit guard (> number of internal nodes)
     int base;                // first sorted position of this k class (identity work list)
+    int2* spill;             // stack entries below the LDS ring: [level][grid lanes] of (ref, d32)
+    int spill_depth;         // levels of spill per lane
 };
 
 struct ExactArgs {
@@ -86,6 +89,11 @@ __host__ __device__ inline int k_class_cap(int cls) { return cls == 0 ? 8 : (cls
 size_t query_prep_scratch_bytes(int nclouds, int use_morton);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);
+// persistent grid (blocks) of the per-lane round for capacity cap and an LDS
+// ring of `ring` entries; grid_req > 0 is taken as given
+int lane_round_grid(int cap, int ring, int grid_req);
+// HBM spill levels per lane for a total stack depth and an LDS ring size
+int lane_spill_depth(int stack_depth, int ring);
 int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
 

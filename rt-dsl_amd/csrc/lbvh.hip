@@ -207,12 +207,17 @@ __global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nb
     const int d = threadIdx.x & 255, g = threadIdx.x >> 8;
     const int per = (nblocks + 3) / 4;
     const int b0 = min(g * per, nblocks), b1 = min(b0 + per, nblocks);
+    // both passes over the tiles issue RS_UNROLL independent loads before using
+    // any of them, so a group waits for one memory latency per RS_UNROLL tiles
+    constexpr int RS_UNROLL = 8;
     uint32_t run = 0;
     int b = b0;
-    for (; b + 4 <= b1; b += 4) {
-        uint32_t v0 = hist[(b + 0) * RS_BINS + d], v1 = hist[(b + 1) * RS_BINS + d];
-        uint32_t v2 = hist[(b + 2) * RS_BINS + d], v3 = hist[(b + 3) * RS_BINS + d];
-        run += v0 + v1 + v2 + v3;
+    for (; b + RS_UNROLL <= b1; b += RS_UNROLL) {
+        uint32_t v[RS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; ++u) run += v[u];
     }
     for (; b < b1; ++b) run += hist[b * RS_BINS + d];
     gsum[g][d] = run;
@@ -238,7 +243,17 @@ __global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nb
     if (g == 0) dtot[d] -= tot;
     __syncthreads();
     run = dtot[d] + gsum[g][d];
-    for (b = b0; b < b1; ++b) {
+    for (b = b0; b + RS_UNROLL <= b1; b += RS_UNROLL) {
+        uint32_t v[RS_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
+#pragma unroll
+        for (int u = 0; u < RS_UNROLL; ++u) {
+            hist[(b + u) * RS_BINS + d] = run;
+            run += v[u];
+        }
+    }
+    for (; b < b1; ++b) {
         uint32_t v = hist[b * RS_BINS + d];
         hist[b * RS_BINS + d] = run;
         run += v;
@@ -306,8 +321,14 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
     }
 }
 
+// res_keys / res_vals: when given, the sorted result is left in whichever of
+// (keys, vals) and the context's temporaries the last pass wrote, and these
+// receive its location (no copy back after an odd number of passes)
 template <typename K>
-static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
+static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit, K** res_keys,
+                           uint32_t** res_vals) {
+    if (res_keys) *res_keys = keys;
+    if (res_vals) *res_vals = vals;
     if (n <= 1 || begin_bit >= end_bit) return GPRT_OK;
     int nblocks = (n + RS_TILE - 1) / RS_TILE;
     GPRT_TRY(ctx->sort_tmp[0].reserve(sizeof(K) * (size_t)n));
@@ -331,7 +352,10 @@ static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_b
         std::swap(v0, v1);
     }
     GPRT_HIP_TRY(hipGetLastError());
-    if (k0 != keys) {
+    if (res_keys && res_vals) {
+        *res_keys = k0;
+        *res_vals = v0;
+    } else if (k0 != keys) {
         GPRT_HIP_TRY(hipMemcpyAsync(keys, k0, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
         GPRT_HIP_TRY(hipMemcpyAsync(vals, v0, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
     }
@@ -341,10 +365,11 @@ static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_b
 size_t radix_sort_hist_bytes(int n) { return sizeof(uint32_t) * (size_t)RS_BINS * ((n + RS_TILE - 1) / RS_TILE) + 1024; }
 
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
-    return radix_sort_impl<uint64_t>(ctx, keys, vals, n, begin_bit, end_bit);
+    return radix_sort_impl<uint64_t>(ctx, keys, vals, n, begin_bit, end_bit, nullptr, nullptr);
 }
-int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
-    return radix_sort_impl<uint32_t>(ctx, keys, vals, n, begin_bit, end_bit);
+int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit,
+                         uint32_t** res_keys, uint32_t** res_vals) {
+    return radix_sort_impl<uint32_t>(ctx, keys, vals, n, begin_bit, end_bit, res_keys, res_vals);
 }
 
 // ---------------------------------------------------------------------------

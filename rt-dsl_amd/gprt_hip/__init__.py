@@ -48,6 +48,7 @@ class Config(ctypes.Structure):
         ("block_threads", ctypes.c_int),
         ("collect_counters", ctypes.c_int),
         ("time_kernels", ctypes.c_int),
+        ("lds_stack", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -66,10 +66,13 @@ def test_knn_leaf_sizes(ctx, leaf):
     dict(use_graph=0),
     dict(stack_depth=32, grid_blocks=7),
     dict(guard_slots=1),
+    dict(lds_stack=8, stack_depth=24),   # most walks spill to HBM
+    dict(lds_stack=8, stack_depth=9),    # walks overflow: exact fallback
+    dict(lds_stack=32, stack_depth=32),  # no spill
 ])
 def test_knn_config_independence(ctx, cfgkw):
     """The result must not depend on the initial radius, multiplier, rounds,
-    query order, graph use, stack depth or grid size."""
+    query order, graph use, stack depth and LDS ring size, or grid size."""
     pts, q = D.uniform_cloud(50_000, 5_000, seed=7)
     ref = oracle.knn_brute(pts, q, 5)
     t = ctx.build(pts)
@@ -153,8 +156,10 @@ def test_knn_road_network_subset(ctx):
 
 
 def test_knn_lidar_full(ctx):
-    """The north-star workload at full size: 1,310,720 points x 400,000 queries,
-    k = 5 and k = 32, against the CPU KD-tree (itself pinned to brute force)."""
+    """The headline workload at full size: 1,310,720 points x 400,000 queries,
+    k = 5 and k = 32, against the CPU KD-tree (itself pinned to brute force);
+    k = 5 also with an 8-entry LDS ring (deep walks spill to HBM) and with
+    one unbounded closest-first round."""
     pts, q = D.lidar_scan()
     t = ctx.build(pts)
     for k in (5, 32):
@@ -164,6 +169,10 @@ def test_knn_lidar_full(ctx):
         print(f"lidar k={k}: rounds={st['rounds_run']} fallback={st['fallback_queries']} "
               f"active={st['active_per_round'][:8]} ms={st['ms_query']:.3f}")
         _check_knn(t.k_nn_plain(q, k), ref, label=f"lidar plain k={k}")
+        if k == 5:
+            for kw in (dict(lds_stack=8), dict(lds_stack=8, radius_heuristic=gprt_hip.RADIUS_INFINITE)):
+                _check_knn(t.k_nn(q, k, gprt_hip.default_config(**kw)), ref, label=f"lidar batched {kw}")
+                assert ctx.stats()["fallback_queries"] == 0
 
 
 def test_knn_device_pointers(ctx):
