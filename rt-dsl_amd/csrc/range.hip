@@ -159,9 +159,14 @@ __device__ __forceinline__ bool pair_gt(double da, uint32_t ia, double db, uint3
     return da > db || (da == db && ia > ib);
 }
 
+constexpr int SEG_LDS = 2048;  // longest segment sorted in LDS in one piece
+
+// big_list: segments longer than 64; segments longer than SEG_LDS also get a
+// range of the merge scratch (big_tmp[slot], from the running total big_total)
 __global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __restrict__ off, int nq,
                                                             uint32_t* ids, double* d2, int64_t capacity,
-                                                            int* big_list, unsigned* big_count) {
+                                                            int* big_list, int64_t* big_tmp, unsigned* big_count,
+                                                            unsigned long long* big_total) {
     const int lane = threadIdx.x & 63;
     const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (seg >= nq) return;  // whole wave leaves together
@@ -169,7 +174,11 @@ __global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __res
     const int64_t m = e - b;
     if (m <= 1) return;
     if (m > 64) {
-        if (lane == 0) big_list[atomicAdd(big_count, 1u)] = seg;
+        if (lane == 0) {
+            const unsigned slot = atomicAdd(big_count, 1u);
+            big_list[slot] = seg;
+            big_tmp[slot] = m > SEG_LDS ? (int64_t)atomicAdd(big_total, (unsigned long long)m) : -1;
+        }
         return;
     }
     double d = lane < m ? d2[b + lane] : INFINITY;
@@ -190,56 +199,92 @@ __global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __res
     }
 }
 
-// one block per long segment: bitonic network over the next power of two in
-// LDS when it fits (<= 2048), else an in-place odd-even transposition sort.
+// bitonic sort of x[0..m) (m <= SEG_LDS) through LDS, by (d64, id)
+__device__ void lds_bitonic(double* __restrict__ xd, uint32_t* __restrict__ xi, int m, double* sd, uint32_t* si) {
+    int P = 1;
+    while (P < m) P <<= 1;
+    for (int j = threadIdx.x; j < P; j += blockDim.x) {
+        sd[j] = j < m ? xd[j] : INFINITY;
+        si[j] = j < m ? xi[j] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int j = threadIdx.x; j < P; j += blockDim.x) {
+                const int p = j ^ stride;
+                if (p > j) {
+                    const bool up = (j & size) == 0;
+                    if (pair_gt(sd[j], si[j], sd[p], si[p]) == up) {
+                        double td = sd[j]; sd[j] = sd[p]; sd[p] = td;
+                        uint32_t ti = si[j]; si[j] = si[p]; si[p] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        xd[j] = sd[j];
+        xi[j] = si[j];
+    }
+    __syncthreads();
+}
+
+// number of elements of the sorted run (rd, ri)[0..n) that order before (d, i)
+__device__ __forceinline__ int rank_in(const double* __restrict__ rd, const uint32_t* __restrict__ ri, int n,
+                                       double d, uint32_t i) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (pair_gt(d, i, rd[mid], ri[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One block per segment longer than 64.  Up to SEG_LDS elements: one bitonic
+// network in LDS.  Longer: SEG_LDS-element chunks are sorted in LDS, then
+// runs are merged pairwise, ping-ponging with the segment's range of the
+// merge scratch; each element finds its output slot as its index in its own
+// run plus its rank in the partner run (binary search; (d64, id) keys are
+// distinct within a segment, so ranks never collide).
 __global__ __launch_bounds__(256) void segsort_big_kernel(const int64_t* __restrict__ off, const int* big_list,
-                                                          const unsigned* big_count, uint32_t* ids, double* d2,
-                                                          int64_t capacity) {
-    __shared__ double sd[2048];
-    __shared__ uint32_t si[2048];
+                                                          const int64_t* big_tmp, const unsigned* big_count,
+                                                          uint32_t* ids, double* d2, int64_t capacity,
+                                                          uint32_t* tmp_ids, double* tmp_d2) {
+    __shared__ double sd[SEG_LDS];
+    __shared__ uint32_t si[SEG_LDS];
     if (blockIdx.x >= *big_count) return;
     const int seg = big_list[blockIdx.x];
     const int64_t b = off[seg], e = min(off[seg + 1], capacity);
     const int m = (int)(e - b);
-    if (m <= 2048) {
-        int P = 1;
-        while (P < m) P <<= 1;
-        for (int j = threadIdx.x; j < P; j += 256) {
-            sd[j] = j < m ? d2[b + j] : INFINITY;
-            si[j] = j < m ? ids[b + j] : 0xFFFFFFFFu;
-        }
-        __syncthreads();
-        for (int size = 2; size <= P; size <<= 1)
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int j = threadIdx.x; j < P; j += 256) {
-                    const int p = j ^ stride;
-                    if (p > j) {
-                        const bool up = (j & size) == 0;
-                        if (pair_gt(sd[j], si[j], sd[p], si[p]) == up) {
-                            double td = sd[j]; sd[j] = sd[p]; sd[p] = td;
-                            uint32_t ti = si[j]; si[j] = si[p]; si[p] = ti;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        for (int j = threadIdx.x; j < m; j += 256) {
-            d2[b + j] = sd[j];
-            ids[b + j] = si[j];
-        }
+    if (m <= SEG_LDS) {
+        lds_bitonic(d2 + b, ids + b, m, sd, si);
         return;
     }
-    // long lists: odd-even transposition directly in global memory
-    for (int pass = 0; pass < m; ++pass) {
-        for (int j = 2 * threadIdx.x + (pass & 1); j + 1 < m; j += 512) {
-            const double da = d2[b + j], db = d2[b + j + 1];
-            const uint32_t ia = ids[b + j], ib = ids[b + j + 1];
-            if (pair_gt(da, ia, db, ib)) {
-                d2[b + j] = db; d2[b + j + 1] = da;
-                ids[b + j] = ib; ids[b + j + 1] = ia;
-            }
+    for (int c0 = 0; c0 < m; c0 += SEG_LDS) lds_bitonic(d2 + b + c0, ids + b + c0, min(SEG_LDS, m - c0), sd, si);
+    double* ad = d2 + b;
+    uint32_t* ai = ids + b;
+    double* bd = tmp_d2 + big_tmp[blockIdx.x];
+    uint32_t* bi = tmp_ids + big_tmp[blockIdx.x];
+    for (int w = SEG_LDS; w < m; w <<= 1) {
+        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+            const int run = j / w, pos = j - run * w;
+            const int pbase = (run ^ 1) * w;
+            const int base = min(run, run ^ 1) * w;
+            const int pn = pbase < m ? min(w, m - pbase) : 0;
+            const int r = pn ? rank_in(ad + pbase, ai + pbase, pn, ad[j], ai[j]) : 0;
+            bd[base + pos + r] = ad[j];
+            bi[base + pos + r] = ai[j];
         }
         __syncthreads();
+        double* td = ad; ad = bd; bd = td;
+        uint32_t* ti = ai; ai = bi; bi = ti;
+    }
+    if (ad != d2 + b) {
+        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+            d2[b + j] = ad[j];
+            ids[b + j] = ai[j];
+        }
     }
 }
 
@@ -286,19 +331,32 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
         return GPRT_OK;
     }
     GPRT_TRY(d2_buf.reserve(8ull * std::max<int64_t>(capacity, 1)));
-    GPRT_TRY(big_buf.reserve(4ull * nq + 16));
+    GPRT_TRY(big_buf.reserve(4ull * (nq + 2) + 8ull * nq + 16));  // segment list, then merge-scratch offsets
     hipLaunchKernelGGL(range_kernel<true>, dim3(grid), dim3(256), 0, s, T.nodes.as<Node>(), T.pts4.as<float4>(),
                        root, qs4_buf.as<float4>(), nq, r, d_radii, nullptr, d_off, d_ids, d2_buf.as<double>(), capacity,
                        max_steps, counters + CNT_ERROR);
+    // counters: [CNT_FALLBACK] long-segment count, stats64[3] their merge-scratch total
     unsigned* big_count = counters + CNT_FALLBACK;
+    unsigned long long* big_total = (unsigned long long*)(counters + CNT_TOTAL) + 3;
+    int64_t* big_tmp = (int64_t*)(big_buf.as<char>() + sizeof(int) * (((size_t)nq + 1) & ~(size_t)1));
     hipLaunchKernelGGL(segsort_small_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, d_off, nq, d_ids,
-                       d2_buf.as<double>(), capacity, big_buf.as<int>(), big_count);
-    unsigned nbig = 0;
-    GPRT_HIP_TRY(hipMemcpyAsync(&nbig, big_count, 4, hipMemcpyDeviceToHost, s));
+                       d2_buf.as<double>(), capacity, big_buf.as<int>(), big_tmp, big_count, big_total);
+    struct {
+        unsigned nbig;
+        unsigned long long total;
+    } h{0, 0};
+    GPRT_HIP_TRY(hipMemcpyAsync(&h.nbig, big_count, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(&h.total, big_total, 8, hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipStreamSynchronize(s));
-    if (nbig)
-        hipLaunchKernelGGL(segsort_big_kernel, dim3(nbig), dim3(256), 0, s, d_off, big_buf.as<int>(), big_count,
-                           d_ids, d2_buf.as<double>(), capacity);
+    if (h.nbig) {
+        if (h.total) {
+            GPRT_TRY(ctx->sort_tmp[0].reserve(8ull * h.total));
+            GPRT_TRY(ctx->sort_tmp[1].reserve(4ull * h.total));
+        }
+        hipLaunchKernelGGL(segsort_big_kernel, dim3(h.nbig), dim3(256), 0, s, d_off, big_buf.as<int>(), big_tmp,
+                           big_count, d_ids, d2_buf.as<double>(), capacity, ctx->sort_tmp[1].as<uint32_t>(),
+                           ctx->sort_tmp[0].as<double>());
+    }
     GPRT_HIP_TRY(hipGetLastError());
     return GPRT_OK;
 }

@@ -165,6 +165,21 @@ def test_range_boundary_and_lattice(ctx):
         _check_range(*t.range_query(q, float(np.float32(r))), *oracle.range_brute(pts, q, float(np.float32(r))))
 
 
+def test_range_long_segments(ctx):
+    """Result lists longer than one LDS sort (2,048) and several merge levels:
+    a dense ball around the queries, lists of ~3K to ~15K ids."""
+    rng = np.random.default_rng(12)
+    ball = rng.normal(0.0, 0.05, (15_000, 3)).astype(np.float32)
+    rest = rng.random((30_000, 3)).astype(np.float32) * 4 - 2
+    pts = np.concatenate([ball, rest])
+    q = np.concatenate([ball[:40], rest[:40], np.zeros((1, 3), np.float32)])
+    t = ctx.build(pts, leaf_size=8)
+    for r in (0.06, 0.3):
+        off, ids = t.range_query(q, r)
+        assert np.diff(off).max() > 2048
+        _check_range(off, ids, *oracle.range_brute(pts, q, r))
+
+
 def test_range_lidar_subset(ctx):
     pts, q = D.lidar_scan(frames=4, nq=20_000)
     t = ctx.build(pts)
