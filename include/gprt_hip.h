@@ -81,6 +81,7 @@ extern "C" {
 /* traversal mapping */
 #define GPRT_TRAV_THREAD 0 /* one query per lane */
 #define GPRT_TRAV_WAVE 1   /* 64 Morton-adjacent queries walk the tree together (packet) */
+#define GPRT_TRAV_WIDE 2   /* one query per lane over the 4-wide view of the tree */
 
 typedef struct gprt_hip_ctx gprt_hip_ctx;
 typedef struct gprt_hip_tree gprt_hip_tree;
@@ -124,6 +125,10 @@ typedef struct {
     int64_t point_tests;
     int64_t rerank_reads;      /* FP64 re-rank point reads (counters enabled) */
     double ms_kernel;          /* first traversal round alone (time_kernels) */
+    int64_t wave_steps;        /* per-lane rounds: sum over wavefront tasks of their longest walk in
+                                  loop iterations (counters enabled); lane steps / (64 * this) is the
+                                  SIMD efficiency of the walk loop */
+    int64_t lane_steps;        /* per-lane rounds: loop iterations summed over lanes (counters enabled) */
 } gprt_hip_run_stats;
 
 typedef struct {

@@ -26,7 +26,8 @@ HIPFLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = ["lbvh.hip", "knn.hip", "api.hip", "range.hip", "bh.hip", "legacy.hip", "autotune.hip"]
+SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hip", "api.hip", "range.hip",
+           "bh.hip", "legacy.hip", "autotune.hip"]
 
 
 def compile_one(src):

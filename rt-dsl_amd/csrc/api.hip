@@ -91,7 +91,7 @@ static void drop_extra(Ctx* c) {
                               &e->r_off, &e->bh_xyz, &e->bh_mass, &e->bh_acc})
                 b->release();
             Tree& T = e->bh_tree;
-            for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm,
+            for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm,
                               &T.inv, &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
                 b->release();
             delete e;
@@ -336,7 +336,7 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
     if (!t) return GPRT_OK;
     if (t->owner) (void)hipSetDevice(t->owner->c.device);
     Tree& T = t->t;
-    for (DevBuf* b : {&T.pts4, &T.nodes, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.inv,
+    for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.inv,
                       &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
         b->release();
     delete t;
@@ -424,13 +424,15 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     if (p.nclouds > 1) GPRT_TRY(x->qcloud.reserve(4 * nq));
     // HBM part of the per-lane traversal stacks of the persistent rounds
     if (p.cfg.traversal != GPRT_TRAV_WAVE) {
-        size_t lanes = 0;
+        size_t bytes = 0;
         for (int cl = 0; cl + 1 < N_KCLASS; ++cl)
-            if (p.class_size[cl] > 0)
-                lanes = std::max(lanes, (size_t)256 * lane_round_grid(k_class_cap(cl), p.cfg.lds_stack,
-                                                                      p.cfg.grid_blocks));
-        const int depth = lane_spill_depth(p.cfg.stack_depth, p.cfg.lds_stack);
-        if (lanes && depth > 0) GPRT_TRY(x->spill.reserve(8 * lanes * depth));
+            if (p.class_size[cl] > 0) {
+                const int cap = k_class_cap(cl);
+                const size_t lanes = (size_t)256 * lane_round_grid(cap, p.cfg.lds_stack, p.cfg.grid_blocks,
+                                                                   p.cfg.traversal);
+                bytes = std::max(bytes, 8 * lanes * lane_spill_depth(p.cfg.stack_depth, cap, p.cfg.lds_stack));
+            }
+        if (bytes) GPRT_TRY(x->spill.reserve(bytes));
     }
     // radix sort temporaries for nq keys
     GPRT_TRY(c.sort_tmp[0].reserve(8 * nq));
@@ -441,7 +443,7 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
 
 static int check_config(const gprt_hip_config& cf) {
     if (cf.max_rounds < 0) return fail(GPRT_ERR_ARG, "max_rounds < 0");
-    if (cf.traversal < GPRT_TRAV_THREAD || cf.traversal > GPRT_TRAV_WAVE) return fail(GPRT_ERR_ARG, "bad traversal");
+    if (cf.traversal < GPRT_TRAV_THREAD || cf.traversal > GPRT_TRAV_WIDE) return fail(GPRT_ERR_ARG, "bad traversal");
     if (cf.stack_depth < 1 || cf.stack_depth > 4096) return fail(GPRT_ERR_ARG, "stack_depth must be in [1, 4096]");
     if (cf.lds_stack < 1 || cf.lds_stack > 32) return fail(GPRT_ERR_ARG, "lds_stack must be in [1, 32]");
     if (cf.grid_blocks < 0 || cf.grid_blocks > (1 << 20)) return fail(GPRT_ERR_ARG, "bad grid_blocks");
@@ -488,6 +490,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     int* lists[2] = {c.ws[WS_LISTA].as<int>(), c.ws[WS_LISTB].as<int>()};
     RoundArgs ra{};
     ra.nodes = T.nodes.as<Node>();
+    ra.nodes4 = T.nodes4.as<Node4>();
     ra.pts = T.pts4.as<float4>();
     ra.roots = T.roots.as<int2>();
     ra.qs = qs4;
@@ -505,7 +508,6 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.qoff = p.d_qoff;
     ra.max_steps = T.n + 64;
     ra.spill = x->spill.as<int2>();
-    ra.spill_depth = lane_spill_depth(cfg.stack_depth, cfg.lds_stack);
     ExactArgs ea{};
     ea.nq = p.nq;
     ea.qs = qs4;
@@ -540,6 +542,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
             hipLaunchKernelGGL(class_init_kernel, dim3(1), dim3(64), 0, s, counters, (unsigned)size);
         const int cap = k_class_cap(cl);
         ra.base = p.class_begin[cl];
+        ra.spill_depth = lane_spill_depth(cfg.stack_depth, cap, cfg.lds_stack);
         for (int r = 0; r < R; ++r) {
             ra.round = r;
             ra.last = (r == R - 1);
@@ -575,7 +578,7 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
         // everything the captured kernels point at: a rebuilt tree or a grown
         // workspace changes a pointer and forces a new capture
         const Tree& T = *p.tree;
-        const void* ptrs[] = {T.nodes.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
+        const void* ptrs[] = {T.nodes.ptr, T.nodes4.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
                               c.ws[0].ptr, c.ws[1].ptr, c.ws[2].ptr, c.ws[3].ptr, c.ws[4].ptr, c.ws[5].ptr,
                               c.ws[6].ptr, c.sort_tmp[0].ptr, c.sort_tmp[1].ptr, c.sort_tmp[2].ptr,
                               c.counters.ptr, extra(&c)->qcloud.ptr, extra(&c)->spill.ptr};
@@ -628,6 +631,8 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters) {
         unsigned long long v64[CNT_STATS64];
         memcpy(v64, h + CNT_TOTAL, sizeof(v64));
         c.stats.rerank_reads = (int64_t)v64[2];
+        c.stats.wave_steps = (int64_t)v64[4];
+        c.stats.lane_steps = (int64_t)v64[5];
         if (c.kernel_timed) {
             float mk = 0;
             if (hipEventElapsedTime(&mk, c.ev2, c.ev3) == hipSuccess) c.stats.ms_kernel = mk;
@@ -1071,6 +1076,7 @@ extern "C" int gprt_hip_bh_forces(gprt_hip_ctx* h, const float* pos_mass, int64_
     if (bad) return fail(GPRT_ERR_INPUT, "bodies contain NaN/Inf, |coordinate| > 1e18 or a negative mass");
     int64_t off[2] = {0, n};
     GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
+    x->bh_tree.build_wide = false;
     GPRT_TRY(tree_build(&c, x->bh_xyz.as<float>(), off, 1, leaf_size, &x->bh_tree));
     GPRT_TRY(bh_run(&c, x->bh_tree, d_pm, (int)n, theta, eps, G, d_acc, x->bh_mass, (flags & GPRT_COLLECT_COUNTERS) != 0));
     GPRT_HIP_TRY(hipEventRecord(c.ev3, s));

@@ -64,6 +64,19 @@ struct __align__(16) Node {
     float4 alo, ahi, blo, bhi;
 };
 
+// 4-wide view of the same tree for the k-NN walk: entry i holds the children
+// of binary node i's two children (its grandchildren), or the child itself
+// when that child is a leaf bucket; a walk from a root that moves from an
+// entry to one of its internal slots only ever reads entries two binary levels
+// apart, so it sees a 4-ary tree of half the depth.  Structure of arrays over
+// the four slots, 128 bytes, read with eight 16-byte loads:
+//   ref[j] >= 0: internal binary node (its own 4-wide entry), < 0: leaf bucket
+//   at sorted point ~ref[j]; cnt[j] = points under the slot, 0 = empty slot.
+struct __align__(16) Node4 {
+    float4 lox, loy, loz, hix, hiy, hiz;
+    int4 ref, cnt;
+};
+
 // Points in Morton order: (x, y, z, id) with id = index of the point in the
 // caller's cloud (int bits in .w).
 struct Tree {
@@ -72,6 +85,7 @@ struct Tree {
     int leaf_size = 8;  // subtrees with <= leaf_size points are scanned linearly
     DevBuf pts4;        // float4[n]
     DevBuf nodes;       // Node[max(n-1,1)]
+    DevBuf nodes4;      // Node4[max(n-1,1)], the 4-wide view used by the k-NN rounds
     DevBuf roots;       // int2[nclouds]: (ref, count) of each cloud's root
     DevBuf cloud_off;   // int64[nclouds+1] point offsets per cloud (device)
     DevBuf cloud_bbox;  // float[6*nclouds]: per-cloud (min xyz, max xyz)
@@ -86,6 +100,7 @@ struct Tree {
     DevBuf bh;          // Barnes-Hut per-node data (allocated on demand)
     std::vector<int64_t> h_cloud_off;
     float h_bbox[6] = {0, 0, 0, 0, 0, 0};
+    bool build_wide = true;  // also build the 4-wide view (k-NN trees)
     int key_bits = 30;  // significant bits of the sort key
     int axis_bits = 10; // Morton bits per axis
     int max_depth_bound = 64;

@@ -11,12 +11,10 @@
 //   * knn_exact_kernel: FP64 traversal with an exact (d64, id) list, used as the
 //     plain single-launch form and as the fallback for queries whose guard band
 //     overflowed in the FP32 pass.
-#include <atomic>
 #include <algorithm>
 #include <cfloat>
 
-#include "gprt_internal.hpp"
-#include "knn_internal.hpp"
+#include "knn_device.hpp"
 
 namespace gprt {
 
@@ -263,471 +261,6 @@ __device__ __forceinline__ uint32_t quant10(float p, float mn, float ext) {
 __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-// FP32 candidate list (ascending d32), capacity C, in registers.
-//
-// The list holds C = S + k + G slots: the first S = C - k - G are -inf
-// sentinels that never move, then the k nearest candidates, then G guard slots
-// for candidates inside the band of the k-th.  The k-th candidate therefore sits
-// at the fixed index C - 1 - G whatever k is, so no slot is ever addressed with
-// a runtime index (which would push the whole list to scratch memory).
-// ---------------------------------------------------------------------------
-template <int C>
-struct CandList {
-    float d[C];
-    int s[C];
-    __device__ __forceinline__ void init(int sentinels) {
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            d[j] = j < sentinels ? -INFINITY : INFINITY;
-            s[j] = -1;
-        }
-    }
-    // insert (dd, ss), dd >= 0; returns the value pushed out of the last slot
-    // (INF if that slot was empty).  Caller guarantees dd < d[C-1].
-    // The new slot j value of a sorted list is the median of the old d[j-1],
-    // d[j] and dd (one v_med3_f32); ids follow with two selects.  An entry
-    // equal to existing ones goes after them.
-    __device__ __forceinline__ float insert(float dd, int ss) {
-        const float ev = d[C - 1];
-#pragma unroll
-        for (int j = C - 1; j >= 1; --j) {
-            const bool shift = d[j - 1] > dd;
-            const bool here = d[j] > dd;
-            s[j] = shift ? s[j - 1] : (here ? ss : s[j]);
-            d[j] = __builtin_amdgcn_fmed3f(d[j - 1], d[j], dd);
-        }
-        s[0] = d[0] > dd ? ss : s[0];
-        d[0] = fminf(d[0], dd);
-        return ev;
-    }
-};
-
-template <int C, int G>
-struct F32Query {
-    float qx, qy, qz;
-    float T0;      // round bound (radius^2), INF in the unbounded round
-    float T, tau;  // current k-th bound and guard threshold
-    float ev_min;  // smallest d32 pushed out of a full list
-    CandList<C> L;
-
-    __device__ __forceinline__ void start(float x, float y, float z, float t0, int k, bool active) {
-        qx = x; qy = y; qz = z;
-        T0 = t0;
-        T = t0;
-        tau = active ? band(t0) : -1.0f;  // inactive lanes accept nothing
-        ev_min = INFINITY;
-        L.init(C - k - G);
-    }
-    __device__ __forceinline__ void visit_point(float4 p, int s) {
-        const float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
-        if (dd <= tau) {
-            if (dd < L.d[C - 1]) {
-                ev_min = fminf(ev_min, L.insert(dd, s));
-                T = fminf(T0, L.d[C - 1 - G]);
-                tau = band(T);
-            } else {
-                ev_min = fminf(ev_min, dd);
-            }
-        }
-    }
-    __device__ __forceinline__ float kth() const { return L.d[C - 1 - G]; }
-};
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// View of read-only device memory in the constant address space: wave-uniform
-// loads through it become scalar loads (s_load_dwordx16 for a 64-byte node).
-#define GPRT_AS4 __attribute__((address_space(4)))
-template <class T>
-__device__ __forceinline__ const GPRT_AS4 T* const_view(const T* p) {
-    return (const GPRT_AS4 T*)(uintptr_t)p;
-}
-__device__ __forceinline__ float4 ld4(const GPRT_AS4 float4* p) {
-    const GPRT_AS4 float* f = (const GPRT_AS4 float*)p;
-    return make_float4(f[0], f[1], f[2], f[3]);
-}
-__device__ __forceinline__ Node ldnode(const GPRT_AS4 Node* p) {
-    const GPRT_AS4 float4* f = (const GPRT_AS4 float4*)p;
-    Node n;
-    n.alo = ld4(f);
-    n.ahi = ld4(f + 1);
-    n.blo = ld4(f + 2);
-    n.bhi = ld4(f + 3);
-    return n;
-}
-__device__ __forceinline__ int2 ld2i(const GPRT_AS4 int2* p) {
-    const GPRT_AS4 int* f = (const GPRT_AS4 int*)p;
-    return make_int2(f[0], f[1]);
-}
-
-// Work queue of a persistent round: a wave takes the next 64 queries.  The
-// head is read first, so once the queue is drained the remaining waves leave
-// without contending on the atomic.
-__device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
-    unsigned h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (h >= n_in) return h;
-    return atomicAdd(head, 64u);
-}
-
-// ---------------------------------------------------------------------------
-// one bounded (or the final unbounded) round as a persistent launch.  Each
-// wavefront takes 64 consecutive queries of the active list (Morton order, so
-// they are spatial neighbours) and walks the tree as a packet: node and leaf
-// point loads are wave-uniform (scalar loads), every lane tests them against its
-// own query and its own threshold, and a child is visited when any lane needs
-// it.  The traversal stack is one uniform LDS stack per wavefront; the radix
-// tree depth bound keeps it from overflowing.
-// ---------------------------------------------------------------------------
-// FP64 re-rank of the guard band in (d64, id) order and output of the first k.
-// Candidates are the list entries with 0 <= d32 <= tf (sentinels are -inf).
-template <int C>
-__device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float* ld, const int* ls, float qx,
-                                                 float qy, float qz, float tf, int g, int c, int k,
-                                                 const float4* __restrict__ pts) {
-    double dd[C];
-    uint32_t id[C];
-    int reads = 0;
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-        const bool in = ld[j] >= 0.0f && ld[j] <= tf && ls[j] >= 0;
-        dd[j] = INFINITY;
-        id[j] = GPRT_ID_NONE;
-        reads += in;
-        if (in) {
-            const float4 p = pts[ls[j]];
-            dd[j] = d2d(qx, qy, qz, p.x, p.y, p.z);
-            id[j] = (uint32_t)__float_as_int(p.w);
-        }
-    }
-    bool swapped = true;
-    while (swapped) {
-        swapped = false;
-#pragma unroll
-        for (int j = 0; j + 1 < C; ++j) {
-            const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
-            const double lo = gt ? dd[j + 1] : dd[j], hi = gt ? dd[j] : dd[j + 1];
-            const uint32_t li = gt ? id[j + 1] : id[j], hiid = gt ? id[j] : id[j + 1];
-            dd[j] = lo; dd[j + 1] = hi; id[j] = li; id[j + 1] = hiid;
-            swapped |= gt;
-        }
-    }
-    const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
-#pragma unroll
-    for (int j = 0; j < C; ++j)
-        if (j < k) {
-            a.ids_out[ob + j] = id[j];
-            if (a.d2_out) a.d2_out[ob + j] = dd[j];
-        }
-    return reads;
-}
-
-constexpr int PACKET_STACK = EXACT_STACK;
-
-template <int C, int G, bool COUNT>
-__global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
-    __shared__ int stk[4][PACKET_STACK];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
-    if (n_in == 0) return;
-    const GPRT_AS4 Node* nodes = const_view(a.nodes);      // uniform node loads
-    const GPRT_AS4 float4* upts = const_view(a.pts);       // uniform leaf-point loads
-    const GPRT_AS4 int2* roots = const_view(a.roots);
-    const float4* __restrict__ pts = a.pts;                // per-lane loads (re-rank)
-    uint64_t visits = 0, tests = 0, reranks = 0;
-    while (true) {
-        unsigned base = 0;
-        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
-        base = __shfl(base, 0);
-        if (base >= n_in) break;
-        const unsigned i = base + lane;
-        const bool valid = i < n_in;
-        const int s = valid ? (a.in_list ? a.in_list[i] : a.base + (int)i) : 0;
-        const float4 q = valid ? a.qs[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const int c = valid && a.qcloud ? a.qcloud[s] : 0;
-        const int k = a.kc ? a.kc[c] : a.k;
-        float R = a.last ? INFINITY : a.radius0[c];
-        if (!a.last)
-            for (int r = 0; r < a.round; ++r) R *= a.mult;
-        const float T0 = a.last ? INFINITY : R * R;
-        F32Query<C, G> Q;
-        Q.start(q.x, q.y, q.z, T0, k, false);
-        bool overflow = false;
-        // lanes of one wave may belong to different clouds (at cloud borders of
-        // the sorted order): walk once per distinct cloud
-        uint64_t todo = __ballot(valid);
-        while (todo) {
-            const int c0 = uniform(__shfl(c, __ffsll((unsigned long long)todo) - 1));
-            const bool mine = valid && c == c0;
-            todo &= ~__ballot(mine);
-            if (mine) Q.tau = band(T0);
-            const int2 root = ld2i(roots + c0);
-            if (root.y > 0 && root.x < 0) {
-                const int f = uniform(~root.x);
-                for (int t = 0; t < root.y; ++t) Q.visit_point(ld4(upts + f + t), f + t);
-                if (COUNT) tests += root.y;
-            }
-            if (root.y <= 0 || root.x < 0) {
-                Q.tau = -1.0f;  // this cloud's lanes are done
-                continue;
-            }
-            int sp = 0;
-            int node = uniform(root.x);
-            int steps = 0;
-            while (true) {
-                if (++steps > a.max_steps) { overflow = true; break; }
-                const Node nd = ldnode(nodes + node);
-                if (COUNT) ++visits;
-                const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
-                const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
-                const int rA = uniform(__float_as_int(nd.alo.w)), rB = uniform(__float_as_int(nd.blo.w));
-                const bool needA = dA <= Q.tau, needB = dB <= Q.tau;
-                const bool anyA = __ballot(needA) != 0, anyB = __ballot(needB) != 0;
-                // lanes vote for the child nearer to them
-                const int votesB = __popcll(__ballot(needB && (!needA || dB < dA)));
-                const int votesA = __popcll(__ballot(needA && (!needB || dA <= dB)));
-                const bool bfirst = votesB > votesA;
-                // leaf buckets are scanned at once
-                if (rA < 0 && rB < 0 && bfirst) {
-                    if (anyB) {
-                        const int f = ~rB, m = uniform(__float_as_int(nd.bhi.w));
-                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
-                        if (COUNT) tests += m;
-                    }
-                    if (anyA) {
-                        const int f = ~rA, m = uniform(__float_as_int(nd.ahi.w));
-                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
-                        if (COUNT) tests += m;
-                    }
-                } else {
-                    if (rA < 0 && anyA) {
-                        const int f = ~rA, m = uniform(__float_as_int(nd.ahi.w));
-                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
-                        if (COUNT) tests += m;
-                    }
-                    if (rB < 0 && anyB) {
-                        const int f = ~rB, m = uniform(__float_as_int(nd.bhi.w));
-                        for (int t = 0; t < m; ++t) Q.visit_point(ld4(upts + f + t), f + t);
-                        if (COUNT) tests += m;
-                    }
-                }
-                const bool goA = rA >= 0 && anyA, goB = rB >= 0 && anyB;
-                if (goA && goB) {
-                    if (sp >= PACKET_STACK) { overflow = true; break; }
-                    if (lane == 0) stk[w][sp] = bfirst ? rA : rB;
-                    ++sp;
-                    node = bfirst ? rB : rA;
-                    continue;
-                }
-                if (goA) { node = rA; continue; }
-                if (goB) { node = rB; continue; }
-                if (sp == 0) break;
-                --sp;
-                __builtin_amdgcn_wave_barrier();
-                node = uniform(stk[w][sp]);
-            }
-            Q.tau = -1.0f;  // this cloud's lanes are done
-            if (overflow) break;
-        }
-        // resolve
-        int status = 0;  // 0 done, 1 unresolved, 2 fallback
-        if (valid) {
-            const float kth = Q.kth();
-            const bool full = kth < INFINITY;
-            const bool resolved = a.last ? true : (full && kth <= T0);
-            if (!resolved) {
-                status = 1;
-            } else {
-                const float tf = band(kth);  // INF when the cloud holds fewer than k points
-                if (overflow || Q.ev_min <= tf) {
-                    status = 2;
-                } else {
-                    const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
-                                                       k, pts);
-                    if (COUNT) reranks += rr;
-                    if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
-                }
-            }
-        }
-        const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
-        if (slot >= 0) a.out_list[slot] = s;
-        const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
-        if (fslot >= 0) a.fb_list[fslot] = s;
-    }
-    if (COUNT) {
-        // visits/tests are wave-uniform in the packet walk: count once per wave
-        for (int o = 32; o > 0; o >>= 1) reranks += __shfl_xor(reranks, o);
-        if (lane == 0) {
-            atomicAdd(&a.stats64[0], (unsigned long long)visits);
-            atomicAdd(&a.stats64[1], (unsigned long long)tests);
-            atomicAdd(&a.stats64[2], (unsigned long long)reranks);
-        }
-    }
-}
-
-// Scan a leaf bucket [f, f+m) issuing the point loads four at a time, so a
-// bucket costs one memory latency per four points instead of one per point.
-template <class Q4>
-__device__ __forceinline__ void scan_bucket(Q4& Q, const float4* __restrict__ pts, int f, int m) {
-    for (int t0 = 0; t0 < m; t0 += 4) {
-        float4 p[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (t0 + u < m) p[u] = pts[f + t0 + u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (t0 + u < m) Q.visit_point(p[u], f + t0 + u);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// The same round with one independent walk per lane (closest child first).
-// The stack holds (node ref, d32 of its box) pairs; a popped entry whose box
-// is already beyond the threshold is dropped without loading the node.  The S
-// most recent entries of each lane live in an LDS ring ([slot][lane], so lanes
-// hit distinct banks); when the ring is full its oldest entry moves to the
-// lane's HBM spill column, and is read back when the ring runs empty.  The
-// bottom of a closest-first stack holds the far siblings of the first
-// descent, which are mostly pruned by the time they are reached, so spills
-// cost little while a small ring keeps occupancy high.  A walk deeper than
-// S + spill_depth is finished by the exact FP64 kernel.
-// ---------------------------------------------------------------------------
-template <int C, int G, int S, bool COUNT, bool ROUND0>
-__global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
-    static_assert((S & (S - 1)) == 0, "LDS ring size must be a power of two");
-    __shared__ int sref[S][256];
-    __shared__ float sdist[S][256];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
-    if (n_in == 0) return;
-    const Node* __restrict__ nodes = a.nodes;
-    const float4* __restrict__ pts = a.pts;
-    const size_t gstride = (size_t)gridDim.x * 256;
-    int2* __restrict__ spill = a.spill + (size_t)blockIdx.x * 256 + tid;
-    const int max_depth = S + a.spill_depth;
-    uint64_t visits = 0, tests = 0, reranks = 0;
-    while (true) {
-        unsigned base = 0;
-        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
-        base = __shfl(base, 0);
-        if (base >= n_in) break;
-        const unsigned i = base + lane;
-        const bool valid = i < n_in;
-        int status = 0;
-        int s = 0;
-        if (valid) {
-            s = (!ROUND0 && a.in_list) ? a.in_list[i] : a.base + (int)i;
-            const float4 q = a.qs[s];
-            const int c = a.qcloud ? a.qcloud[s] : 0;
-            const int k = a.kc ? a.kc[c] : a.k;
-            float R = a.last ? INFINITY : a.radius0[c];
-            if (!a.last)
-                for (int r = 0; r < a.round; ++r) R *= a.mult;
-            const float T0 = a.last ? INFINITY : R * R;
-            F32Query<C, G> Q;
-            Q.start(q.x, q.y, q.z, T0, k, true);
-            bool overflow = false;
-            const int2 root = a.roots[c];
-            if (root.y > 0 && root.x < 0) {
-                scan_bucket(Q, pts, ~root.x, root.y);
-                if (COUNT) tests += root.y;
-            } else if (root.y > 0) {
-                int sp = 0, lo = 0;  // stack depth; entries [0, lo) are in HBM
-                int node = root.x, steps = 0;
-                while (true) {
-                    if (++steps > a.max_steps) { overflow = true; break; }
-                    const Node nd = nodes[node];
-                    if (COUNT) ++visits;
-                    const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
-                    const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
-                    const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
-                    const bool lA = rA < 0, lB = rB < 0;
-                    const bool bfirst = dB < dA;
-                    // leaf buckets in range are scanned at once, from one code
-                    // site: the two buckets of a node are adjacent ranges of
-                    // the sorted points (A's then B's), so both are one range
-                    {
-                        const bool sA = lA && dA <= Q.tau, sB = lB && dB <= Q.tau;
-                        const int mA = __float_as_int(nd.ahi.w), mB = __float_as_int(nd.bhi.w);
-                        const int f = sA ? ~rA : ~rB;
-                        const int m = (sA ? mA : 0) + (sB ? mB : 0);
-                        if (m > 0) {
-                            scan_bucket(Q, pts, f, m);
-                            if (COUNT) tests += m;
-                        }
-                    }
-                    const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
-                    if (gA && gB) {
-                        if (sp >= max_depth) { overflow = true; break; }
-                        if (sp - lo == S) {  // ring full: its oldest entry goes to HBM
-                            const int j = lo & (S - 1);
-                            spill[lo * gstride] = make_int2(sref[j][tid], __float_as_int(sdist[j][tid]));
-                            ++lo;
-                        }
-                        sref[sp & (S - 1)][tid] = bfirst ? rA : rB;
-                        sdist[sp & (S - 1)][tid] = bfirst ? dA : dB;
-                        ++sp;
-                        node = bfirst ? rB : rA;
-                        continue;
-                    }
-                    if (gA) { node = rA; continue; }
-                    if (gB) { node = rB; continue; }
-                    // pop the nearest-pushed entry still inside the threshold
-                    node = -1;
-                    while (sp > 0) {
-                        --sp;
-                        int r;
-                        float d;
-                        if (sp >= lo) {
-                            r = sref[sp & (S - 1)][tid];
-                            d = sdist[sp & (S - 1)][tid];
-                        } else {
-                            const int2 e = spill[sp * gstride];
-                            lo = sp;
-                            r = e.x;
-                            d = __int_as_float(e.y);
-                        }
-                        if (d <= Q.tau) { node = r; break; }
-                    }
-                    if (node < 0) break;
-                }
-            }
-            const float kth = Q.kth();
-            const bool full = kth < INFINITY;
-            const bool resolved = a.last ? true : (full && kth <= T0);
-            if (!resolved) {
-                status = 1;
-            } else {
-                const float tf = band(kth);
-                if (overflow || Q.ev_min <= tf) {
-                    status = 2;
-                } else {
-                    const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
-                                                       k, pts);
-                    if (COUNT) reranks += rr;
-                    if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
-                }
-            }
-        }
-        const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
-        if (slot >= 0) a.out_list[slot] = s;
-        const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
-        if (fslot >= 0) a.fb_list[fslot] = s;
-    }
-    if (COUNT) {
-        for (int o = 32; o > 0; o >>= 1) {
-            visits += __shfl_xor(visits, o);
-            tests += __shfl_xor(tests, o);
-            reranks += __shfl_xor(reranks, o);
-        }
-        if (lane == 0) {
-            atomicAdd(&a.stats64[0], (unsigned long long)visits);
-            atomicAdd(&a.stats64[1], (unsigned long long)tests);
-            atomicAdd(&a.stats64[2], (unsigned long long)reranks);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -990,8 +523,6 @@ int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
     return GPRT_OK;
 }
 
-constexpr int GUARD = 3;
-
 int capacity_class(int kmax, int guard) {
     (void)guard;
     int need = kmax + GUARD;
@@ -1001,89 +532,18 @@ int capacity_class(int kmax, int guard) {
     return 64;
 }
 
-static int ring_size(int ring) { return ring <= 8 ? 8 : (ring <= 16 ? 16 : 32); }
-
-int lane_spill_depth(int stack_depth, int ring) { return std::max(0, stack_depth - ring_size(ring)); }
-
-template <int C, int S>
-static int lane_grid_cs(int grid_req) {
-    if (grid_req > 0) return grid_req;
-    // persistent grid: as many blocks as the device holds at once (cached per device)
-    static std::atomic<int> cache[64];
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    const int slot = dev & 63;
-    int g = cache[slot].load(std::memory_order_relaxed);
-    if (g > 0) return g;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, knn_lane_kernel<C, GUARD, S, false, true>, 256, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    g = per_cu * cus;
-    cache[slot].store(g, std::memory_order_relaxed);
-    return g;
+int lane_spill_depth(int stack_depth, int cap, int ring) {
+    return std::max(0, stack_depth - effective_ring(cap, ring));
 }
 
-template <int C>
-static int lane_grid_c(int ring, int grid_req) {
-    switch (ring_size(ring)) {
-        case 8: return lane_grid_cs<C, 8>(grid_req);
-        case 16: return lane_grid_cs<C, 16>(grid_req);
-        default: return lane_grid_cs<C, 32>(grid_req);
-    }
-}
-
-int lane_round_grid(int cap, int ring, int grid_req) {
-    switch (cap) {
-        case 8: return lane_grid_c<8>(ring, grid_req);
-        case 16: return lane_grid_c<16>(ring, grid_req);
-        case 32: return lane_grid_c<32>(ring, grid_req);
-        default: return lane_grid_c<64>(ring, grid_req);
-    }
-}
-
-// The first round (all queries, identity work list) gets its own kernel symbol
-// so profiles report it apart from the short later rounds.
-template <int C, int S>
-static void launch_lane(hipStream_t s, const RoundArgs& a, int grid_req, bool count) {
-    // later rounds hold the few queries round 0 left unresolved (often none):
-    // a quarter of the resident grid keeps their launches cheap
-    const int full = lane_grid_cs<C, S>(grid_req);
-    const int grid = (a.round == 0 || grid_req > 0) ? full : std::max(1, full / 4);
-    if (a.round == 0) {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, true>), dim3(grid), dim3(256), 0, s, a);
-    } else {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, false>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, false>), dim3(grid), dim3(256), 0, s, a);
-    }
-}
-
-template <int C>
-static void launch_round_c(hipStream_t s, const RoundArgs& a, int grid, int trav, int ring, bool count) {
-    if (trav == GPRT_TRAV_WAVE) {
-        if (grid <= 0) grid = 1024;
-        if (count) hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_packet_kernel<C, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
-    } else {
-        switch (ring_size(ring)) {
-            case 8: launch_lane<C, 8>(s, a, grid, count); break;
-            case 16: launch_lane<C, 16>(s, a, grid, count); break;
-            default: launch_lane<C, 32>(s, a, grid, count); break;
-        }
-    }
+int lane_round_grid(int cap, int ring, int grid_req, int trav) {
+    return trav == GPRT_TRAV_WIDE ? wide_grid(cap, ring, grid_req) : lane_grid(cap, ring, grid_req);
 }
 
 int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int ring, bool count) {
-    switch (cap) {
-        case 8: launch_round_c<8>(s, a, grid, trav, ring, count); break;
-        case 16: launch_round_c<16>(s, a, grid, trav, ring, count); break;
-        case 32: launch_round_c<32>(s, a, grid, trav, ring, count); break;
-        default: launch_round_c<64>(s, a, grid, trav, ring, count); break;
-    }
+    if (trav == GPRT_TRAV_WAVE) launch_packet_round(s, a, cap, grid, count);
+    else if (trav == GPRT_TRAV_WIDE) launch_wide_round(s, a, cap, ring, grid, count);
+    else launch_lane_round(s, a, cap, ring, grid, count);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }
 

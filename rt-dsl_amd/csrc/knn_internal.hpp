@@ -12,7 +12,8 @@ constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
 constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
 constexpr int CNT_TOTAL = 48;
-constexpr int CNT_STATS64 = 4;   // u64 statistics after the counters: visits, tests, re-rank reads
+constexpr int CNT_STATS64 = 6;   // u64 statistics after the counters: visits, tests, re-rank reads,
+                                 // (spare), wave steps, lane steps
 constexpr int CNT_ALL = CNT_TOTAL + 2 * CNT_STATS64;  // words of the counter buffer
 
 // Karras depth bound: 30 Morton bits + up to 32 cloud bits... a path of the
@@ -22,6 +23,7 @@ constexpr int EXACT_STACK = 96;
 
 struct RoundArgs {
     const Node* nodes;
+    const Node4* nodes4;     // 4-wide view (wide per-lane walk)
     const float4* pts;
     const int2* roots;
     const float4* qs;        // queries in processing order, w = global query index
@@ -91,9 +93,17 @@ int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
 int capacity_class(int kmax, int guard);
 // persistent grid (blocks) of the per-lane round for capacity cap and an LDS
 // ring of `ring` entries; grid_req > 0 is taken as given
-int lane_round_grid(int cap, int ring, int grid_req);
-// HBM spill levels per lane for a total stack depth and an LDS ring size
-int lane_spill_depth(int stack_depth, int ring);
+int lane_round_grid(int cap, int ring, int grid_req, int trav);
+// HBM spill levels per lane for a total stack depth, a capacity class and the
+// configured LDS ring size
+int lane_spill_depth(int stack_depth, int cap, int ring);
+// per-file round launchers and resident grids (knn_lane.hip, knn_wide.hip,
+// knn_packet.hip)
+int lane_grid(int cap, int ring, int grid_req);
+int wide_grid(int cap, int ring, int grid_req);
+void launch_lane_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count);
+void launch_wide_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count);
+void launch_packet_round(hipStream_t s, const RoundArgs& a, int cap, int grid, bool count);
 int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
 

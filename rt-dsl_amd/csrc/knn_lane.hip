@@ -1,0 +1,223 @@
+// knn_lane.hip -- one TrueKNN round (gprt-macros/src/lib.rs:466-594) as a
+// persistent launch with one query per lane over the binary LBVH.
+#include <algorithm>
+#include <atomic>
+
+#include "knn_device.hpp"
+
+namespace gprt {
+
+// ---------------------------------------------------------------------------
+// The same round with one independent walk per lane (closest child first).
+// The stack holds (node ref, d32 of its box) pairs; a popped entry whose box
+// is already beyond the threshold is dropped without loading the node.  The S
+// most recent entries of each lane live in an LDS ring ([slot][lane], so lanes
+// hit distinct banks); when the ring is full its oldest entry moves to the
+// lane's HBM spill column, and is read back when the ring runs empty.  The
+// bottom of a closest-first stack holds the far siblings of the first
+// descent, which are mostly pruned by the time they are reached, so spills
+// cost little while a small ring keeps occupancy high.  A walk deeper than
+// S + spill_depth is finished by the exact FP64 kernel.
+// ---------------------------------------------------------------------------
+template <int C, int G, int S, bool COUNT, bool ROUND0>
+__global__ __launch_bounds__(256) void knn_lane_kernel(RoundArgs a) {
+    static_assert((S & (S - 1)) == 0, "LDS ring size must be a power of two");
+    __shared__ int sref[S][256];
+    __shared__ float sdist[S][256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
+    if (n_in == 0) return;
+    const Node* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ pts = a.pts;
+    const size_t gstride = (size_t)gridDim.x * 256;
+    int2* __restrict__ spill = a.spill + (size_t)blockIdx.x * 256 + tid;
+    const int max_depth = S + a.spill_depth;
+    uint64_t visits = 0, tests = 0, reranks = 0, lane_steps = 0, wave_steps = 0;
+    while (true) {
+        unsigned base = 0;
+        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
+        base = __shfl(base, 0);
+        if (base >= n_in) break;
+        const unsigned i = base + lane;
+        const bool valid = i < n_in;
+        int task_steps = 0;
+        int status = 0;
+        int s = 0;
+        if (valid) {
+            s = (!ROUND0 && a.in_list) ? a.in_list[i] : a.base + (int)i;
+            const float4 q = a.qs[s];
+            const int c = a.qcloud ? a.qcloud[s] : 0;
+            const int k = a.kc ? a.kc[c] : a.k;
+            float R = a.last ? INFINITY : a.radius0[c];
+            if (!a.last)
+                for (int r = 0; r < a.round; ++r) R *= a.mult;
+            const float T0 = a.last ? INFINITY : R * R;
+            F32Query<C, G> Q;
+            Q.start(q.x, q.y, q.z, T0, k, true);
+            bool overflow = false;
+            const int2 root = a.roots[c];
+            if (root.y > 0 && root.x < 0) {
+                scan_bucket(Q, pts, ~root.x, root.y);
+                if (COUNT) tests += root.y;
+            } else if (root.y > 0) {
+                int sp = 0, lo = 0;  // stack depth; entries [0, lo) are in HBM
+                int node = root.x, steps = 0;
+                while (true) {
+                    if (++steps > a.max_steps) { overflow = true; break; }
+                    const Node nd = nodes[node];
+                    if (COUNT) ++visits;
+                    const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
+                    const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
+                    const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                    const bool lA = rA < 0, lB = rB < 0;
+                    const bool bfirst = dB < dA;
+                    // leaf buckets in range are scanned at once, from one code
+                    // site: the two buckets of a node are adjacent ranges of
+                    // the sorted points (A's then B's), so both are one range
+                    {
+                        const bool sA = lA && dA <= Q.tau, sB = lB && dB <= Q.tau;
+                        const int mA = __float_as_int(nd.ahi.w), mB = __float_as_int(nd.bhi.w);
+                        const int f = sA ? ~rA : ~rB;
+                        const int m = (sA ? mA : 0) + (sB ? mB : 0);
+                        if (m > 0) {
+                            scan_bucket(Q, pts, f, m);
+                            if (COUNT) tests += m;
+                        }
+                    }
+                    const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
+                    if (gA && gB) {
+                        if (sp >= max_depth) { overflow = true; break; }
+                        if (sp - lo == S) {  // ring full: its oldest entry goes to HBM
+                            const int j = lo & (S - 1);
+                            spill[lo * gstride] = make_int2(sref[j][tid], __float_as_int(sdist[j][tid]));
+                            ++lo;
+                        }
+                        sref[sp & (S - 1)][tid] = bfirst ? rA : rB;
+                        sdist[sp & (S - 1)][tid] = bfirst ? dA : dB;
+                        ++sp;
+                        node = bfirst ? rB : rA;
+                        continue;
+                    }
+                    if (gA) { node = rA; continue; }
+                    if (gB) { node = rB; continue; }
+                    // pop the nearest-pushed entry still inside the threshold
+                    node = -1;
+                    while (sp > 0) {
+                        --sp;
+                        int r;
+                        float d;
+                        if (sp >= lo) {
+                            r = sref[sp & (S - 1)][tid];
+                            d = sdist[sp & (S - 1)][tid];
+                        } else {
+                            const int2 e = spill[sp * gstride];
+                            lo = sp;
+                            r = e.x;
+                            d = __int_as_float(e.y);
+                        }
+                        if (d <= Q.tau) { node = r; break; }
+                    }
+                    if (node < 0) break;
+                }
+                task_steps = steps;
+            }
+            const float kth = Q.kth();
+            const bool full = kth < INFINITY;
+            const bool resolved = a.last ? true : (full && kth <= T0);
+            if (!resolved) {
+                status = 1;
+            } else {
+                const float tf = band(kth);
+                if (overflow || Q.ev_min <= tf) {
+                    status = 2;
+                } else {
+                    const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
+                                                       k, pts);
+                    if (COUNT) reranks += rr;
+                    if (!full) atomicAdd(&a.counters[CNT_PARTIAL], 1u);
+                }
+            }
+        }
+        if (COUNT) {
+            int mx = task_steps;
+            for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+            lane_steps += task_steps;
+            if (lane == 0) wave_steps += mx;
+        }
+        const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
+        if (slot >= 0) a.out_list[slot] = s;
+        const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
+        if (fslot >= 0) a.fb_list[fslot] = s;
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            visits += __shfl_xor(visits, o);
+            tests += __shfl_xor(tests, o);
+            reranks += __shfl_xor(reranks, o);
+            lane_steps += __shfl_xor(lane_steps, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats64[0], (unsigned long long)visits);
+            atomicAdd(&a.stats64[1], (unsigned long long)tests);
+            atomicAdd(&a.stats64[2], (unsigned long long)reranks);
+            atomicAdd(&a.stats64[4], (unsigned long long)wave_steps);
+            atomicAdd(&a.stats64[5], (unsigned long long)lane_steps);
+        }
+    }
+}
+
+
+template <int C, int S>
+static int grid_cs(int grid_req) {
+    if (grid_req > 0) return grid_req;
+    static std::atomic<int> cache[64];
+    return resident_grid(knn_lane_kernel<C, GUARD, S, false, true>, cache);
+}
+
+int lane_grid(int cap, int ring, int grid_req) {
+    switch (cap) {
+        case 8:
+            switch (ring_size(ring)) {
+                case 8: return grid_cs<8, 8>(grid_req);
+                case 16: return grid_cs<8, 16>(grid_req);
+                default: return grid_cs<8, 32>(grid_req);
+            }
+        case 16: return grid_cs<16, 8>(grid_req);
+        case 32: return grid_cs<32, 8>(grid_req);
+        default: return grid_cs<64, 8>(grid_req);
+    }
+}
+
+// The first round (all queries, identity work list) gets its own kernel symbol
+// so profiles report it apart from the short later rounds, which hold the few
+// queries round 0 left unresolved (often none) and run on a quarter of the
+// resident grid to keep their launches cheap.
+template <int C, int S>
+static void launch_cs(hipStream_t s, const RoundArgs& a, int grid_req, bool count) {
+    const int full = grid_cs<C, S>(grid_req);
+    const int grid = (a.round == 0 || grid_req > 0) ? full : std::max(1, full / 4);
+    if (a.round == 0) {
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, true>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, false>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, false>), dim3(grid), dim3(256), 0, s, a);
+    }
+}
+
+void launch_lane_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count) {
+    switch (cap) {
+        case 8:
+            switch (ring_size(ring)) {
+                case 8: launch_cs<8, 8>(s, a, grid_req, count); break;
+                case 16: launch_cs<8, 16>(s, a, grid_req, count); break;
+                default: launch_cs<8, 32>(s, a, grid_req, count); break;
+            }
+            break;
+        case 16: launch_cs<16, 8>(s, a, grid_req, count); break;
+        case 32: launch_cs<32, 8>(s, a, grid_req, count); break;
+        default: launch_cs<64, 8>(s, a, grid_req, count); break;
+    }
+}
+
+}  // namespace gprt

@@ -440,6 +440,52 @@ __global__ void karras_kernel(const uint64_t* __restrict__ keys, int cshift, int
     if ((int64_t)lo == off[c] && (int64_t)hi == off[c + 1] - 1 && (hi - lo + 1) > leaf_size) roots[c] = make_int2(i, hi - lo + 1);
 }
 
+// 4-wide view: entry i = the slots of binary node i's children, a leaf-bucket
+// child standing for itself in its first slot (the second stays empty)
+__global__ void collapse4_kernel(const Node* __restrict__ nodes, int m, Node4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const Node nd = nodes[i];
+    float lo[4][3], hi[4][3];
+    int ref[4], cnt[4];
+    for (int h = 0; h < 2; ++h) {
+        const float4 clo = h ? nd.blo : nd.alo, chi = h ? nd.bhi : nd.ahi;
+        const int r = __float_as_int(clo.w);
+        if (r >= 0) {
+            const Node c = nodes[r];
+            const float4 l0 = c.alo, h0 = c.ahi, l1 = c.blo, h1 = c.bhi;
+            lo[2 * h][0] = l0.x; lo[2 * h][1] = l0.y; lo[2 * h][2] = l0.z;
+            hi[2 * h][0] = h0.x; hi[2 * h][1] = h0.y; hi[2 * h][2] = h0.z;
+            ref[2 * h] = __float_as_int(l0.w);
+            cnt[2 * h] = __float_as_int(h0.w);
+            lo[2 * h + 1][0] = l1.x; lo[2 * h + 1][1] = l1.y; lo[2 * h + 1][2] = l1.z;
+            hi[2 * h + 1][0] = h1.x; hi[2 * h + 1][1] = h1.y; hi[2 * h + 1][2] = h1.z;
+            ref[2 * h + 1] = __float_as_int(l1.w);
+            cnt[2 * h + 1] = __float_as_int(h1.w);
+        } else {
+            lo[2 * h][0] = clo.x; lo[2 * h][1] = clo.y; lo[2 * h][2] = clo.z;
+            hi[2 * h][0] = chi.x; hi[2 * h][1] = chi.y; hi[2 * h][2] = chi.z;
+            ref[2 * h] = r;
+            cnt[2 * h] = __float_as_int(chi.w);
+            // empty slot: a point box at the first child's corner, never entered (cnt 0)
+            lo[2 * h + 1][0] = clo.x; lo[2 * h + 1][1] = clo.y; lo[2 * h + 1][2] = clo.z;
+            hi[2 * h + 1][0] = clo.x; hi[2 * h + 1][1] = clo.y; hi[2 * h + 1][2] = clo.z;
+            ref[2 * h + 1] = -1;
+            cnt[2 * h + 1] = 0;
+        }
+    }
+    Node4 o;
+    o.lox = make_float4(lo[0][0], lo[1][0], lo[2][0], lo[3][0]);
+    o.loy = make_float4(lo[0][1], lo[1][1], lo[2][1], lo[3][1]);
+    o.loz = make_float4(lo[0][2], lo[1][2], lo[2][2], lo[3][2]);
+    o.hix = make_float4(hi[0][0], hi[1][0], hi[2][0], hi[3][0]);
+    o.hiy = make_float4(hi[0][1], hi[1][1], hi[2][1], hi[3][1]);
+    o.hiz = make_float4(hi[0][2], hi[1][2], hi[2][2], hi[3][2]);
+    o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    o.cnt = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
+    out[i] = o;
+}
+
 __global__ void roots_small_kernel(const int64_t* __restrict__ off, int nclouds, int leaf_size, int2* roots) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < nclouds) {
@@ -492,6 +538,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     GPRT_TRY(t->roots.reserve(sizeof(int2) * nclouds));
     GPRT_TRY(t->pts4.reserve(sizeof(float4) * std::max(n, 1)));
     GPRT_TRY(t->nodes.reserve(sizeof(Node) * std::max(n - 1, 1)));
+    if (t->build_wide) GPRT_TRY(t->nodes4.reserve(sizeof(Node4) * std::max(n - 1, 1)));
     GPRT_TRY(t->perm.reserve(sizeof(uint32_t) * std::max(n, 1)));
     GPRT_TRY(t->inv.reserve(sizeof(uint32_t) * std::max(n, 1)));
     GPRT_TRY(t->codes.reserve(sizeof(uint64_t) * std::max(n, 1)));
@@ -537,6 +584,9 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
         BoxSlots slots{t->nodes.as<Node>(), t->pts4.as<float4>()};
         launch_refit<BoxP>(s, slots, t->parent.as<int>(), t->ranges.as<int2>(), n, t->flags.as<int>(), ready,
                            (unsigned*)(ready + n));
+        if (t->build_wide)
+            hipLaunchKernelGGL(collapse4_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->nodes.as<Node>(), n - 1,
+                               t->nodes4.as<Node4>());
     }
     GPRT_HIP_TRY(hipGetLastError());
     GPRT_HIP_TRY(hipMemcpyAsync(t->h_bbox, t->cloud_bbox.ptr, sizeof(float) * 6, hipMemcpyDeviceToHost, s));

@@ -23,7 +23,7 @@ PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30 = 0, 1, 2, 4, 8
 ID_NONE = 0xFFFFFFFF
 RADIUS_MAX, RADIUS_PERCENTILE, RADIUS_FIXED, RADIUS_INFINITE = 0, 1, 2, 3
 SAMPLER_LCG, SAMPLER_STRIDED = 0, 1
-TRAV_THREAD, TRAV_WAVE = 0, 1
+TRAV_THREAD, TRAV_WAVE, TRAV_WIDE = 0, 1, 2
 
 
 class GprtError(RuntimeError):
@@ -70,6 +70,8 @@ class RunStats(ctypes.Structure):
         ("point_tests", ctypes.c_int64),
         ("rerank_reads", ctypes.c_int64),
         ("ms_kernel", ctypes.c_double),
+        ("wave_steps", ctypes.c_int64),
+        ("lane_steps", ctypes.c_int64),
     ]
 
     def as_dict(self):

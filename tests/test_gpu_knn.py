@@ -69,6 +69,10 @@ def test_knn_leaf_sizes(ctx, leaf):
     dict(lds_stack=8, stack_depth=24),   # most walks spill to HBM
     dict(lds_stack=8, stack_depth=9),    # walks overflow: exact fallback
     dict(lds_stack=32, stack_depth=32),  # no spill
+    dict(traversal=gprt_hip.TRAV_WIDE),
+    dict(traversal=gprt_hip.TRAV_WIDE, lds_stack=8, stack_depth=10),  # wide walks overflow: exact fallback
+    dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_INFINITE, lds_stack=16),
+    dict(traversal=gprt_hip.TRAV_WAVE),
 ])
 def test_knn_config_independence(ctx, cfgkw):
     """The result must not depend on the initial radius, multiplier, rounds,
@@ -170,7 +174,8 @@ def test_knn_lidar_full(ctx):
               f"active={st['active_per_round'][:8]} ms={st['ms_query']:.3f}")
         _check_knn(t.k_nn_plain(q, k), ref, label=f"lidar plain k={k}")
         if k == 5:
-            for kw in (dict(lds_stack=8), dict(lds_stack=8, radius_heuristic=gprt_hip.RADIUS_INFINITE)):
+            for kw in (dict(lds_stack=8), dict(lds_stack=8, radius_heuristic=gprt_hip.RADIUS_INFINITE),
+                       dict(traversal=gprt_hip.TRAV_WIDE), dict(traversal=gprt_hip.TRAV_WIDE, lds_stack=16)):
                 _check_knn(t.k_nn(q, k, gprt_hip.default_config(**kw)), ref, label=f"lidar batched {kw}")
                 assert ctx.stats()["fallback_queries"] == 0
 
