@@ -109,6 +109,9 @@ typedef struct {
     int time_kernels;     /* with use_graph == 0: time the first traversal round on its own */
     int lds_stack;        /* most recent stack entries kept in LDS per query: 8, 16 or 32
                              (default 8); older ones spill to an HBM workspace */
+    int compact_rounds;   /* 1: one launch per round over the compacted list of unresolved queries
+                             (the reference's round structure); 0 (default): an unresolved query
+                             retries with the next radius inside the first round's launch */
 } gprt_hip_config;
 
 typedef struct {

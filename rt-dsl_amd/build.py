@@ -30,9 +30,9 @@ SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hi
            "bh.hip", "legacy.hip", "autotune.hip"]
 
 
-def compile_one(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def compile_one(src, build_dir=BUILD, defines=()):
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + HIPFLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -40,26 +40,32 @@ def compile_one(src):
     return obj, err
 
 
-def build(jobs=None, verbose=True):
-    os.makedirs(BUILD, exist_ok=True)
+def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=()):
+    """Compile every translation unit and link the library.  `defines` are
+    extra -D options (tuning constants of the kernels), for builds of the
+    same library into another `out` / `build_dir`."""
+    os.makedirs(build_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(compile_one, srcs))
+        results = list(ex.map(lambda src: compile_one(src, build_dir, defines), srcs))
     for obj, err in results:
         if err.strip() and verbose:
             print(err, file=sys.stderr)
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
     subprocess.check_call(cmd)
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print("built", OUT)
-    return OUT
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor define")
+    ap.add_argument("--out", default=OUT)
+    ap.add_argument("--build-dir", default=BUILD)
     a = ap.parse_args()
-    build(a.j)
+    build(a.j, out=a.out, build_dir=a.build_dir, defines=a.defines)

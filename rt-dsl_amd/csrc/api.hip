@@ -169,6 +169,7 @@ extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     c->collect_counters = 0;
     c->time_kernels = 0;
     c->lds_stack = 8;
+    c->compact_rounds = 0;
 }
 
 extern "C" int gprt_hip_num_devices(int* out) {
@@ -543,7 +544,13 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
         const int cap = k_class_cap(cl);
         ra.base = p.class_begin[cl];
         ra.spill_depth = lane_spill_depth(cfg.stack_depth, cap, cfg.lds_stack);
-        for (int r = 0; r < R; ++r) {
+        // later rounds inside the first launch (a lane retries its unresolved
+        // query with the next radius) unless the schedule asks for one launch
+        // per round over the compacted unresolved list; packets always compact
+        const int inl = (!cfg.compact_rounds && cfg.traversal != GPRT_TRAV_WAVE) ? 1 : 0;
+        ra.rounds = R;
+        ra.inline_rounds = inl;
+        for (int r = 0; r < (inl ? 1 : R); ++r) {
             ra.round = r;
             ra.last = (r == R - 1);
             ra.in_list = r == 0 ? nullptr : lists[r & 1];

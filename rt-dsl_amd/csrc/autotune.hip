@@ -5,7 +5,7 @@
 // change when every query got k ids and the run was faster) and the funnel
 // profiler (gprt-profiler/src/main.rs:37-133: grid over percentile x multiplier
 // x cap on growing subsets, saturation gate).  Here the knobs are the ones that
-// matter on the GPU -- traversal mapping, leaf size, LDS stack ring size, initial-radius rule
+// matter on the GPU -- traversal mapping, leaf size, LDS stack ring size, round structure, initial-radius rule
 // (percentile / unbounded) and growth multiplier -- and the gate is stronger:
 // every candidate's ids must equal those of the reference schedule bit for bit.
 // Timing uses hipEvents around the query phase (median of the timed repeats).
@@ -54,11 +54,11 @@ std::string schedule_json(const gprt_hip_config& c, int leaf, double ms) {
              "\"build_strategy\": \"PreferFastTrace\", \"geom_type\": \"Spheres\", "
              "\"gprt_hip\": {\"sampler\": %d, \"radius_heuristic\": %d, \"radius_param\": %.6g, "
              "\"radius_mult\": %.6g, \"max_rounds\": %d, \"traversal\": %d, \"stack_depth\": %d, "
-             "\"guard_slots\": %d, \"sort_queries\": %d, \"use_graph\": %d, \"grid_blocks\": %d, \"lds_stack\": %d, "
+             "\"guard_slots\": %d, \"sort_queries\": %d, \"use_graph\": %d, \"grid_blocks\": %d, \"lds_stack\": %d, \"compact_rounds\": %d, "
              "\"leaf_size\": %d, \"ms_query\": %.6g}}\n",
              c.radius_mult, c.sort_queries ? "true" : "false", heur, c.sampler, c.radius_heuristic, c.radius_param,
              c.radius_mult, c.max_rounds, c.traversal, c.stack_depth, c.guard_slots, c.sort_queries, c.use_graph,
-             c.grid_blocks, c.lds_stack, leaf, ms);
+             c.grid_blocks, c.lds_stack, c.compact_rounds, leaf, ms);
     return buf;
 }
 
@@ -129,7 +129,12 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
             cd.cfg.lds_stack = ring;
             rc = try_cand(cd);
         }
-        for (int trav : {GPRT_TRAV_THREAD, GPRT_TRAV_WAVE}) {
+        if (rc == GPRT_OK) {
+            Cand cd = best;
+            cd.cfg.compact_rounds = best.cfg.compact_rounds ? 0 : 1;
+            rc = try_cand(cd);
+        }
+        for (int trav : {GPRT_TRAV_THREAD, GPRT_TRAV_WAVE, GPRT_TRAV_WIDE}) {
             if (trav == best.cfg.traversal || rc != GPRT_OK) continue;
             Cand cd = best;
             cd.cfg.traversal = trav;

@@ -433,7 +433,7 @@ int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "sampler launch failed");
 }
 
-static int g_query_sort_bits = 24;
+static int g_query_sort_bits = 16;
 int query_sort_bits() { return g_query_sort_bits; }
 extern "C" GPRT_HIP_API void gprt_hip_debug_set_query_sort_bits(int b) { g_query_sort_bits = b < 8 ? 8 : (b > 30 ? 30 : b); }
 

@@ -45,6 +45,8 @@ struct RoundArgs {
     const int64_t* qoff;     // per cloud first global query index
     int max_steps;           // node-visit guard (> number of internal nodes)
     int base;                // first sorted position of this k class (identity work list)
+    int rounds;              // rounds of this schedule (the last is unbounded)
+    int inline_rounds;       // 1: an unresolved query retries with the next radius in the same launch
     int2* spill;             // stack entries below the LDS ring: [level][grid lanes] of (ref, d32)
     int spill_depth;         // levels of spill per lane
 };

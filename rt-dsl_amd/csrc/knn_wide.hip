@@ -7,6 +7,12 @@
 
 namespace gprt {
 
+// minimum wavefronts per SIMD the register allocation of the capacity-8
+// class must allow (1: no constraint)
+#ifndef GPRT_WIDE_MIN_WAVES
+#define GPRT_WIDE_MIN_WAVES 1
+#endif
+
 // ---------------------------------------------------------------------------
 // The per-lane round over the 4-wide view: one node visit tests four child
 // boxes (half the depth of the binary walk, half the dependent node loads).
@@ -25,7 +31,7 @@ __device__ __forceinline__ void cswap(float& da, int& ra, float& db, int& rb) {
 }
 
 template <int C, int G, int S, bool COUNT, bool ROUND0>
-__global__ __launch_bounds__(256) void knn_wide_kernel(RoundArgs a) {
+__global__ __launch_bounds__(256, C == 8 ? GPRT_WIDE_MIN_WAVES : 1) void knn_wide_kernel(RoundArgs a) {
     static_assert((S & (S - 1)) == 0, "LDS ring size must be a power of two");
     __shared__ int sref[S][256];
     __shared__ float sdist[S][256];
@@ -53,14 +59,20 @@ __global__ __launch_bounds__(256) void knn_wide_kernel(RoundArgs a) {
             const float4 q = a.qs[s];
             const int c = a.qcloud ? a.qcloud[s] : 0;
             const int k = a.kc ? a.kc[c] : a.k;
-            float R = a.last ? INFINITY : a.radius0[c];
-            if (!a.last)
-                for (int r = 0; r < a.round; ++r) R *= a.mult;
-            const float T0 = a.last ? INFINITY : R * R;
-            F32Query<C, G> Q;
-            Q.start(q.x, q.y, q.z, T0, k, true);
-            bool overflow = false;
+            // radius of this round; with inline rounds an unresolved query
+            // retries here with the next radius instead of joining a list
+            int round = a.round;
+            float R = a.radius0[c];
+            for (int r = 0; r < round; ++r) R *= a.mult;
             const int2 root = a.roots[c];
+            F32Query<C, G> Q;
+            bool overflow, resolved, last;
+            float T0;
+            while (true) {
+            last = a.last || (a.inline_rounds && round == a.rounds - 1);
+            T0 = last ? INFINITY : R * R;
+            Q.start(q.x, q.y, q.z, T0, k, true);
+            overflow = false;
             if (root.y > 0 && root.x < 0) {
                 scan_bucket(Q, pts, ~root.x, root.y);
                 if (COUNT) tests += root.y;
@@ -172,13 +184,20 @@ __global__ __launch_bounds__(256) void knn_wide_kernel(RoundArgs a) {
                     }
                     if (node < 0) break;
                 }
-                task_steps = steps;
+                task_steps += steps;
+            }
+            resolved = last || (Q.kth() < INFINITY && Q.kth() <= T0);
+            if (resolved || overflow || !a.inline_rounds) break;
+            ++round;
+            R *= a.mult;
+            atomicAdd(&a.counters[CNT_ACTIVE + round], 1u);  // statistics: queries entering this round
             }
             const float kth = Q.kth();
             const bool full = kth < INFINITY;
-            const bool resolved = a.last ? true : (full && kth <= T0);
             if (!resolved) {
-                status = 1;
+                // next round's list; with inline rounds only a walk that outgrew
+                // its stack is left unresolved, and it goes to the exact kernel
+                status = a.inline_rounds ? 2 : 1;
             } else {
                 const float tf = band(kth);
                 if (overflow || Q.ev_min <= tf) {

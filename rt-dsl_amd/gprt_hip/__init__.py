@@ -16,6 +16,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgprt_hip.so")
+# another build of the same library (for A/B measurements); the default is the in-tree build
+LIB_PATH = os.environ.get("GPRT_HIP_LIBRARY", LIB_PATH)
 
 OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
@@ -49,6 +51,7 @@ class Config(ctypes.Structure):
         ("collect_counters", ctypes.c_int),
         ("time_kernels", ctypes.c_int),
         ("lds_stack", ctypes.c_int),
+        ("compact_rounds", ctypes.c_int),
     ]
 
     def as_dict(self):

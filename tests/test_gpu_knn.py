@@ -73,6 +73,10 @@ def test_knn_leaf_sizes(ctx, leaf):
     dict(traversal=gprt_hip.TRAV_WIDE, lds_stack=8, stack_depth=10),  # wide walks overflow: exact fallback
     dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_INFINITE, lds_stack=16),
     dict(traversal=gprt_hip.TRAV_WAVE),
+    dict(compact_rounds=1),
+    dict(compact_rounds=1, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=15, radius_mult=2.0),
+    dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=15,
+         radius_mult=2.0),
 ])
 def test_knn_config_independence(ctx, cfgkw):
     """The result must not depend on the initial radius, multiplier, rounds,
