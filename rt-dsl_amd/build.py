@@ -9,6 +9,7 @@ import concurrent.futures as cf
 import os
 import subprocess
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -31,13 +32,14 @@ SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hi
 
 
 def compile_one(src, build_dir=BUILD, defines=()):
+    t0 = time.time()
     obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + HIPFLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{err}")
-    return obj, err
+    return obj, err + f"\n[{src}: {time.time() - t0:.0f} s]"
 
 
 def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=()):

@@ -55,7 +55,7 @@ enum {
     WS_QS4,        // float4[nq]
     WS_LISTA,      // int[nq]
     WS_LISTB,      // int[nq]
-    WS_FB,         // int[nq]
+    WS_FB,         // int2[nq]: fallback list
     WS_MISC,       // radius/kth/offsets
     WS_BBOXTMP,    // tree-build scratch
 };
@@ -419,19 +419,18 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     GPRT_TRY(c.ws[WS_QS4].reserve(16 * nq));
     GPRT_TRY(c.ws[WS_LISTA].reserve(4 * nq));
     GPRT_TRY(c.ws[WS_LISTB].reserve(4 * nq));
-    GPRT_TRY(c.ws[WS_FB].reserve(4 * nq));
+    GPRT_TRY(c.ws[WS_FB].reserve(8 * nq));
     GPRT_TRY(c.ws[WS_MISC].reserve(sizeof(float) * 101 * std::max(p.nclouds, 1) + 1024));
     CtxExtra* x = extra(&c);
     if (p.nclouds > 1) GPRT_TRY(x->qcloud.reserve(4 * nq));
     // HBM part of the per-lane traversal stacks of the persistent rounds
     if (p.cfg.traversal != GPRT_TRAV_WAVE) {
         size_t bytes = 0;
-        for (int cl = 0; cl + 1 < N_KCLASS; ++cl)
+        for (int cl = 0; cl < N_KCLASS; ++cl)
             if (p.class_size[cl] > 0) {
-                const int cap = k_class_cap(cl);
-                const size_t lanes = (size_t)256 * lane_round_grid(cap, p.cfg.lds_stack, p.cfg.grid_blocks,
+                const size_t lanes = (size_t)256 * lane_round_grid(cl, p.cfg.lds_stack, p.cfg.grid_blocks,
                                                                    p.cfg.traversal);
-                bytes = std::max(bytes, 8 * lanes * lane_spill_depth(p.cfg.stack_depth, cap, p.cfg.lds_stack));
+                bytes = std::max(bytes, 8 * lanes * lane_spill_depth(p.cfg.stack_depth, cl, p.cfg.lds_stack));
             }
         if (bytes) GPRT_TRY(x->spill.reserve(bytes));
     }
@@ -500,7 +499,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.k = p.k;
     ra.radius0 = radius;
     ra.mult = cfg.radius_mult;
-    ra.fb_list = c.ws[WS_FB].as<int>();
+    ra.fb_list = c.ws[WS_FB].as<int2>();
     ra.counters = counters;
     ra.stats64 = stats64;
     ra.ids_out = p.d_ids;
@@ -512,7 +511,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ExactArgs ea{};
     ea.nq = p.nq;
     ea.qs = qs4;
-    ea.fb_list = c.ws[WS_FB].as<int>();
+    ea.fb_list = c.ws[WS_FB].as<int2>();
     ea.qcloud = qcloud;
     ea.kc = p.d_kc;
     ea.k = p.k;
@@ -531,19 +530,10 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     for (int cl = 0; cl < N_KCLASS; ++cl) {
         const int size = p.class_size[cl];
         if (size == 0) continue;
-        if (cl == N_KCLASS - 1) {
-            // k too large for the FP32 candidate list: exact FP64 walk directly
-            ea.mode = 2;
-            ea.base = p.class_begin[cl];
-            ea.nq = size;
-            GPRT_TRY(launch_exact(s, ea, p.class_kmax[cl], size, cfg.collect_counters != 0));
-            continue;
-        }
         if (nclasses > 1)
             hipLaunchKernelGGL(class_init_kernel, dim3(1), dim3(64), 0, s, counters, (unsigned)size);
-        const int cap = k_class_cap(cl);
         ra.base = p.class_begin[cl];
-        ra.spill_depth = lane_spill_depth(cfg.stack_depth, cap, cfg.lds_stack);
+        ra.spill_depth = lane_spill_depth(cfg.stack_depth, cl, cfg.lds_stack);
         // later rounds inside the first launch (a lane retries its unresolved
         // query with the next radius) unless the schedule asks for one launch
         // per round over the compacted unresolved list; packets always compact
@@ -557,7 +547,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
             ra.out_list = lists[(r + 1) & 1];
             const bool tk = r == 0 && time_k && !timed;
             if (tk) GPRT_HIP_TRY(hipEventRecord(c.ev2, s));
-            GPRT_TRY(launch_round(s, ra, cap, grid, cfg.traversal, cfg.lds_stack, cfg.collect_counters != 0));
+            GPRT_TRY(launch_round(s, ra, cl, grid, cfg.traversal, cfg.lds_stack, cfg.collect_counters != 0));
             if (tk) {
                 GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
                 timed = true;

@@ -301,10 +301,12 @@ __device__ __forceinline__ float f64_lower_f32(double v) {
     return f;
 }
 
+// limit: a known upper bound on the k-th d64 (+inf when none); points and
+// boxes beyond it cannot hold a result and are skipped
 template <int KMAX>
 __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, const float4* __restrict__ pts,
                                             int2 root, float qx, float qy, float qz, int k, ExactList<KMAX>& L,
-                                            uint64_t& visits, uint64_t& tests, int max_steps) {
+                                            uint64_t& visits, uint64_t& tests, int max_steps, double limit) {
     int stk_ref[EXACT_STACK];
     float stk_d[EXACT_STACK];
     int sp = 0;
@@ -319,12 +321,12 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
                 if (t0 + u < m) {
                     const double dd = d2d(qx, qy, qz, p[u].x, p[u].y, p[u].z);
                     const uint32_t ii = (uint32_t)__float_as_int(p[u].w);
-                    if (dd < L.d[KMAX - 1] || (dd == L.d[KMAX - 1] && ii < L.id[KMAX - 1])) L.insert(dd, ii);
+                    if (dd <= limit && (dd < L.d[KMAX - 1] || (dd == L.d[KMAX - 1] && ii < L.id[KMAX - 1])))
+                        L.insert(dd, ii);
                 }
         }
         tests += m;
     };
-    const double& wd = L.d[KMAX - 1];
     if (root.y <= 0) return true;
     int ref = root.x;
     if (ref < 0) { scan(~ref, root.y); return true; }
@@ -333,6 +335,7 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
         if (++steps > max_steps) return false;
         const Node nd = nodes[ref];
         ++visits;
+        const double wd = fmin(L.d[KMAX - 1], limit);
         const double dA = box2d(qx, qy, qz, nd.alo, nd.ahi);
         const double dB = box2d(qx, qy, qz, nd.blo, nd.bhi);
         const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
@@ -345,7 +348,8 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
             if (lA && dA <= wd) scan(~rA, __float_as_int(nd.ahi.w));
             if (lB && dB <= wd) scan(~rB, __float_as_int(nd.bhi.w));
         }
-        const bool gA = !lA && dA <= wd, gB = !lB && dB <= wd;
+        const double wg = fmin(L.d[KMAX - 1], limit);  // the scans may have lowered it
+        const bool gA = !lA && dA <= wg, gB = !lB && dB <= wg;
         if (gA && gB) {
             const bool nearA = dA <= dB;
             if (sp >= EXACT_STACK) return false;
@@ -358,9 +362,10 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
         if (gA) { ref = rA; continue; }
         if (gB) { ref = rB; continue; }
         bool found = false;
+        const double wp = fmin(L.d[KMAX - 1], limit);
         while (sp > 0) {
             --sp;
-            if ((double)stk_d[sp] <= wd) { ref = stk_ref[sp]; found = true; break; }
+            if ((double)stk_d[sp] <= wp) { ref = stk_ref[sp]; found = true; break; }
         }
         if (!found) break;
     }
@@ -372,13 +377,15 @@ __device__ __forceinline__ bool exact_query(const Node* __restrict__ nodes, cons
 // mode 2: sorted positions base + i (a k class too large for the FP32 list)
 template <int KMAX, bool COUNT>
 __device__ __forceinline__ void exact_one(const ExactArgs& a, int i) {
+    double limit = INFINITY;
     float qx, qy, qz;
     int g, c = 0;
     if (a.mode == 0) {
         qx = a.q_xyz[3 * (int64_t)i]; qy = a.q_xyz[3 * (int64_t)i + 1]; qz = a.q_xyz[3 * (int64_t)i + 2];
         g = i;
     } else {
-        const int s = a.mode == 1 ? a.fb_list[i] : a.base + i;
+        if (a.mode == 1) limit = (double)__int_as_float(a.fb_list[i].y);
+        const int s = a.mode == 1 ? a.fb_list[i].x : a.base + i;
         const float4 q = a.qs[s];
         qx = q.x; qy = q.y; qz = q.z;
         g = __float_as_int(q.w);
@@ -388,7 +395,7 @@ __device__ __forceinline__ void exact_one(const ExactArgs& a, int i) {
     ExactList<KMAX> L;
     L.init(k);
     uint64_t visits = 0, tests = 0;
-    if (!exact_query<KMAX>(a.nodes, a.pts, a.roots[c], qx, qy, qz, k, L, visits, tests, a.max_steps))
+    if (!exact_query<KMAX>(a.nodes, a.pts, a.roots[c], qx, qy, qz, k, L, visits, tests, a.max_steps, limit))
         atomicAdd(&a.counters[CNT_ERROR], 1u);
     const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
 #pragma unroll
@@ -523,27 +530,18 @@ int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
     return GPRT_OK;
 }
 
-int capacity_class(int kmax, int guard) {
-    (void)guard;
-    int need = kmax + GUARD;
-    if (need <= 8) return 8;
-    if (need <= 16) return 16;
-    if (need <= 32) return 32;
-    return 64;
+int lane_spill_depth(int stack_depth, int cls, int ring) {
+    return std::max(0, stack_depth - effective_ring(cls, ring));
 }
 
-int lane_spill_depth(int stack_depth, int cap, int ring) {
-    return std::max(0, stack_depth - effective_ring(cap, ring));
+int lane_round_grid(int cls, int ring, int grid_req, int trav) {
+    return trav == GPRT_TRAV_WIDE ? wide_grid(cls, ring, grid_req) : lane_grid(cls, ring, grid_req);
 }
 
-int lane_round_grid(int cap, int ring, int grid_req, int trav) {
-    return trav == GPRT_TRAV_WIDE ? wide_grid(cap, ring, grid_req) : lane_grid(cap, ring, grid_req);
-}
-
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int ring, bool count) {
-    if (trav == GPRT_TRAV_WAVE) launch_packet_round(s, a, cap, grid, count);
-    else if (trav == GPRT_TRAV_WIDE) launch_wide_round(s, a, cap, ring, grid, count);
-    else launch_lane_round(s, a, cap, ring, grid, count);
+int launch_round(hipStream_t s, const RoundArgs& a, int cls, int grid, int trav, int ring, bool count) {
+    if (trav == GPRT_TRAV_WAVE) launch_packet_round(s, a, cls, grid, count);
+    else if (trav == GPRT_TRAV_WIDE) launch_wide_round(s, a, cls, ring, grid, count);
+    else launch_lane_round(s, a, cls, ring, grid, count);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "round launch failed");
 }
 

@@ -15,9 +15,9 @@ constexpr int GUARD = 3;
 // LDS ring sizes the per-lane kernels are built for
 __host__ __device__ inline int ring_size(int ring) { return ring <= 8 ? 8 : (ring <= 16 ? 16 : 32); }
 
-// LDS ring a class runs with: the capacity-8 class is built for 8, 16 and 32
-// entries; the larger classes, whose occupancy is bounded by registers, for 8
-__host__ __device__ inline int effective_ring(int cap, int ring) { return cap == 8 ? ring_size(ring) : 8; }
+// LDS ring a k class runs with: class 0 (capacity 8) is built for 8, 16 and
+// 32 entries; the larger classes, whose occupancy is bounded by registers, for 8
+__host__ __device__ inline int effective_ring(int cls, int ring) { return cls == 0 ? ring_size(ring) : 8; }
 
 // persistent grid of a round kernel: as many blocks of 256 as the device
 // holds at once, cached per device in the caller's slot array

@@ -36,7 +36,7 @@ struct RoundArgs {
     int last;                // 1: unbounded round, every query resolves
     const int* in_list;      // nullptr: identity over [0, count)
     int* out_list;
-    int* fb_list;
+    int2* fb_list;           // fallback: (sorted position, float bits of an upper bound on the k-th d64)
     unsigned* counters;
     unsigned long long* stats64;
     uint32_t* ids_out;
@@ -58,7 +58,7 @@ struct ExactArgs {
     int64_t nq;
     const float* q_xyz;
     const float4* qs;
-    const int* fb_list;
+    const int2* fb_list;
     const int* qcloud;
     const int* kc;
     int k;
@@ -87,26 +87,30 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
 // k classes: queries of one class share a candidate-list capacity.  Classes
 // 0..3 run the FP32 rounds with capacity 8/16/32/64 (k + 3 guard slots);
 // class 4 (k > 61) goes straight to the exact FP64 kernel.
+// k classes: queries of one class share a candidate list of capacity C with
+// G = 3 guard slots (k + G <= C): C = 8 for k <= 5, 16 for k <= 13, 32 for
+// k <= 29, 64 for k <= 61 and 67 for k <= 64.
 constexpr int N_KCLASS = 5;
 __host__ __device__ inline int k_class(int k) { return k <= 5 ? 0 : (k <= 13 ? 1 : (k <= 29 ? 2 : (k <= 61 ? 3 : 4))); }
-__host__ __device__ inline int k_class_cap(int cls) { return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 32 : 64)); }
+__host__ __device__ inline int k_class_cap(int cls) {
+    return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 32 : (cls == 3 ? 64 : 67)));
+}
 size_t query_prep_scratch_bytes(int nclouds, int use_morton);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
-int capacity_class(int kmax, int guard);
-// persistent grid (blocks) of the per-lane round for capacity cap and an LDS
+// persistent grid (blocks) of the per-lane round for k class cls and an LDS
 // ring of `ring` entries; grid_req > 0 is taken as given
-int lane_round_grid(int cap, int ring, int grid_req, int trav);
-// HBM spill levels per lane for a total stack depth, a capacity class and the
+int lane_round_grid(int cls, int ring, int grid_req, int trav);
+// HBM spill levels per lane for a total stack depth, a k class and the
 // configured LDS ring size
-int lane_spill_depth(int stack_depth, int cap, int ring);
+int lane_spill_depth(int stack_depth, int cls, int ring);
 // per-file round launchers and resident grids (knn_lane.hip, knn_wide.hip,
 // knn_packet.hip)
-int lane_grid(int cap, int ring, int grid_req);
-int wide_grid(int cap, int ring, int grid_req);
-void launch_lane_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count);
-void launch_wide_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count);
-void launch_packet_round(hipStream_t s, const RoundArgs& a, int cap, int grid, bool count);
-int launch_round(hipStream_t s, const RoundArgs& a, int cap, int grid, int trav, int stack, bool count);
+int lane_grid(int cls, int ring, int grid_req);
+int wide_grid(int cls, int ring, int grid_req);
+void launch_lane_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int grid_req, bool count);
+void launch_wide_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int grid_req, bool count);
+void launch_packet_round(hipStream_t s, const RoundArgs& a, int cls, int grid, bool count);
+int launch_round(hipStream_t s, const RoundArgs& a, int cls, int grid, int trav, int ring, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
 
 }  // namespace gprt

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
         const bool valid = i < n_in;
         int task_steps = 0;
         int status = 0;
+        float fbound = INFINITY;  // fallback: an upper bound on the k-th d64
         int s = 0;
         if (valid) {
             s = (!ROUND0 && a.in_list) ? a.in_list[i] : a.base + (int)i;
@@ -145,10 +146,12 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                 // next round's list; with inline rounds only a walk that outgrew
                 // its stack is left unresolved, and it goes to the exact kernel
                 status = a.inline_rounds ? 2 : 1;
+                fbound = full ? band(kth) : INFINITY;
             } else {
                 const float tf = band(kth);
                 if (overflow || Q.ev_min <= tf) {
                     status = 2;
+                    fbound = full ? band(kth) : INFINITY;
                 } else {
                     const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
                                                        k, pts);
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
         const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
         if (slot >= 0) a.out_list[slot] = s;
         const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
-        if (fslot >= 0) a.fb_list[fslot] = s;
+        if (fslot >= 0) a.fb_list[fslot] = make_int2(s, __float_as_int(fbound));
     }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -186,24 +189,25 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
 }
 
 
-template <int C, int S>
+template <int C, int G, int S>
 static int grid_cs(int grid_req) {
     if (grid_req > 0) return grid_req;
     static std::atomic<int> cache[64];
-    return resident_grid(knn_lane_kernel<C, GUARD, S, false, true>, cache);
+    return resident_grid(knn_lane_kernel<C, G, S, false, true>, cache);
 }
 
-int lane_grid(int cap, int ring, int grid_req) {
-    switch (cap) {
-        case 8:
+int lane_grid(int cls, int ring, int grid_req) {
+    switch (cls) {
+        case 0:
             switch (ring_size(ring)) {
-                case 8: return grid_cs<8, 8>(grid_req);
-                case 16: return grid_cs<8, 16>(grid_req);
-                default: return grid_cs<8, 32>(grid_req);
+                case 8: return grid_cs<8, GUARD, 8>(grid_req);
+                case 16: return grid_cs<8, GUARD, 16>(grid_req);
+                default: return grid_cs<8, GUARD, 32>(grid_req);
             }
-        case 16: return grid_cs<16, 8>(grid_req);
-        case 32: return grid_cs<32, 8>(grid_req);
-        default: return grid_cs<64, 8>(grid_req);
+        case 1: return grid_cs<16, GUARD, 8>(grid_req);
+        case 2: return grid_cs<32, GUARD, 8>(grid_req);
+        case 3: return grid_cs<64, GUARD, 8>(grid_req);
+        default: return grid_cs<67, GUARD, 8>(grid_req);
     }
 }
 
@@ -211,31 +215,32 @@ int lane_grid(int cap, int ring, int grid_req) {
 // so profiles report it apart from the short later rounds, which hold the few
 // queries round 0 left unresolved (often none) and run on a quarter of the
 // resident grid to keep their launches cheap.
-template <int C, int S>
+template <int C, int G, int S>
 static void launch_cs(hipStream_t s, const RoundArgs& a, int grid_req, bool count) {
-    const int full = grid_cs<C, S>(grid_req);
+    const int full = grid_cs<C, G, S>(grid_req);
     const int grid = (a.round == 0 || grid_req > 0) ? full : std::max(1, full / 4);
     if (a.round == 0) {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, true>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, true>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, G, S, true, true>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, G, S, false, true>), dim3(grid), dim3(256), 0, s, a);
     } else {
-        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, true, false>), dim3(grid), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((knn_lane_kernel<C, GUARD, S, false, false>), dim3(grid), dim3(256), 0, s, a);
+        if (count) hipLaunchKernelGGL((knn_lane_kernel<C, G, S, true, false>), dim3(grid), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((knn_lane_kernel<C, G, S, false, false>), dim3(grid), dim3(256), 0, s, a);
     }
 }
 
-void launch_lane_round(hipStream_t s, const RoundArgs& a, int cap, int ring, int grid_req, bool count) {
-    switch (cap) {
-        case 8:
+void launch_lane_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int grid_req, bool count) {
+    switch (cls) {
+        case 0:
             switch (ring_size(ring)) {
-                case 8: launch_cs<8, 8>(s, a, grid_req, count); break;
-                case 16: launch_cs<8, 16>(s, a, grid_req, count); break;
-                default: launch_cs<8, 32>(s, a, grid_req, count); break;
+                case 8: launch_cs<8, GUARD, 8>(s, a, grid_req, count); break;
+                case 16: launch_cs<8, GUARD, 16>(s, a, grid_req, count); break;
+                default: launch_cs<8, GUARD, 32>(s, a, grid_req, count); break;
             }
             break;
-        case 16: launch_cs<16, 8>(s, a, grid_req, count); break;
-        case 32: launch_cs<32, 8>(s, a, grid_req, count); break;
-        default: launch_cs<64, 8>(s, a, grid_req, count); break;
+        case 1: launch_cs<16, GUARD, 8>(s, a, grid_req, count); break;
+        case 2: launch_cs<32, GUARD, 8>(s, a, grid_req, count); break;
+        case 3: launch_cs<64, GUARD, 8>(s, a, grid_req, count); break;
+        default: launch_cs<67, GUARD, 8>(s, a, grid_req, count); break;
     }
 }
 

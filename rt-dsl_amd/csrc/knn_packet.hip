@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
         }
         // resolve
         int status = 0;  // 0 done, 1 unresolved, 2 fallback
+        float fbound = INFINITY;  // fallback: an upper bound on the k-th d64
         if (valid) {
             const float kth = Q.kth();
             const bool full = kth < INFINITY;
@@ -134,6 +135,7 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
                 const float tf = band(kth);  // INF when the cloud holds fewer than k points
                 if (overflow || Q.ev_min <= tf) {
                     status = 2;
+                    fbound = full ? band(kth) : INFINITY;
                 } else {
                     const int rr = rerank_and_write<C>(a, Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, __float_as_int(q.w), c,
                                                        k, pts);
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
         const int slot = wave_append(&a.counters[CNT_ACTIVE + a.round + 1], status == 1);
         if (slot >= 0) a.out_list[slot] = s;
         const int fslot = wave_append(&a.counters[CNT_FALLBACK], status == 2);
-        if (fslot >= 0) a.fb_list[fslot] = s;
+        if (fslot >= 0) a.fb_list[fslot] = make_int2(s, __float_as_int(fbound));
     }
     if (COUNT) {
         // visits/tests are wave-uniform in the packet walk: count once per wave
@@ -159,16 +161,17 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
 }
 
 
-void launch_packet_round(hipStream_t s, const RoundArgs& a, int cap, int grid, bool count) {
+void launch_packet_round(hipStream_t s, const RoundArgs& a, int cls, int grid, bool count) {
     if (grid <= 0) grid = 1024;
-#define GPRT_LP(CAP)                                                                                  \
-    if (count) hipLaunchKernelGGL((knn_packet_kernel<CAP, GUARD, true>), dim3(grid), dim3(256), 0, s, a); \
-    else hipLaunchKernelGGL((knn_packet_kernel<CAP, GUARD, false>), dim3(grid), dim3(256), 0, s, a);
-    switch (cap) {
-        case 8: GPRT_LP(8) break;
-        case 16: GPRT_LP(16) break;
-        case 32: GPRT_LP(32) break;
-        default: GPRT_LP(64) break;
+#define GPRT_LP(CAP, G)                                                                              \
+    if (count) hipLaunchKernelGGL((knn_packet_kernel<CAP, G, true>), dim3(grid), dim3(256), 0, s, a); \
+    else hipLaunchKernelGGL((knn_packet_kernel<CAP, G, false>), dim3(grid), dim3(256), 0, s, a);
+    switch (cls) {
+        case 0: GPRT_LP(8, GUARD) break;
+        case 1: GPRT_LP(16, GUARD) break;
+        case 2: GPRT_LP(32, GUARD) break;
+        case 3: GPRT_LP(64, GUARD) break;
+        default: GPRT_LP(67, GUARD) break;
     }
 #undef GPRT_LP
 }
