@@ -88,12 +88,17 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
 // 0..3 run the FP32 rounds with capacity 8/16/32/64 (k + 3 guard slots);
 // class 4 (k > 61) goes straight to the exact FP64 kernel.
 // k classes: queries of one class share a candidate list of capacity C with
-// G = 3 guard slots (k + G <= C): C = 8 for k <= 5, 16 for k <= 13, 32 for
-// k <= 29, 64 for k <= 61 and 67 for k <= 64.
-constexpr int N_KCLASS = 5;
-__host__ __device__ inline int k_class(int k) { return k <= 5 ? 0 : (k <= 13 ? 1 : (k <= 29 ? 2 : (k <= 61 ? 3 : 4))); }
+// G = 3 guard slots, the smallest C of {8, 16, 24, 32, 48, 64, 67} with
+// k + G <= C (registers hold the list, so a tighter class runs at a higher
+// occupancy).
+constexpr int N_KCLASS = 7;
 __host__ __device__ inline int k_class_cap(int cls) {
-    return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 32 : (cls == 3 ? 64 : 67)));
+    return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 24 : (cls == 3 ? 32 : (cls == 4 ? 48 : (cls == 5 ? 64 : 67)))));
+}
+__host__ __device__ inline int k_class(int k) {
+    int c = 0;
+    while (c + 1 < N_KCLASS && k + 3 > k_class_cap(c)) ++c;
+    return c;
 }
 size_t query_prep_scratch_bytes(int nclouds, int use_morton);
 int launch_fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
