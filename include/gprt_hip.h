@@ -250,6 +250,19 @@ GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t 
                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                    const char* json_path, int ptr_flags);
 
+/* ---------------- device programs from the shader IR ---------------- */
+/* Lower a program of the reference's shader IR (ShaderNode trees,
+ * gprt-ir/src/lib.rs:8-18, serialised as JSON; the forms are listed at the top
+ * of codegen.hip) to a HIP translation unit, as gprt-codegen's compile_program
+ * does to CUDA (gprt-codegen/src/lib.rs:78-182).  Two-phase: with src_out NULL
+ * only *src_size (bytes incl. the terminating NUL) is set.  Host only. */
+GPRT_HIP_API int gprt_hip_codegen(const char* ir_json, size_t ir_size, char* src_out, size_t src_capacity,
+                                  size_t* src_size);
+/* Lower and compile with hiprtc for `arch` (NULL: gfx950), replacing the nvcc
+ * step of gprt-codegen/src/lib.rs:184-203; *code_size receives the size of the
+ * code object.  Host only (no GPU needed); errors carry the compiler log. */
+GPRT_HIP_API int gprt_hip_codegen_compile(const char* ir_json, size_t ir_size, const char* arch, size_t* code_size);
+
 /* ---------------- the reference FFI (gprt-optix/src/lib.rs:13-31) ---------------- */
 typedef struct CGprtPipeline CGprtPipeline;
 typedef struct CGprtBvh CGprtBvh;
@@ -257,7 +270,10 @@ typedef struct CGprtBvh CGprtBvh;
 /* main.cpp:101-180.  ir_data selects the device program: NULL/empty or a
  * descriptor "gprt-hip:knn-heap" (register-heap k-NN, default),
  * "gprt-hip:push" (per-query push of every hit), "gprt-hip:autorope" (the
- * Barnes-Hut walk).  Any other bytes (an OptiX-IR blob) select the default. */
+ * Barnes-Hut walk), or "gprt-hip:ir:" followed by a shader-IR program in JSON,
+ * compiled here for the GPU (see gprt_hip_codegen; NULL on a codegen or
+ * compile error, with the message in gprt_hip_last_error).  Any other bytes
+ * (an OptiX-IR blob) select the default. */
 GPRT_HIP_API CGprtPipeline* gprt_pipeline_create(const void* ir_data, size_t ir_size, int is_hw_triangle);
 GPRT_HIP_API void gprt_pipeline_destroy(CGprtPipeline* pipe);
 /* main.cpp:200-248: h_geom holds float4 spheres (x, y, z, r); the LBVH is

@@ -28,7 +28,7 @@ HIPFLAGS = [
 ]
 
 SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hip", "api.hip", "range.hip",
-           "bh.hip", "legacy.hip", "autotune.hip"]
+           "bh.hip", "legacy.hip", "autotune.hip", "codegen.hip"]
 
 
 def compile_one(src, build_dir=BUILD, defines=()):
@@ -55,7 +55,7 @@ def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=()):
         if err.strip() and verbose:
             print(err, file=sys.stderr)
     tmp = out + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results] + ["-lhiprtc"]
     subprocess.check_call(cmd)
     os.replace(tmp, out)
     if verbose:

@@ -17,11 +17,16 @@
 //                        count, like the atomicAdd counter.
 //   "gprt-hip:autorope"  the Barnes-Hut walk of barnes_hut! (:760-806) over the
 //                        caller's flattened octree.
+//   "gprt-hip:ir:" + JSON a program of the reference's shader IR, lowered to HIP
+//                        and compiled at pipeline creation (codegen.hip); it runs
+//                        one lane per query over the LBVH with the arrays
+//                        registered on the pipeline, in registration order.
 // Every distance uses the binary64 contract d64 of the rest of the library, and
 // r^2 = (double)r * (double)r.
 #include <cstring>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "gprt_internal.hpp"
 #include "knn_internal.hpp"
@@ -34,7 +39,30 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
 
 using namespace gprt;
 
-enum { PROG_KNN_HEAP = 0, PROG_PUSH = 1, PROG_AUTOROPE = 2 };
+enum { PROG_KNN_HEAP = 0, PROG_PUSH = 1, PROG_AUTOROPE = 2, PROG_GENERATED = 3 };
+
+namespace gprt {
+int codegen_program(const char* ir, size_t n, std::string& src);
+int codegen_compile(const std::string& src, const char* arch, std::vector<char>& code);
+}
+
+// host mirror of the generated kernel's launch block (codegen.hip)
+struct GenBundle {
+    unsigned long long dyn_ptrs[8];
+    unsigned int dyn_caps[8];
+    unsigned long long dyn_lens[8];
+    unsigned long long val_ptrs[8];
+};
+struct GenLaunch {
+    const void* nodes;
+    const void* pts;
+    int root_ref, root_count;
+    const void* rays;
+    int num_rays;
+    void* bundle;
+    int max_steps;
+    unsigned* err;
+};
 
 struct LegacyArray {
     DevBuf data, lens;
@@ -47,6 +75,10 @@ struct CGprtPipeline {
     int program = PROG_KNN_HEAP;
     bool is_hw_triangle = false;
     std::map<std::string, LegacyArray> arrays;
+    std::vector<std::string> order;  // registration order (array indices of generated programs)
+    hipModule_t module = nullptr;    // generated program
+    hipFunction_t kernel = nullptr;
+    DevBuf bundle;
     std::map<std::string, DevBuf> values;
     DevBuf q4, xyz, radii, off, tmp_ids, counts, d2, big, keys, perm, qs4, sums;
     DevBuf bodies, spatial, routing, forces;
@@ -218,13 +250,32 @@ __global__ void autorope_kernel(const LBody* __restrict__ bodies, int nb, const 
 
 }  // namespace gprt
 
+static const char kIrPrefix[] = "gprt-hip:ir:";
+
 static int parse_program(const void* ir, size_t n) {
     if (!ir || n == 0) return PROG_KNN_HEAP;
     std::string s((const char*)ir, n);
     while (!s.empty() && (s.back() == '\0' || s.back() == '\n')) s.pop_back();
     if (s == "gprt-hip:push") return PROG_PUSH;
     if (s == "gprt-hip:autorope") return PROG_AUTOROPE;
+    if (s.compare(0, sizeof(kIrPrefix) - 1, kIrPrefix) == 0) return PROG_GENERATED;
     return PROG_KNN_HEAP;
+}
+
+// lower, compile for the context's GPU and load a generated program
+static int load_generated(CGprtPipeline* p, const char* ir, size_t n) {
+    std::string src;
+    const size_t skip = sizeof(kIrPrefix) - 1;
+    GPRT_TRY(codegen_program(ir + skip, n - skip, src));
+    hipDeviceProp_t prop;
+    GPRT_HIP_TRY(hipGetDeviceProperties(&prop, p->ctx->c.device));
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    std::vector<char> code;
+    GPRT_TRY(codegen_compile(src, arch.c_str(), code));
+    GPRT_HIP_TRY(hipModuleLoadData(&p->module, code.data()));
+    GPRT_HIP_TRY(hipModuleGetFunction(&p->kernel, p->module, "gprt_generated"));
+    return GPRT_OK;
 }
 
 extern "C" CGprtPipeline* gprt_pipeline_create(const void* ir_data, size_t ir_size, int is_hw_triangle) {
@@ -234,6 +285,16 @@ extern "C" CGprtPipeline* gprt_pipeline_create(const void* ir_data, size_t ir_si
     p->ctx = ctx;
     p->program = parse_program(ir_data, ir_size);
     p->is_hw_triangle = is_hw_triangle != 0;
+    if (p->program == PROG_GENERATED) {
+        size_t n = ir_size;
+        while (n && ((const char*)ir_data)[n - 1] == '\0') --n;
+        if (load_generated(p, (const char*)ir_data, n) != GPRT_OK) {
+            const std::string msg = gprt_hip_last_error();
+            gprt_pipeline_destroy(p);
+            fail(GPRT_ERR_ARG, msg);
+            return nullptr;
+        }
+    }
     return p;
 }
 
@@ -245,8 +306,9 @@ extern "C" void gprt_pipeline_destroy(CGprtPipeline* p) {
     }
     for (auto& kv : p->values) kv.second.release();
     for (DevBuf* b : {&p->q4, &p->xyz, &p->radii, &p->off, &p->tmp_ids, &p->counts, &p->d2, &p->big, &p->keys,
-                      &p->perm, &p->qs4, &p->sums, &p->bodies, &p->spatial, &p->routing, &p->forces})
+                      &p->perm, &p->qs4, &p->sums, &p->bodies, &p->spatial, &p->routing, &p->forces, &p->bundle})
         b->release();
+    if (p->module) (void)hipModuleUnload(p->module);
     gprt_hip_ctx_destroy(p->ctx);
     delete p;
 }
@@ -313,6 +375,7 @@ extern "C" void gprt_register_array(CGprtPipeline* p, const char* name, unsigned
         return;
     }
     (void)hipSetDevice(p->ctx->c.device);
+    if (!p->arrays.count(name)) p->order.push_back(name);
     LegacyArray& a = p->arrays[name];
     a.cap = cap;
     a.nq = nq;
@@ -351,9 +414,41 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
     int2 root;
     (void)hipMemcpyAsync(&root, T.roots.ptr, sizeof(int2), hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
+    if (p->program == PROG_GENERATED) {
+        if (p->order.size() > 8) {
+            fail(GPRT_ERR_ARG, "gprt_execute: a generated program binds at most 8 arrays");
+            return;
+        }
+        GenBundle hb;
+        memset(&hb, 0, sizeof(hb));
+        for (size_t j = 0; j < p->order.size(); ++j) {
+            LegacyArray& a = p->arrays[p->order[j]];
+            if (a.nq < count) {
+                fail(GPRT_ERR_ARG, "gprt_execute: array " + p->order[j] + " holds fewer queries than the launch");
+                return;
+            }
+            hb.dyn_ptrs[j] = (unsigned long long)(uintptr_t)a.data.ptr;
+            hb.dyn_caps[j] = a.cap;
+            hb.dyn_lens[j] = (unsigned long long)(uintptr_t)a.lens.ptr;
+            (void)launch_fill_u32(s, a.lens.as<unsigned>(), count, 0u);
+        }
+        if (p->bundle.reserve(sizeof(GenBundle)) != GPRT_OK) return;
+        if (hipMemcpyAsync(p->bundle.ptr, &hb, sizeof(hb), hipMemcpyHostToDevice, s) != hipSuccess) {
+            fail(GPRT_ERR_HIP, "gprt_execute: bundle upload failed");
+            return;
+        }
+        GenLaunch L{T.nodes.ptr, T.pts4.ptr, root.x, root.y, p->q4.ptr, count, p->bundle.ptr, T.n + 64,
+                    counters + CNT_ERROR};
+        void* args[] = {&L};
+        if (count && hipModuleLaunchKernel(p->kernel, (count + 255) / 256, 1, 1, 256, 1, 1, 0, s, args, nullptr) !=
+                         hipSuccess) {
+            fail(GPRT_ERR_HIP, "gprt_execute: generated program launch failed");
+            return;
+        }
+    }
     for (auto& kv : p->arrays) {
         LegacyArray& a = kv.second;
-        if (a.nq < count || count == 0) continue;
+        if (p->program == PROG_GENERATED || a.nq < count || count == 0) continue;
         if (p->program == PROG_KNN_HEAP) {
             const int k = (int)(a.cap / 2);
             if (k < 1 || k > 64) {

@@ -123,6 +123,10 @@ SIGNATURES = [
     ("gprt_hip_sort_pairs", _int, [_vp, _vp, _vp, _i64, _int, _int, _int]),
     ("gprt_hip_autotune", _int, [_vp, _vp, _i64, _vp, _i64, _int, _int, ctypes.POINTER(Config),
                                  ctypes.POINTER(_int), ctypes.c_char_p, _int]),
+    ("gprt_hip_codegen", _int, [ctypes.c_char_p, ctypes.c_size_t, _vp, ctypes.c_size_t,
+                                ctypes.POINTER(ctypes.c_size_t)]),
+    ("gprt_hip_codegen_compile", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_size_t)]),
     # the reference FFI (gprt-optix/src/lib.rs:13-31)
     ("gprt_pipeline_create", _vp, [_vp, ctypes.c_size_t, _int]),
     ("gprt_pipeline_destroy", None, [_vp]),
@@ -454,6 +458,28 @@ class Tree:
         _check(lib().gprt_hip_range(self.ctx.h, self.h, q, nq, radius, off.ctypes.data, ids.ctypes.data,
                                     len(ids), PTR_HOST))
         return off, ids[: int(off[-1])]
+
+
+def codegen(program):
+    """HIP source of a shader-IR program (a dict or JSON text); host only."""
+    import json as _json
+    text = program if isinstance(program, str) else _json.dumps(program)
+    raw = text.encode()
+    n = ctypes.c_size_t()
+    _check(lib().gprt_hip_codegen(raw, len(raw), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    _check(lib().gprt_hip_codegen(raw, len(raw), buf, n.value, ctypes.byref(n)))
+    return buf.value.decode()
+
+
+def codegen_compile(program, arch="gfx950"):
+    """Lower and compile a shader-IR program with hiprtc; returns the code-object size."""
+    import json as _json
+    text = program if isinstance(program, str) else _json.dumps(program)
+    raw = text.encode()
+    n = ctypes.c_size_t()
+    _check(lib().gprt_hip_codegen_compile(raw, len(raw), arch.encode(), ctypes.byref(n)))
+    return n.value
 
 
 def num_devices():
