@@ -250,6 +250,30 @@ GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t 
                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                    const char* json_path, int ptr_flags);
 
+/* ---------------- sandbox inputs ---------------- */
+#define GPRT_CSV_POINTS 0 /* "x,y,z[,...]" per line (gprt-sandbox/src/io.rs:8-30) */
+#define GPRT_CSV_BODIES 1 /* "x,y,z,m[,...]" per line (gprt-sandbox/src/bin/barneshut.rs:8-34) */
+
+/* Read a text point file as the reference's sandbox does.  POINTS: a line
+ * whose first three ','-separated fields (trimmed) are not all numbers, or
+ * that is not UTF-8, is skipped.  BODIES: blank lines and lines with fewer
+ * than four fields are skipped, a field that is not a number reads as 0, a
+ * body with mass <= 0 is skipped, and a line that is not UTF-8 fails with
+ * GPRT_ERR_INPUT.  Rows keep the file's order (row i is id i); numbers are
+ * rounded once to binary32.  Writes 3 (POINTS) or 4 (BODIES) floats per row
+ * to out and the row count to *rows; when the file holds more than capacity
+ * rows, writes nothing, sets *rows and returns GPRT_ERR_ARG (capacity 0
+ * counts).  Replaces load_points_fast (io.rs:8) and load_galaxy
+ * (barneshut.rs:8).  Host only; parses on all host cores. */
+GPRT_HIP_API int gprt_hip_read_points(const char* path, int format, float* out, int64_t capacity, int64_t* rows);
+
+/* Read a Schedule / wisdom JSON file's text (gprt-ir/src/lib.rs:58-89, the
+ * *_wisdom.json files) into a config.  Missing fields take the reference's
+ * Schedule::default() (lib.rs:76-87), so files without build_strategy /
+ * geom_type load; a "gprt_hip" object (as gprt_hip_autotune writes) sets the
+ * config fields it names and *leaf_size (default 8).  Host only. */
+GPRT_HIP_API int gprt_hip_schedule_from_json(const char* json, size_t size, gprt_hip_config* cfg, int* leaf_size);
+
 /* ---------------- device programs from the shader IR ---------------- */
 /* Lower a program of the reference's shader IR (ShaderNode trees,
  * gprt-ir/src/lib.rs:8-18, serialised as JSON; the forms are listed at the top

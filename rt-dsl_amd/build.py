@@ -28,7 +28,7 @@ HIPFLAGS = [
 ]
 
 SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hip", "api.hip", "range.hip",
-           "bh.hip", "legacy.hip", "autotune.hip", "codegen.hip"]
+           "bh.hip", "legacy.hip", "autotune.hip", "codegen.hip", "io.hip"]
 
 
 def compile_one(src, build_dir=BUILD, defines=()):

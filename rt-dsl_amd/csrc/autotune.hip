@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gprt_internal.hpp"
+#include "json.hpp"
 
 using namespace gprt;
 
@@ -173,5 +174,76 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
         fputs(js.c_str(), f);
         fclose(f);
     }
+    return GPRT_OK;
+}
+
+// Schedule / wisdom JSON (gprt-ir/src/lib.rs:58-89) -> config.  Fields missing
+// from the file take Schedule::default() (lib.rs:76-87): the reference's own
+// *_wisdom.json files lack build_strategy and geom_type and therefore fail its
+// serde reader; here they load.  max_hits_per_query, memory_strategy,
+// build_strategy and geom_type have no counterpart (candidates live in
+// registers, the tree is the LBVH) and are ignored.  An object "gprt_hip"
+// (written by gprt_hip_autotune) sets the config fields it names and the leaf
+// size.
+extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_config* cfg, int* leaf_size) {
+    if (!json || !cfg) return fail(GPRT_ERR_ARG, "null argument");
+    JParser jp(json, n);
+    JVal v;
+    if (!jp.value(v)) return fail(GPRT_ERR_ARG, "schedule: " + jp.err);
+    if (v.kind != JVal::Obj) return fail(GPRT_ERR_ARG, "schedule: expected a JSON object");
+    gprt_hip_config c;
+    gprt_hip_config_default(&c);
+    // Schedule::default()
+    c.radius_heuristic = GPRT_RADIUS_PERCENTILE;
+    c.radius_param = 0.10f;
+    c.radius_mult = 3.0f;
+    c.sort_queries = 0;
+    int leaf = 8;
+    auto num = [](const JVal* x, double& out) {
+        if (!x || x->kind != JVal::Num) return false;
+        out = x->num;
+        return true;
+    };
+    double d = 0.0;
+    if (num(v.get("radius_increment_mult"), d)) c.radius_mult = (float)d;
+    if (const JVal* m = v.get("use_morton_lbv")) {
+        if (m->kind != JVal::Bool) return fail(GPRT_ERR_ARG, "schedule: use_morton_lbv must be a boolean");
+        c.sort_queries = m->b ? 1 : 0;
+    }
+    if (const JVal* h = v.get("radius_heuristic")) {
+        if (h->kind == JVal::Str && h->str == "SampledMax") {
+            c.radius_heuristic = GPRT_RADIUS_MAX;
+        } else if (h->kind == JVal::Obj && num(h->get("SampledPercentile"), d)) {
+            c.radius_heuristic = GPRT_RADIUS_PERCENTILE;
+            c.radius_param = (float)d;
+        } else if (h->kind == JVal::Obj && num(h->get("Fixed"), d)) {
+            c.radius_heuristic = GPRT_RADIUS_FIXED;
+            c.radius_param = (float)d;
+        } else {
+            return fail(GPRT_ERR_ARG, "schedule: radius_heuristic must be \"SampledMax\", {\"SampledPercentile\": p} "
+                                      "or {\"Fixed\": r}");
+        }
+    }
+    if (const JVal* g = v.get("gprt_hip")) {
+        if (g->kind != JVal::Obj) return fail(GPRT_ERR_ARG, "schedule: gprt_hip must be an object");
+        struct {
+            const char* key;
+            int* ip;
+            float* fp;
+        } fields[] = {{"sampler", &c.sampler, nullptr},           {"radius_heuristic", &c.radius_heuristic, nullptr},
+                      {"radius_param", nullptr, &c.radius_param}, {"radius_mult", nullptr, &c.radius_mult},
+                      {"max_rounds", &c.max_rounds, nullptr},     {"traversal", &c.traversal, nullptr},
+                      {"stack_depth", &c.stack_depth, nullptr},   {"guard_slots", &c.guard_slots, nullptr},
+                      {"sort_queries", &c.sort_queries, nullptr}, {"use_graph", &c.use_graph, nullptr},
+                      {"grid_blocks", &c.grid_blocks, nullptr},   {"lds_stack", &c.lds_stack, nullptr},
+                      {"compact_rounds", &c.compact_rounds, nullptr}, {"leaf_size", &leaf, nullptr}};
+        for (auto& f : fields)
+            if (num(g->get(f.key), d)) {
+                if (f.ip) *f.ip = (int)d;
+                else *f.fp = (float)d;
+            }
+    }
+    *cfg = c;
+    if (leaf_size) *leaf_size = leaf;
     return GPRT_OK;
 }

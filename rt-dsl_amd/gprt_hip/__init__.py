@@ -123,6 +123,10 @@ SIGNATURES = [
     ("gprt_hip_sort_pairs", _int, [_vp, _vp, _vp, _i64, _int, _int, _int]),
     ("gprt_hip_autotune", _int, [_vp, _vp, _i64, _vp, _i64, _int, _int, ctypes.POINTER(Config),
                                  ctypes.POINTER(_int), ctypes.c_char_p, _int]),
+    ("gprt_hip_read_points", _int, [ctypes.c_char_p, _int, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.POINTER(ctypes.c_int64)]),
+    ("gprt_hip_schedule_from_json", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Config),
+                                           ctypes.POINTER(_int)]),
     ("gprt_hip_codegen", _int, [ctypes.c_char_p, ctypes.c_size_t, _vp, ctypes.c_size_t,
                                 ctypes.POINTER(ctypes.c_size_t)]),
     ("gprt_hip_codegen_compile", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
@@ -458,6 +462,37 @@ class Tree:
         _check(lib().gprt_hip_range(self.ctx.h, self.h, q, nq, radius, off.ctypes.data, ids.ctypes.data,
                                     len(ids), PTR_HOST))
         return off, ids[: int(off[-1])]
+
+
+CSV_POINTS = 0
+CSV_BODIES = 1
+
+
+def read_points(path, fmt=CSV_POINTS):
+    """(n, 3) float32 points (CSV_POINTS) or (n, 4) x, y, z, mass bodies
+    (CSV_BODIES) from a text file, read as the reference's sandbox reads it."""
+    width = 4 if fmt == CSV_BODIES else 3
+    # a row takes at least 6 (points: "a,b,c\n") or 5 (bodies: ",,,m\n") bytes
+    cap = (os.path.getsize(path) + 1) // (6 if fmt == CSV_POINTS else 5) + 1
+    out = np.empty((cap, width), np.float32)
+    rows = ctypes.c_int64()
+    _check(lib().gprt_hip_read_points(os.fsencode(path), fmt, out.ctypes.data, cap, ctypes.byref(rows)))
+    return out[:rows.value].copy()
+
+
+def parse_schedule(text):
+    """(Config, leaf_size) from the text of a Schedule / wisdom JSON object."""
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    cfg = Config()
+    leaf = _int()
+    _check(lib().gprt_hip_schedule_from_json(raw, len(raw), ctypes.byref(cfg), ctypes.byref(leaf)))
+    return cfg, leaf.value
+
+
+def load_schedule(path):
+    """(Config, leaf_size) from a Schedule / wisdom JSON file."""
+    with open(path, "rb") as f:
+        return parse_schedule(f.read())
 
 
 def codegen(program):

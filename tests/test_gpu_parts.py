@@ -359,3 +359,8 @@ def test_autotune_writes_schedule(ctx, tmp_path):
     assert "radius_increment_mult" in js and "radius_heuristic" in js and js["gprt_hip"]["leaf_size"] == leaf
     t = ctx.build(pts, leaf_size=leaf)
     assert np.array_equal(t.k_nn(q, 5, cfg), oracle.knn_brute(pts, q, 5))
+    # the written file reads back to the same schedule
+    cfg2, leaf2 = gprt_hip.load_schedule(path)
+    assert leaf2 == leaf
+    for f, _ in gprt_hip.Config._fields_:
+        assert getattr(cfg2, f) == pytest.approx(getattr(cfg, f), rel=1e-5), f

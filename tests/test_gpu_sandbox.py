@@ -1,0 +1,109 @@
+"""Sandbox harnesses end to end on the GPU (gprt-sandbox/src/main.rs,
+src/bin/barneshut.rs): text files in, the reference's output lines and
+knn_results.txt out, ids checked against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import gprt_hip
+import oracle
+from gprt_hip import barneshut, sandbox
+
+pytestmark = pytest.mark.gpu
+
+
+def _csv(path, a):
+    np.savetxt(path, a, delimiter=",", fmt="%.9g")
+    return str(path)
+
+
+def _read_dump(path):
+    rows = []
+    for line in open(path):
+        head, body = line.rstrip("\n").split(":", 1)
+        assert head == f"Q{len(rows)}"
+        body = body.strip("[]")
+        rows.append([int(x) for x in body.split(", ")] if body else [])
+    return rows
+
+
+def test_sandbox_knn_dump(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    rng = np.random.default_rng(21)
+    pts = rng.normal(size=(40_000, 3)).astype(np.float32)
+    q = rng.normal(size=(3_000, 3)).astype(np.float32)
+    dp, qp = _csv(tmp_path / "data.csv", pts), _csv(tmp_path / "queries.csv", q)
+    lines = []
+    sandbox.run(["gprt-sandbox", dp, qp, "7", "0.3", "2.0", "dump"], lines.append)
+    assert lines[0].startswith("[IO] Loaded 40000 points and 3000 queries in ")
+    assert lines[1] == "[LOAD] Using CLI defaults: P=0.30, M=2.0"
+    assert any(l.startswith("[GPRT_LOOP] Initial Radius Calculated: ") for l in lines)
+    stats = [l for l in lines if l.startswith("[GPRT_STATS] search_ms=")]
+    assert len(stats) == 1 and stats[0].endswith("%")
+    last = lines[-1].split(",")
+    assert last[0] == dp and last[1:3] == ["3000", "7"] and float(last[3]) > 0 and last[4] == str(3000 * 7)
+    # the text round trip of the inputs is exact (%.9g), so the oracle sees the same points
+    pts_r, q_r = gprt_hip.read_points(dp), gprt_hip.read_points(qp)
+    assert np.array_equal(pts_r, pts) and np.array_equal(q_r, q)
+    ref = oracle.knn_brute(pts, q, 7)
+    assert _read_dump(tmp_path / "knn_results.txt") == ref.tolist()
+
+
+def test_sandbox_small_cloud_and_wisdom(tmp_path, monkeypatch):
+    """Fewer points than k: short lists; a reference-form wisdom file is loaded."""
+    monkeypatch.chdir(tmp_path)
+    rng = np.random.default_rng(22)
+    pts = rng.random((3, 3), dtype=np.float32)
+    q = rng.random((50, 3), dtype=np.float32)
+    dp, qp = _csv(tmp_path / "d.csv", pts), _csv(tmp_path / "q.csv", q)
+    (tmp_path / "gprt_wisdom.json").write_text('{"radius_increment_mult": 2.0, "max_hits_per_query": 2000, '
+                                               '"use_morton_lbv": true, "radius_heuristic": '
+                                               '{"SampledPercentile": 0.2}, "memory_strategy": '
+                                               '"PayloadRegisterHeap"}')
+    lines = []
+    sandbox.run(["s", dp, qp, "5", "dump"], lines.append)
+    assert lines[1] == "[LOAD] Loaded auto-tuned schedule from gprt_wisdom.json"
+    assert lines[-1].endswith(f",50,5,{lines[-1].split(',')[3]},150")
+    ref = oracle.knn_brute(pts, q, 5)
+    assert _read_dump(tmp_path / "knn_results.txt") == [r[:3] for r in ref.tolist()]
+
+
+def test_sandbox_tune_then_load(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    rng = np.random.default_rng(23)
+    pts = rng.random((30_000, 3), dtype=np.float32)
+    q = rng.random((5_000, 3), dtype=np.float32)
+    dp, qp = _csv(tmp_path / "d.csv", pts), _csv(tmp_path / "q.csv", q)
+    lines = []
+    ids = sandbox.run(["s", dp, qp, "5", "tune"], lines.append)
+    assert "[TUNE] Saved optimal schedule to gprt_wisdom.json" in lines
+    assert (tmp_path / "gprt_wisdom.json").exists()
+    ref = oracle.knn_brute(pts, q, 5)
+    assert np.array_equal(ids, ref)
+    lines = []
+    ids = sandbox.run(["s", dp, qp, "5"], lines.append)
+    assert lines[1] == "[LOAD] Loaded auto-tuned schedule from gprt_wisdom.json"
+    assert np.array_equal(ids, ref)
+
+
+def test_barneshut_sandbox(tmp_path):
+    rng = np.random.default_rng(24)
+    n = 4_000
+    pm = np.concatenate([rng.normal(size=(n, 3)), rng.random((n, 1)) + 0.5], axis=1).astype(np.float32)
+    path = tmp_path / "galaxy.txt"
+    with open(path, "w") as f:
+        f.write("0,0,0,0\n")  # the dummy line the reader skips
+        np.savetxt(f, pm, delimiter=",", fmt="%.9g")
+    lines = []
+    forces = barneshut.run(["barneshut", str(path), "0.0"], lines.append)
+    assert lines[:3] == [f"Loading Galaxy dataset from {path}...", f"Loaded {n} bodies.",
+                         "Compiling and launching RT-BarnesHut via GPRT DSL..."]
+    assert lines[3].startswith("Total Force Magnitude: ")
+    assert f"Forces Computed: {n}" in lines
+    # theta = 0 visits every body: the FP64 direct sum of the softened law
+    acc = oracle.bh_direct(pm, G=barneshut.G_NEWTON, eps=barneshut.EPS)
+    want = acc * pm[:, 3:4].astype(np.float64)
+    err = np.linalg.norm(forces - want, axis=1) / np.linalg.norm(want, axis=1)
+    assert err.max() < 2e-5
+    total = np.linalg.norm(want, axis=1).sum()
+    got = float(lines[3].split(": ")[1])
+    assert abs(got - total) / total < 1e-3
