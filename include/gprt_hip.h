@@ -146,6 +146,9 @@ typedef struct {
 /* ---------------- library / context ---------------- */
 GPRT_HIP_API int gprt_hip_version(void);
 GPRT_HIP_API const char* gprt_hip_last_error(void);
+/* Empty this thread's last-error message, so that a caller of the
+ * no-return-value reference symbols can tell whether a call failed. */
+GPRT_HIP_API void gprt_hip_clear_error(void);
 GPRT_HIP_API void gprt_hip_config_default(gprt_hip_config* cfg);
 GPRT_HIP_API int gprt_hip_ctx_create(int device, gprt_hip_ctx** out);
 GPRT_HIP_API int gprt_hip_ctx_destroy(gprt_hip_ctx* ctx);
@@ -293,7 +296,9 @@ typedef struct CGprtBvh CGprtBvh;
 
 /* main.cpp:101-180.  ir_data selects the device program: NULL/empty or a
  * descriptor "gprt-hip:knn-heap" (register-heap k-NN, default),
- * "gprt-hip:push" (per-query push of every hit), "gprt-hip:autorope" (the
+ * "gprt-hip:push" (per-query push of every hit; an array registered for
+ * one query but executed with several rays collects all their hits in one
+ * list, as gprt_trace! does), "gprt-hip:autorope" (the
  * Barnes-Hut walk), or "gprt-hip:ir:" followed by a shader-IR program in JSON,
  * compiled here for the GPU (see gprt_hip_codegen; NULL on a codegen or
  * compile error, with the message in gprt_hip_last_error).  Any other bytes

@@ -151,6 +151,7 @@ using namespace gprt;
 extern "C" int gprt_hip_version(void) { return 10000; }
 
 extern "C" const char* gprt_hip_last_error(void) { return g_err_msg.c_str(); }
+extern "C" void gprt_hip_clear_error(void) { g_err_msg.clear(); }
 
 extern "C" void gprt_hip_config_default(gprt_hip_config* c) {
     if (!c) return;

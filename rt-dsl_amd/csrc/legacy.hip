@@ -14,7 +14,10 @@
 //   "gprt-hip:push"      global push of gprt_rnn_run!/dispatch (:276-295): every
 //                        point with d64 <= r^2 (the IS test of :341), first `cap`
 //                        of them per query in (d64, id) order; lengths = total
-//                        count, like the atomicAdd counter.
+//                        count, like the atomicAdd counter.  An array
+//                        registered for one query and executed with several
+//                        rays holds the hits of all rays in one list (query
+//                        order), as gprt_trace! (:637-665) shares one counter.
 //   "gprt-hip:autorope"  the Barnes-Hut walk of barnes_hut! (:760-806) over the
 //                        caller's flattened octree.
 //   "gprt-hip:ir:" + JSON a program of the reference's shader IR, lowered to HIP
@@ -448,7 +451,10 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
     }
     for (auto& kv : p->arrays) {
         LegacyArray& a = kv.second;
-        if (p->program == PROG_GENERATED || a.nq < count || count == 0) continue;
+        // push program, one-query array, several rays: the hits of all rays
+        // share one list (gprt_trace!, gprt-macros/src/lib.rs:637-665)
+        const bool flat = p->program == PROG_PUSH && a.nq == 1 && count > 1;
+        if (p->program == PROG_GENERATED || (a.nq < count && !flat) || count == 0) continue;
         if (p->program == PROG_KNN_HEAP) {
             const int k = (int)(a.cap / 2);
             if (k < 1 || k > 64) {
@@ -480,8 +486,21 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
                           total, p->counts, p->d2, p->big, p->keys, p->perm, p->qs4, p->sums, nullptr,
                           p->radii.as<float>()) != GPRT_OK)
                 return;
-            hipLaunchKernelGGL(push_pack_kernel, dim3((count + 255) / 256), dim3(256), 0, s, p->off.as<int64_t>(),
-                               p->tmp_ids.as<uint32_t>(), count, a.cap, a.data.as<unsigned>(), a.lens.as<unsigned>());
+            if (flat) {
+                // query-major order, each ray's hits by (d64, id); the length
+                // counts every hit, the list keeps the first cap
+                const int64_t m = std::min<int64_t>(total, (int64_t)a.cap);
+                if (m > 0 && hipMemcpyAsync(a.data.ptr, p->tmp_ids.ptr, 4ull * m, hipMemcpyDeviceToDevice, s) !=
+                                 hipSuccess) {
+                    fail(GPRT_ERR_HIP, "gprt_execute: copy failed");
+                    return;
+                }
+                (void)launch_fill_u32(s, a.lens.as<unsigned>(), 1, (unsigned)std::min<int64_t>(total, 0xFFFFFFFFll));
+            } else {
+                hipLaunchKernelGGL(push_pack_kernel, dim3((count + 255) / 256), dim3(256), 0, s,
+                                   p->off.as<int64_t>(), p->tmp_ids.as<uint32_t>(), count, a.cap,
+                                   a.data.as<unsigned>(), a.lens.as<unsigned>());
+            }
         }
     }
     unsigned err = 0;
