@@ -199,3 +199,23 @@ def test_knn_device_pointers(ctx):
         t.k_nn(dq, 5, ids_out=out, sync=False)
     ctx.sync()
     _check_knn(out.to_host(), ref, label="device no-sync")
+
+
+@pytest.mark.parametrize("bits", [8, 12, 16, 20])
+def test_query_cell_sort_bits(ctx, bits):
+    """Query ordering by 8 to 20 Morton bits (one to three radix passes):
+    results are unchanged."""
+    L = gprt_hip.lib()
+    pts, q = D.uniform_cloud(50_000, 5_000, seed=8)
+    ref = oracle.knn_brute(pts, q, 5)
+    t = ctx.build(pts)
+    L.gprt_hip_debug_set_query_sort_bits(bits)
+    try:
+        cfg = gprt_hip.default_config(use_graph=0)
+        for _ in range(2):
+            _check_knn(t.k_nn(q, 5, cfg), ref, label=f"bits={bits}")
+        off, ids = t.range_query(q[:800], 0.03)
+        roff, rids = oracle.range_brute(pts, q[:800], 0.03)
+        assert np.array_equal(off, roff) and np.array_equal(ids, rids)
+    finally:
+        L.gprt_hip_debug_set_query_sort_bits(16)
