@@ -15,6 +15,18 @@ namespace gprt {
 
 // ---------------------------------------------------------------------------
 // The same round with one independent walk per lane (closest child first).
+//
+// Every trip of the walk loop issues exactly one 64-byte load per lane: either
+// the next node (two child boxes) or the next four points of the leaf range
+// being scanned.  A node's in-range leaf buckets (the two buckets of a node
+// are adjacent ranges of the sorted points, so both form one range) are
+// scanned over the following trips, and only then is the node's successor
+// chosen, with the threshold those points left; so the walk visits exactly the
+// nodes a scan-then-descend walk visits.  What the one-load trips buy is
+// lockstep: a wavefront waits one memory latency per trip whether its lanes
+// are descending or scanning, instead of one per node plus the longest scan
+// of any lane.
+//
 // The stack holds (node ref, d32 of its box) pairs; a popped entry whose box
 // is already beyond the threshold is dropped without loading the node.  The S
 // most recent entries of each lane live in an LDS ring ([slot][lane], so lanes
@@ -33,7 +45,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
     const int tid = threadIdx.x, lane = tid & 63;
     const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
     if (n_in == 0) return;
-    const Node* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ nodes4 = (const float4*)a.nodes;
     const float4* __restrict__ pts = a.pts;
     const size_t gstride = (size_t)gridDim.x * 256;
     int2* __restrict__ spill = a.spill + (size_t)blockIdx.x * 256 + tid;
@@ -69,35 +81,54 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
             T0 = last ? INFINITY : R * R;
             Q.start(q.x, q.y, q.z, T0, k, true);
             overflow = false;
-            if (root.y > 0 && root.x < 0) {
-                scan_bucket(Q, pts, ~root.x, root.y);
-                if (COUNT) tests += root.y;
-            } else if (root.y > 0) {
+            if (root.y > 0) {
                 int sp = 0, lo = 0;  // stack depth; entries [0, lo) are in HBM
-                int node = root.x, steps = 0;
+                int steps = 0, trips = 0;
+                // walk state: the node to load next, or a leaf range [sf, sf + sm)
+                // whose points from sf + st on are still to be scanned
+                bool scanning = root.x < 0;
+                int node = root.x;
+                int sf = scanning ? ~root.x : 0, sm = scanning ? root.y : 0, st = 0;
+                // children of the last node: box distances and refs (negative
+                // refs are leaf buckets, never descended into)
+                float cdA = INFINITY, cdB = INFINITY;
+                int crA = -1, crB = -1;
                 while (true) {
-                    if (++steps > a.max_steps) { overflow = true; break; }
-                    const Node nd = nodes[node];
-                    if (COUNT) ++visits;
-                    const float dA = box2f(Q.qx, Q.qy, Q.qz, nd.alo, nd.ahi);
-                    const float dB = box2f(Q.qx, Q.qy, Q.qz, nd.blo, nd.bhi);
-                    const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
-                    const bool lA = rA < 0, lB = rB < 0;
-                    const bool bfirst = dB < dA;
-                    // leaf buckets in range are scanned at once, from one code
-                    // site: the two buckets of a node are adjacent ranges of
-                    // the sorted points (A's then B's), so both are one range
-                    {
-                        const bool sA = lA && dA <= Q.tau, sB = lB && dB <= Q.tau;
-                        const int mA = __float_as_int(nd.ahi.w), mB = __float_as_int(nd.bhi.w);
-                        const int f = sA ? ~rA : ~rB;
-                        const int m = (sA ? mA : 0) + (sB ? mB : 0);
-                        if (m > 0) {
-                            scan_bucket(Q, pts, f, m);
-                            if (COUNT) tests += m;
+                    ++trips;
+                    const float4* src = scanning ? pts + sf + st : nodes4 + 4 * (size_t)node;
+                    const int avail = scanning ? sm - st : 4;
+                    float4 v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (u < avail) v[u] = src[u];
+                    if (!scanning) {
+                        if (++steps > a.max_steps) { overflow = true; break; }
+                        if (COUNT) ++visits;
+                        // v = child A's box (lo, hi), child B's box (lo, hi)
+                        cdA = box2f(Q.qx, Q.qy, Q.qz, v[0], v[1]);
+                        cdB = box2f(Q.qx, Q.qy, Q.qz, v[2], v[3]);
+                        crA = __float_as_int(v[0].w);
+                        crB = __float_as_int(v[2].w);
+                        const bool sA = crA < 0 && cdA <= Q.tau, sB = crB < 0 && cdB <= Q.tau;
+                        sf = sA ? ~crA : ~crB;
+                        sm = (sA ? __float_as_int(v[1].w) : 0) + (sB ? __float_as_int(v[3].w) : 0);
+                        st = 0;
+                        if (sm > 0) {
+                            if (COUNT) tests += sm;
+                            scanning = true;
+                            continue;
                         }
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (u < avail) Q.visit_point(v[u], sf + st + u);
+                        st += 4;
+                        if (st < sm) continue;
+                        scanning = false;
                     }
-                    const bool gA = !lA && dA <= Q.tau, gB = !lB && dB <= Q.tau;
+                    // the last node's successor, with the threshold its buckets left
+                    const bool gA = crA >= 0 && cdA <= Q.tau, gB = crB >= 0 && cdB <= Q.tau;
+                    const bool bfirst = cdB < cdA;
                     if (gA && gB) {
                         if (sp >= max_depth) { overflow = true; break; }
                         if (sp - lo == S) {  // ring full: its oldest entry goes to HBM
@@ -105,14 +136,14 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                             spill[lo * gstride] = make_int2(sref[j][tid], __float_as_int(sdist[j][tid]));
                             ++lo;
                         }
-                        sref[sp & (S - 1)][tid] = bfirst ? rA : rB;
-                        sdist[sp & (S - 1)][tid] = bfirst ? dA : dB;
+                        sref[sp & (S - 1)][tid] = bfirst ? crA : crB;
+                        sdist[sp & (S - 1)][tid] = bfirst ? cdA : cdB;
                         ++sp;
-                        node = bfirst ? rB : rA;
+                        node = bfirst ? crB : crA;
                         continue;
                     }
-                    if (gA) { node = rA; continue; }
-                    if (gB) { node = rB; continue; }
+                    if (gA) { node = crA; continue; }
+                    if (gB) { node = crB; continue; }
                     // pop the nearest-pushed entry still inside the threshold
                     node = -1;
                     while (sp > 0) {
@@ -132,7 +163,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                     }
                     if (node < 0) break;
                 }
-                task_steps += steps;
+                task_steps += trips;
             }
             resolved = last || (Q.kth() < INFINITY && Q.kth() <= T0);
             if (resolved || overflow || !a.inline_rounds) break;

@@ -42,6 +42,11 @@ def test_knn_k_values(ctx, k):
     t = ctx.build(pts, leaf_size=8)
     _check_knn(t.k_nn(q, k), ref, label=f"batched k={k}")
     _check_knn(t.k_nn_plain(q, k), ref, label=f"plain k={k}")
+    # the 4-wide walk, also with most queries reaching the unbounded last round
+    for kw in (dict(traversal=gprt_hip.TRAV_WIDE),
+               dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4,
+                    max_rounds=1)):
+        _check_knn(t.k_nn(q, k, gprt_hip.default_config(**kw)), ref, label=f"batched k={k} {kw}")
 
 
 @pytest.mark.parametrize("leaf", [1, 2, 4, 16, 32])
@@ -73,6 +78,7 @@ def test_knn_leaf_sizes(ctx, leaf):
     dict(traversal=gprt_hip.TRAV_WIDE, lds_stack=8, stack_depth=10),  # wide walks overflow: exact fallback
     dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_INFINITE, lds_stack=16),
     dict(traversal=gprt_hip.TRAV_WAVE),
+    dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=2),
     dict(compact_rounds=1),
     dict(compact_rounds=1, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=15, radius_mult=2.0),
     dict(traversal=gprt_hip.TRAV_WIDE, radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-4, max_rounds=15,
