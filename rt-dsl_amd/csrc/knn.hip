@@ -15,6 +15,7 @@
 #include <cfloat>
 
 #include "knn_device.hpp"
+#include "radix.hpp"
 
 namespace gprt {
 
@@ -490,6 +491,49 @@ __global__ void query_gather_kernel(const float* __restrict__ qs, const uint32_t
     if (i == 0) counters[CNT_ACTIVE] = (unsigned)nq;
 }
 
+// the same key as query_key32_kernel for one cloud, generated inside the
+// first radix pass's histogram kernel
+struct QueryKeyGen {
+    const float* qs;
+    const float* bbox;
+    int use_morton;
+    uint32_t head;
+    int mshift;  // 30 - Morton bits kept
+    unsigned* flag;
+    __device__ __forceinline__ uint32_t operator()(int i, bool& bad) const {
+        const float x = qs[3 * (size_t)i], y = qs[3 * (size_t)i + 1], z = qs[3 * (size_t)i + 2];
+        bad |= !(fabsf(x) <= 1e18f && fabsf(y) <= 1e18f && fabsf(z) <= 1e18f);
+        uint32_t m = 0;
+        if (use_morton) {
+            const float ex = fmaxf(bbox[3] - bbox[0], 1e-6f), ey = fmaxf(bbox[4] - bbox[1], 1e-6f),
+                        ez = fmaxf(bbox[5] - bbox[2], 1e-6f);
+            m = (expand10(quant10(x, bbox[0], ex)) << 2) | (expand10(quant10(y, bbox[1], ey)) << 1) |
+                expand10(quant10(z, bbox[2], ez));
+        }
+        return head | (m >> mshift);
+    }
+    __device__ __forceinline__ void report(bool bad) const {
+        if (__ballot(bad) && __lane_id() == 0) atomicOr(flag, 1u);
+    }
+};
+
+// the last radix pass writes each query's float4 (input index in .w) and cloud
+// at its sorted position: the gather fused into the scatter
+struct QueryGatherSink {
+    const float* qs;
+    float4* out;
+    int* qcloud;
+    int cshift;
+    uint32_t cmask;
+    unsigned* counters;
+    int n;
+    __device__ __forceinline__ void operator()(uint32_t pos, uint32_t k, uint32_t v) const {
+        out[pos] = make_float4(qs[3 * (size_t)v], qs[3 * (size_t)v + 1], qs[3 * (size_t)v + 2], __int_as_float((int)v));
+        if (qcloud) qcloud[pos] = (int)((k >> cshift) & cmask);
+        if (pos == 0) counters[CNT_ACTIVE] = (unsigned)n;
+    }
+};
+
 static int cloud_bits_of(int nclouds) {
     int cb = 0;
     while ((1 << cb) < nclouds) ++cb;
@@ -514,14 +558,29 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
     const int qbits = use_morton ? std::min(query_sort_bits(), mbits) : 0;
     const int begin = mbits - qbits;
     uint32_t* k32 = (uint32_t*)keys;
-    const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
-    hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb, kc,
-                       k_uniform, k32, perm, flag);
     // the class bits (29..31) only need sorting when several k classes are present
-    uint32_t *sk = k32, *sv = perm;
-    GPRT_TRY(radix_sort_pairs_u32(ctx, k32, perm, nq, begin, sort_classes ? 32 : 29, &sk, &sv));
-    hipLaunchKernelGGL(query_gather_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, qs, sv, sk, cb, nq, qs4,
-                       qcloud, counters);
+    const int end = sort_classes ? 32 : 29;
+    const QueryGatherSink fin{qs, qs4, qcloud, mbits, (1u << cb) - 1u, counters, nq};
+    // one cloud (so one k class, whose bits are not sorted): the keys come out
+    // of the first pass's histogram kernel, without the class bits
+    const bool fused_keys = nclouds == 1 && !sort_classes && nq > 1 && begin < end;
+    if (fused_keys) {
+        GPRT_TRY(ctx->sort_tmp[2].reserve(radix_sort_hist_bytes(nq)));
+        const QueryKeyGen gen{qs, bbox, use_morton, 0u, 30 - mbits, flag};
+        hipLaunchKernelGGL(rs_keygen_hist_kernel<QueryKeyGen>, dim3((nq + RS_TILE - 1) / RS_TILE), dim3(RS_THREADS),
+                           0, s, gen, nq, begin, (1u << std::min(8, end - begin)) - 1u, k32, perm,
+                           ctx->sort_tmp[2].as<uint32_t>());
+    } else {
+        const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
+        hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb,
+                           kc, k_uniform, k32, perm, flag);
+    }
+    int passes = 0;
+    GPRT_TRY((radix_sort_run<uint32_t, QueryGatherSink>(ctx, k32, perm, nq, begin, end, fused_keys, &fin, nullptr,
+                                                        nullptr, &passes)));
+    if (passes == 0)  // nothing to sort: input order
+        hipLaunchKernelGGL(query_gather_kernel, dim3(std::max(1, (nq + 255) / 256)), dim3(256), 0, s, qs, perm, k32,
+                           cb, nq, qs4, qcloud, counters);
     return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "query prep launch failed");
 }
 

@@ -13,6 +13,7 @@ namespace gprt {
 #define GPRT_LANE_MIN_WAVES 1
 #endif
 
+
 // ---------------------------------------------------------------------------
 // The same round with one independent walk per lane (closest child first).
 //
@@ -95,12 +96,16 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                 int crA = -1, crB = -1;
                 while (true) {
                     ++trips;
-                    const float4* src = scanning ? pts + sf + st : nodes4 + 4 * (size_t)node;
                     const int avail = scanning ? sm - st : 4;
                     float4 v[4];
+                    // four loads without branches (so they issue back to back
+                    // whichever lanes scan): past the end of a leaf range the index
+                    // is clamped to its last point, which is loaded but not visited
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (u < avail) v[u] = src[u];
+                    for (int u = 0; u < 4; ++u) {
+                        const size_t e = scanning ? (size_t)(sf + st + min(u, avail - 1)) : 4 * (size_t)node + u;
+                        v[u] = (scanning ? pts : nodes4)[e];
+                    }
                     if (!scanning) {
                         if (++steps > a.max_steps) { overflow = true; break; }
                         if (COUNT) ++visits;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "gprt_internal.hpp"
+#include "radix.hpp"
 #include "refit.hpp"
 
 namespace gprt {
@@ -169,207 +170,16 @@ __global__ void morton_codes_kernel(const float* __restrict__ pts, int n, const 
     }
 }
 
-// ---------------------------------------------------------------------------
-// stable LSD radix sort, 8-bit digits, 4096-key tiles (256 lanes x 16 keys)
-// ---------------------------------------------------------------------------
-constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
-constexpr int RS_BINS = 256;
-constexpr int RS_WAVES = RS_THREADS / 64;
-
-// hist layout: tile-major [nblocks][RS_BINS] (each tile writes 1 KB
-// contiguously); rs_offsets_kernel turns it into the output offset of every
-// (tile, digit) pair.
-template <typename K>
-__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict__ keys, int n, int shift,
-                                                             uint32_t mask, uint32_t* __restrict__ hist,
-                                                             int nblocks) {
-    __shared__ uint32_t h[RS_BINS];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    int base = blockIdx.x * RS_TILE;
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-        int idx = base + i * RS_THREADS + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & mask], 1u);
-    }
-    __syncthreads();
-    hist[blockIdx.x * RS_BINS + threadIdx.x] = h[threadIdx.x];
-}
-
-// One block of 1024 threads = 4 groups x 256 digits.  Group g scans tiles
-// [g*nb/4, (g+1)*nb/4) per digit; the groups and the digit totals are then
-// combined so hist[b][d] = (keys with digit < d) + (keys with digit d in tiles < b).
-__global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nblocks) {
-    __shared__ uint32_t gsum[4][RS_BINS];
-    __shared__ uint32_t dtot[RS_BINS];
-    const int d = threadIdx.x & 255, g = threadIdx.x >> 8;
-    const int per = (nblocks + 3) / 4;
-    const int b0 = min(g * per, nblocks), b1 = min(b0 + per, nblocks);
-    // both passes over the tiles issue RS_UNROLL independent loads before using
-    // any of them, so a group waits for one memory latency per RS_UNROLL tiles
-    constexpr int RS_UNROLL = 8;
-    uint32_t run = 0;
-    int b = b0;
-    for (; b + RS_UNROLL <= b1; b += RS_UNROLL) {
-        uint32_t v[RS_UNROLL];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) run += v[u];
-    }
-    for (; b < b1; ++b) run += hist[b * RS_BINS + d];
-    gsum[g][d] = run;
-    __syncthreads();
-    if (g == 0) {
-        uint32_t t = 0;
-        for (int gg = 0; gg < 4; ++gg) {
-            uint32_t v = gsum[gg][d];
-            gsum[gg][d] = t;
-            t += v;
-        }
-        dtot[d] = t;
-    }
-    __syncthreads();
-    // exclusive scan of the 256 digit totals (Hillis-Steele, first 256 lanes)
-    uint32_t tot = g == 0 ? dtot[d] : 0u;
-    for (int o = 1; o < RS_BINS; o <<= 1) {
-        uint32_t v = (g == 0 && d >= o) ? dtot[d - o] : 0u;
-        __syncthreads();
-        if (g == 0) dtot[d] += v;
-        __syncthreads();
-    }
-    if (g == 0) dtot[d] -= tot;
-    __syncthreads();
-    run = dtot[d] + gsum[g][d];
-    for (b = b0; b + RS_UNROLL <= b1; b += RS_UNROLL) {
-        uint32_t v[RS_UNROLL];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) {
-            hist[(b + u) * RS_BINS + d] = run;
-            run += v[u];
-        }
-    }
-    for (; b < b1; ++b) {
-        uint32_t v = hist[b * RS_BINS + d];
-        hist[b * RS_BINS + d] = run;
-        run += v;
-    }
-}
-
-template <typename K>
-__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin,
-                                                                K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                                int n, int shift, uint32_t mask,
-                                                                const uint32_t* __restrict__ hist, int nblocks) {
-    __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
-    __shared__ uint32_t boff[RS_BINS];
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    for (int i = t; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    boff[t] = hist[blockIdx.x * RS_BINS + t];
-    __syncthreads();
-
-    const int base = blockIdx.x * RS_TILE + w * (64 * RS_ITEMS);
-    K k[RS_ITEMS];
-    uint32_t v[RS_ITEMS];
-    uint32_t rank[RS_ITEMS];
-    const uint64_t lt = lanemask_lt();
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-        int idx = base + i * 64 + lane;
-        bool valid = idx < n;
-        k[i] = valid ? kin[idx] : (K)0;
-        v[i] = valid ? vin[idx] : 0u;
-        uint32_t d = (uint32_t)(k[i] >> shift) & mask;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            bool b = (d >> bit) & 1u;
-            uint64_t m = __ballot(b);
-            peers &= b ? m : ~m;
-        }
-        uint32_t cur = valid ? wcnt[w][d] : 0u;
-        rank[i] = cur + (uint32_t)__popcll(peers & lt);
-        __builtin_amdgcn_wave_barrier();
-        if (valid && lane == __ffsll((unsigned long long)peers) - 1) wcnt[w][d] = cur + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
-    {
-        uint32_t s = 0;
-#pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ++ww) {
-            uint32_t c = wcnt[ww][t];
-            wcnt[ww][t] = s;
-            s += c;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-        int idx = base + i * 64 + lane;
-        if (idx < n) {
-            uint32_t d = (uint32_t)(k[i] >> shift) & mask;
-            uint32_t pos = boff[d] + wcnt[w][d] + rank[i];
-            kout[pos] = k[i];
-            vout[pos] = v[i];
-        }
-    }
-}
-
-// res_keys / res_vals: when given, the sorted result is left in whichever of
-// (keys, vals) and the context's temporaries the last pass wrote, and these
-// receive its location (no copy back after an odd number of passes)
-template <typename K>
-static int radix_sort_impl(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit, K** res_keys,
-                           uint32_t** res_vals) {
-    if (res_keys) *res_keys = keys;
-    if (res_vals) *res_vals = vals;
-    if (n <= 1 || begin_bit >= end_bit) return GPRT_OK;
-    int nblocks = (n + RS_TILE - 1) / RS_TILE;
-    GPRT_TRY(ctx->sort_tmp[0].reserve(sizeof(K) * (size_t)n));
-    GPRT_TRY(ctx->sort_tmp[1].reserve(sizeof(uint32_t) * (size_t)n));
-    GPRT_TRY(ctx->sort_tmp[2].reserve(sizeof(uint32_t) * (size_t)RS_BINS * nblocks));
-    K* k0 = keys;
-    uint32_t* v0 = vals;
-    K* k1 = ctx->sort_tmp[0].as<K>();
-    uint32_t* v1 = ctx->sort_tmp[1].as<uint32_t>();
-    uint32_t* hist = ctx->sort_tmp[2].as<uint32_t>();
-    int passes = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
-        const int nb = std::min(8, end_bit - shift);
-        const uint32_t mask = (1u << nb) - 1u;
-        hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, n, shift, mask,
-                           hist, nblocks);
-        hipLaunchKernelGGL(rs_offsets_kernel, dim3(1), dim3(1024), 0, ctx->stream, hist, nblocks);
-        hipLaunchKernelGGL(rs_scatter_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, k0, v0, k1, v1,
-                           n, shift, mask, hist, nblocks);
-        std::swap(k0, k1);
-        std::swap(v0, v1);
-    }
-    GPRT_HIP_TRY(hipGetLastError());
-    if (res_keys && res_vals) {
-        *res_keys = k0;
-        *res_vals = v0;
-    } else if (k0 != keys) {
-        GPRT_HIP_TRY(hipMemcpyAsync(keys, k0, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
-        GPRT_HIP_TRY(hipMemcpyAsync(vals, v0, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    return GPRT_OK;
-}
-
 size_t radix_sort_hist_bytes(int n) { return sizeof(uint32_t) * (size_t)RS_BINS * ((n + RS_TILE - 1) / RS_TILE) + 1024; }
 
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
-    return radix_sort_impl<uint64_t>(ctx, keys, vals, n, begin_bit, end_bit, nullptr, nullptr);
+    return radix_sort_run<uint64_t, KVSink<uint64_t>>(ctx, keys, vals, n, begin_bit, end_bit, false, nullptr,
+                                                      nullptr, nullptr, nullptr);
 }
 int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit,
                          uint32_t** res_keys, uint32_t** res_vals) {
-    return radix_sort_impl<uint32_t>(ctx, keys, vals, n, begin_bit, end_bit, res_keys, res_vals);
+    return radix_sort_run<uint32_t, KVSink<uint32_t>>(ctx, keys, vals, n, begin_bit, end_bit, false, nullptr,
+                                                      res_keys, res_vals, nullptr);
 }
 
 // ---------------------------------------------------------------------------
