@@ -93,16 +93,18 @@ struct F32Query {
         ev_min = INFINITY;
         L.init(C - k - G);
     }
+    // One branch per point (inside the band or not).  Within the band a point
+    // that does not beat the last slot (rare: the guard slots are full) is
+    // inserted as +inf, which leaves the sorted list unchanged, rather than
+    // taking a second branch.
     __device__ __forceinline__ void visit_point(float4 p, int s) {
         const float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
         if (dd <= tau) {
-            if (dd < L.d[C - 1]) {
-                ev_min = fminf(ev_min, L.insert(dd, s));
-                T = fminf(T0, L.d[C - 1 - G]);
-                tau = band(T);
-            } else {
-                ev_min = fminf(ev_min, dd);
-            }
+            const bool ins = dd < L.d[C - 1];
+            const float ev = L.insert(ins ? dd : INFINITY, s);
+            ev_min = fminf(ev_min, ins ? ev : dd);
+            T = fminf(T0, L.d[C - 1 - G]);
+            tau = band(T);
         }
     }
     __device__ __forceinline__ float kth() const { return L.d[C - 1 - G]; }
