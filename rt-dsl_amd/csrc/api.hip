@@ -468,23 +468,24 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     CtxExtra* x = extra(&c);
     hipLaunchKernelGGL(init_call_kernel, dim3(1), dim3(64), 0, s, counters, CNT_ALL,
                        p.nclouds == 1 ? (int64_t*)p.d_qoff : nullptr, p.nq);
-    // the radius sampler runs on the side stream, overlapping the query sort;
-    // the rounds wait for both
-    GPRT_HIP_TRY(hipEventRecord(c.ev_fork, s));
-    GPRT_HIP_TRY(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-    GPRT_TRY(launch_sampler(c.side, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(),
-                            T.cloud_off.as<int64_t>(), p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic,
-                            cfg.radius_param, kth, radius));
-    GPRT_HIP_TRY(hipEventRecord(c.ev_join, c.side));
     uint64_t* keys = c.ws[WS_QKEYS].as<uint64_t>();
     uint32_t* perm = c.ws[WS_QPERM].as<uint32_t>();
     float4* qs4 = c.ws[WS_QS4].as<float4>();
     int* qcloud = p.nclouds > 1 ? x->qcloud.as<int>() : nullptr;
     int nclasses = 0;
     for (int cl = 0; cl < N_KCLASS; ++cl) nclasses += p.class_size[cl] > 0;
+    // the radius sampler runs on the side stream, overlapping the query sort;
+    // the rounds wait for both.  The sort, the longer branch, is enqueued
+    // first (a captured graph launches its branches in that order).
+    GPRT_HIP_TRY(hipEventRecord(c.ev_fork, s));
     GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
                                perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
                                p.d_kc, p.k, nclasses > 1));
+    GPRT_HIP_TRY(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    GPRT_TRY(launch_sampler(c.side, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(),
+                            T.cloud_off.as<int64_t>(), p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic,
+                            cfg.radius_param, kth, radius));
+    GPRT_HIP_TRY(hipEventRecord(c.ev_join, c.side));
     GPRT_HIP_TRY(hipStreamWaitEvent(s, c.ev_join, 0));
     int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : 0;
     int R = cfg.radius_heuristic == GPRT_RADIUS_INFINITE ? 1 : std::min(cfg.max_rounds, MAX_ROUNDS - 1) + 1;
@@ -613,8 +614,10 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
     return GPRT_OK;
 }
 
-static int finish_stats(Ctx& c, int64_t nq, bool read_counters) {
+// d_radius: the sampled radius of cloud 0 on the device (batched k-NN), else null
+static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_radius = nullptr) {
     GPRT_HIP_TRY(hipStreamSynchronize(c.stream));
+    if (d_radius) GPRT_HIP_TRY(hipMemcpy(&c.stats.initial_radius, d_radius, sizeof(float), hipMemcpyDeviceToHost));
     float ms = 0;
     if (hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess) {
         (void)hipGetLastError();
@@ -708,7 +711,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
         if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nq * k, hipMemcpyDeviceToHost, s));
     }
     if (flags & GPRT_NO_SYNC) return GPRT_OK;
-    return finish_stats(c, nq, true);
+    return finish_stats(c, nq, true, c.ws[WS_MISC].as<float>() + 100 * (p.nclouds));
 }
 
 extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
@@ -835,7 +838,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
         if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nout, hipMemcpyDeviceToHost, s));
     }
     if (flags & GPRT_NO_SYNC) return GPRT_OK;
-    return finish_stats(c, nq, true);
+    return finish_stats(c, nq, true, c.ws[WS_MISC].as<float>() + 100 * (p.nclouds));
 }
 
 extern "C" int gprt_hip_stats(gprt_hip_ctx* h, gprt_hip_run_stats* out) {

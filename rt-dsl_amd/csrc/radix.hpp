@@ -39,17 +39,17 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict
 }
 
 // The first pass's histogram from a key generator: keys[i] = gen(i, bad),
-// vals[i] = i, with the same tiling as rs_hist_kernel.  gen.report(bad) is
-// called once per lane with whether any of its keys came from a bad input.
+// vals[i] = i, with the same tiling as rs_hist_kernel (workgroup `tile` of
+// RS_THREADS lanes, all of which call this).  gen.report(bad) is called once
+// per lane with whether any of its keys came from a bad input.
 template <class KeyGen>
-__global__ __launch_bounds__(RS_THREADS) void rs_keygen_hist_kernel(KeyGen gen, int n, int shift, uint32_t mask,
-                                                                    uint32_t* __restrict__ keys,
-                                                                    uint32_t* __restrict__ vals,
-                                                                    uint32_t* __restrict__ hist) {
+__device__ __forceinline__ void rs_keygen_hist_tile(const KeyGen& gen, int n, int shift, uint32_t mask,
+                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ hist, int tile) {
     __shared__ uint32_t h[RS_BINS];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
+    const int base = tile * RS_TILE;
     bool bad = false;
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
@@ -63,7 +63,15 @@ __global__ __launch_bounds__(RS_THREADS) void rs_keygen_hist_kernel(KeyGen gen, 
     }
     gen.report(bad);
     __syncthreads();
-    hist[blockIdx.x * RS_BINS + threadIdx.x] = h[threadIdx.x];
+    hist[tile * RS_BINS + threadIdx.x] = h[threadIdx.x];
+}
+
+template <class KeyGen>
+__global__ __launch_bounds__(RS_THREADS) void rs_keygen_hist_kernel(KeyGen gen, int n, int shift, uint32_t mask,
+                                                                    uint32_t* __restrict__ keys,
+                                                                    uint32_t* __restrict__ vals,
+                                                                    uint32_t* __restrict__ hist) {
+    rs_keygen_hist_tile(gen, n, shift, mask, keys, vals, hist, blockIdx.x);
 }
 
 // One block of 1024 threads = 4 groups x 256 digits.  Group g scans tiles
@@ -111,15 +119,26 @@ __global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nb
         dtot[d] = t;
     }
     __syncthreads();
-    // exclusive scan of the 256 digit totals (Hillis-Steele, first 256 lanes)
-    const uint32_t tot = g == 0 ? dtot[d] : 0u;
-    for (int o = 1; o < RS_BINS; o <<= 1) {
-        const uint32_t v = (g == 0 && d >= o) ? dtot[d - o] : 0u;
-        __syncthreads();
-        if (g == 0) dtot[d] += v;
-        __syncthreads();
+    // exclusive scan of the 256 digit totals by the first four wavefronts:
+    // a shuffle scan within each, then the wavefront totals
+    __shared__ uint32_t wtot[4];
+    if (g == 0) {
+        const uint32_t tot = dtot[d];
+        uint32_t inc = tot;
+        const int lane = d & 63;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t x = __shfl_up(inc, o);
+            if (lane >= o) inc += x;
+        }
+        if (lane == 63) wtot[d >> 6] = inc;
+        dtot[d] = inc - tot;
     }
-    if (g == 0) dtot[d] -= tot;
+    __syncthreads();
+    if (g == 0) {
+        uint32_t before = 0;
+        for (int w = 0; w < (d >> 6); ++w) before += wtot[w];
+        dtot[d] += before;
+    }
     __syncthreads();
     run = dtot[d] + gsum[g][d];
     if (in_regs) {

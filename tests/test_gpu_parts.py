@@ -55,6 +55,23 @@ def test_samplers_bit_exact(ctx, sampler, heur, param, k):
         assert got == want, (got, want)
 
 
+@pytest.mark.parametrize("heur,param", [(gprt_hip.RADIUS_PERCENTILE, 0.95), (gprt_hip.RADIUS_PERCENTILE, 0.1),
+                                         (gprt_hip.RADIUS_MAX, 0.0)])
+def test_knn_uses_sampled_radius(ctx, heur, param):
+    """The radius a k-NN call starts from (run statistics) is the standalone
+    sampler's and the oracle's, eager and on replays of the captured graph."""
+    pts, q = D.lidar_scan(frames=3, nq=20_000)
+    want = oracle.sampler_lcg(pts, q, heur, param)
+    assert ctx.sample_radius(pts, q, gprt_hip.SAMPLER_LCG, heur, param, 5) == want
+    t = ctx.build(pts)
+    ref = oracle.knn_brute(pts, q, 5)
+    for use_graph in (0, 1):
+        cfg = gprt_hip.default_config(radius_heuristic=heur, radius_param=param, use_graph=use_graph)
+        for _ in range(3):
+            assert np.array_equal(t.k_nn(q, 5, cfg), ref)
+            assert np.float32(ctx.stats()["initial_radius"]) == np.float32(want)
+
+
 def test_sampler_small_inputs(ctx):
     pts = np.zeros((3, 3), np.float32)
     q = np.ones((5, 3), np.float32)
