@@ -84,16 +84,16 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
                       int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc = nullptr,
                       int k_uniform = 1, int sort_classes = 0);
-// k classes: queries of one class share a candidate-list capacity.  Classes
-// 0..3 run the FP32 rounds with capacity 8/16/32/64 (k + 3 guard slots);
-// class 4 (k > 61) goes straight to the exact FP64 kernel.
 // k classes: queries of one class share a candidate list of capacity C with
-// G = 3 guard slots, the smallest C of {8, 16, 24, 32, 48, 64, 67} with
+// G = 3 guard slots, the smallest C of {8, 16, 24, 32, 40, 48, 64, 67} with
 // k + G <= C (registers hold the list, so a tighter class runs at a higher
 // occupancy).
-constexpr int N_KCLASS = 7;
+constexpr int N_KCLASS = 8;  // the query sort key holds the class in 3 bits
 __host__ __device__ inline int k_class_cap(int cls) {
-    return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 24 : (cls == 3 ? 32 : (cls == 4 ? 48 : (cls == 5 ? 64 : 67)))));
+    return cls == 0 ? 8
+                    : (cls == 1 ? 16
+                                : (cls == 2 ? 24
+                                            : (cls == 3 ? 32 : (cls == 4 ? 40 : (cls == 5 ? 48 : (cls == 6 ? 64 : 67))))));
 }
 __host__ __device__ inline int k_class(int k) {
     int c = 0;

@@ -33,7 +33,7 @@ def test_knn_uniform_plain_and_batched(ctx):
     assert st["queries"] == len(q)
 
 
-@pytest.mark.parametrize("k", [1, 2, 5, 13, 16, 32, 61, 62, 64])
+@pytest.mark.parametrize("k", [1, 2, 5, 13, 16, 29, 32, 37, 38, 45, 61, 62, 64])
 def test_knn_k_values(ctx, k):
     rng = np.random.default_rng(10 + k)
     pts = rng.random((20_000, 3), dtype=np.float32)
