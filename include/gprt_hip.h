@@ -253,6 +253,13 @@ GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t 
                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                    const char* json_path, int ptr_flags);
 
+/* ---------------- testing hooks ---------------- */
+/* Morton bits (8..30, default 16) by which the batched k-NN orders its queries
+ * before the rounds.  Process-wide and not synchronised: set it only while no
+ * other thread calls the library.  Results never depend on it; a captured
+ * graph keeps the ordering it was captured with (use_graph = 0 to compare). */
+GPRT_HIP_API void gprt_hip_debug_set_query_sort_bits(int bits);
+
 /* ---------------- sandbox inputs ---------------- */
 #define GPRT_CSV_POINTS 0 /* "x,y,z[,...]" per line (gprt-sandbox/src/io.rs:8-30) */
 #define GPRT_CSV_BODIES 1 /* "x,y,z,m[,...]" per line (gprt-sandbox/src/bin/barneshut.rs:8-34) */

@@ -71,19 +71,19 @@ __global__ __launch_bounds__(256) void bbox_reduce_kernel(const float* __restric
     }
 }
 
-// Box used to normalise the sort keys.  mode 0: the cloud's own box (per-axis
-// normalisation, morton.rs:37-47); mode 1: a cube of the largest extent, so a
-// flat cloud (a LiDAR scene is ~240 x 240 x 6 m) is not cut along its thin axis
-// as often as along its wide ones.  Keys only order the points: the search
-// results do not depend on this choice.
-__global__ void key_box_kernel(const float* __restrict__ bbox, int nclouds, int mode, float* __restrict__ kb) {
+// Box used to normalise the sort keys: a cube of the cloud's largest extent
+// rather than its own box (the per-axis normalisation of morton.rs:37-47), so
+// a flat cloud (a LiDAR scene is ~240 x 240 x 6 m) is not cut along its thin
+// axis as often as along its wide ones.  Keys only order the points: the
+// search results do not depend on this choice.
+__global__ void key_box_kernel(const float* __restrict__ bbox, int nclouds, float* __restrict__ kb) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nclouds) return;
     const float* b = bbox + 6 * c;
     float e = fmaxf(b[3] - b[0], fmaxf(b[4] - b[1], b[5] - b[2]));
     for (int a = 0; a < 3; ++a) {
         kb[6 * c + a] = b[a];
-        kb[6 * c + 3 + a] = mode == 1 ? b[a] + e : b[3 + a];
+        kb[6 * c + 3 + a] = b[a] + e;
     }
 }
 
@@ -319,8 +319,6 @@ __global__ void fill_int_kernel(int* p, int n, int v) {
 // ---------------------------------------------------------------------------
 static inline int cdiv(int64_t a, int b) { return (int)((a + b - 1) / b); }
 
-static int g_key_mode = 1;
-extern "C" GPRT_HIP_API void gprt_hip_debug_set_key_mode(int m) { g_key_mode = m ? 1 : 0; }
 
 int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, int leaf_size, Tree* t,
                int axis_bits) {
@@ -370,7 +368,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     hipLaunchKernelGGL(bbox_decode_kernel, dim3(cdiv(6 * nclouds, 256)), dim3(256), 0, s, enc,
                        t->cloud_bbox.as<float>(), nclouds);
     hipLaunchKernelGGL(key_box_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, t->cloud_bbox.as<float>(),
-                       nclouds, g_key_mode, t->key_bbox.as<float>());
+                       nclouds, t->key_bbox.as<float>());
     hipLaunchKernelGGL(roots_small_kernel, dim3(cdiv(nclouds, 256)), dim3(256), 0, s, t->cloud_off.as<int64_t>(),
                        nclouds, leaf_size, t->roots.as<int2>());
     if (n > 0) {
