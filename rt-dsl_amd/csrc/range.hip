@@ -208,16 +208,18 @@ __device__ void lds_bitonic(double* __restrict__ xd, uint32_t* __restrict__ xi, 
         si[j] = j < m ? xi[j] : 0xFFFFFFFFu;
     }
     __syncthreads();
+    // each lane takes compare-exchange pairs (j, j + stride) directly, so no
+    // lane idles on the upper element of a pair
     for (int size = 2; size <= P; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int j = threadIdx.x; j < P; j += blockDim.x) {
-                const int p = j ^ stride;
-                if (p > j) {
-                    const bool up = (j & size) == 0;
-                    if (pair_gt(sd[j], si[j], sd[p], si[p]) == up) {
-                        double td = sd[j]; sd[j] = sd[p]; sd[p] = td;
-                        uint32_t ti = si[j]; si[j] = si[p]; si[p] = ti;
-                    }
+            for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
+                const int j = 2 * t - (t & (stride - 1)), p = j + stride;
+                const bool up = (j & size) == 0;
+                const double dj = sd[j], dp = sd[p];
+                const uint32_t ij = si[j], ip = si[p];
+                if (pair_gt(dj, ij, dp, ip) == up) {
+                    sd[j] = dp; sd[p] = dj;
+                    si[j] = ip; si[p] = ij;
                 }
             }
             __syncthreads();
@@ -247,7 +249,7 @@ __device__ __forceinline__ int rank_in(const double* __restrict__ rd, const uint
 // merge scratch; each element finds its output slot as its index in its own
 // run plus its rank in the partner run (binary search; (d64, id) keys are
 // distinct within a segment, so ranks never collide).
-__global__ __launch_bounds__(256) void segsort_big_kernel(const int64_t* __restrict__ off, const int* big_list,
+__global__ __launch_bounds__(512) void segsort_big_kernel(const int64_t* __restrict__ off, const int* big_list,
                                                           const int64_t* big_tmp, const unsigned* big_count,
                                                           uint32_t* ids, double* d2, int64_t capacity,
                                                           uint32_t* tmp_ids, double* tmp_d2) {
@@ -353,7 +355,7 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
             GPRT_TRY(ctx->sort_tmp[0].reserve(8ull * h.total));
             GPRT_TRY(ctx->sort_tmp[1].reserve(4ull * h.total));
         }
-        hipLaunchKernelGGL(segsort_big_kernel, dim3(h.nbig), dim3(256), 0, s, d_off, big_buf.as<int>(), big_tmp,
+        hipLaunchKernelGGL(segsort_big_kernel, dim3(h.nbig), dim3(512), 0, s, d_off, big_buf.as<int>(), big_tmp,
                            big_count, d_ids, d2_buf.as<double>(), capacity, ctx->sort_tmp[1].as<uint32_t>(),
                            ctx->sort_tmp[0].as<double>());
     }
