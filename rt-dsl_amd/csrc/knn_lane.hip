@@ -101,11 +101,10 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                     // four loads without branches (so they issue back to back
                     // whichever lanes scan): past the end of a leaf range the index
                     // is clamped to its last point, which is loaded but not visited
+                    const float4* src = scanning ? pts + (sf + st) : nodes4 + 4 * (size_t)node;
+                    const int last = scanning ? avail - 1 : 3;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const size_t e = scanning ? (size_t)(sf + st + min(u, avail - 1)) : 4 * (size_t)node + u;
-                        v[u] = (scanning ? pts : nodes4)[e];
-                    }
+                    for (int u = 0; u < 4; ++u) v[u] = src[min(u, last)];
                     if (!scanning) {
                         if (++steps > a.max_steps) { overflow = true; break; }
                         if (COUNT) ++visits;
