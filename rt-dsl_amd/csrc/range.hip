@@ -37,23 +37,30 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
     const float tau = band(r * r);
     int64_t cnt = 0;
     int64_t pos = WRITE ? off[g] : 0, end = WRITE ? min(off[g + 1], capacity) : 0;
+    // a leaf range four points at a time: the loads of a group issue together
+    // (indices past the range clamped to its last point, loaded but not
+    // tested), and hits keep the range's order
     auto visit = [&](int f, int m) {
-        for (int t = 0; t < m; ++t) {
-            const float4 p = pts[f + t];
-            if (d2f(q.x, q.y, q.z, p.x, p.y, p.z) <= tau) {
-                const double dd = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
-                if (dd <= r2) {
-                    if (WRITE) {
-                        if (pos < end) {
-                            ids[pos] = (uint32_t)__float_as_int(p.w);
-                            d2[pos] = dd;
+        for (int t0 = 0; t0 < m; t0 += 4) {
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] = pts[f + min(t0 + u, m - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + u < m && d2f(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= tau) {
+                    const double dd = d2d(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z);
+                    if (dd <= r2) {
+                        if (WRITE) {
+                            if (pos < end) {
+                                ids[pos] = (uint32_t)__float_as_int(p[u].w);
+                                d2[pos] = dd;
+                            }
+                            ++pos;
+                        } else {
+                            ++cnt;
                         }
-                        ++pos;
-                    } else {
-                        ++cnt;
                     }
                 }
-            }
         }
     };
     if (root.y > 0) {
@@ -69,8 +76,13 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
                 const float dB = box2f(q.x, q.y, q.z, nd.blo, nd.bhi);
                 const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
                 const bool inA = dA <= tau, inB = dB <= tau;
-                if (rA < 0 && inA) visit(~rA, __float_as_int(nd.ahi.w));
-                if (rB < 0 && inB) visit(~rB, __float_as_int(nd.bhi.w));
+                {
+                    // the two buckets of a node are adjacent ranges of the sorted
+                    // points (A's then B's): in-range ones are scanned as one range
+                    const bool sA = rA < 0 && inA, sB = rB < 0 && inB;
+                    const int mm = (sA ? __float_as_int(nd.ahi.w) : 0) + (sB ? __float_as_int(nd.bhi.w) : 0);
+                    if (mm > 0) visit(sA ? ~rA : ~rB, mm);
+                }
                 const bool gA = rA >= 0 && inA, gB = rB >= 0 && inB;
                 if (gA && gB) {
                     stk[sp++] = rB;
