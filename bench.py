@@ -177,6 +177,37 @@ def _extra_all(ctx, args, oracle):
     return ex
 
 
+def _only(args):
+    """One secondary workload alone, for profiling: the r = 0.5 m range query on
+    the LiDAR data, or Barnes-Hut on the 1M-body Plummer sphere.  Prints one
+    JSON line for that workload (not the headline metric)."""
+    import gprt_hip
+    from gprt_hip import datasets as D
+    ctx = gprt_hip.Context(0)
+    steps = max(2, args.steps)
+    if args.only == "range":
+        pts, q = D.lidar_scan()
+        tree = ctx.build(pts, leaf_size=args.leaf)
+        ms = []
+        for _ in range(steps + 1):
+            off, ids = tree.range_query(q, 0.5)
+            ms.append(ctx.stats()["ms_query"])
+        line = {"workload": "lidar_scan range r=0.5 m, 1,310,720 pts x 400,000 queries", "steps": steps,
+                "ms_device_fill_call": float(np.median(ms[1:])), "results": int(off[-1])}
+    else:
+        pm = D.plummer()
+        dpm = gprt_hip.DeviceArray.from_host(ctx, pm)
+        line = {"workload": "barnes-hut plummer 1,000,000 bodies, eps=1e-3, G=1", "steps": steps}
+        for theta in (0.3, 0.5, 0.7):
+            ms = []
+            for _ in range(steps + 1):
+                ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0).free()
+                ms.append(ctx.stats()["ms_query"])
+            line[f"ms_force_theta_{theta}"] = float(np.median(ms[1:]))
+    print(json.dumps(line))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,11 +217,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip autotune, end-to-end, k=32 and range")
     ap.add_argument("--all", action="store_true", help="also the road, batched-cloud and Barnes-Hut workloads")
+    ap.add_argument("--only", choices=["range", "barnes-hut"],
+                    help="run one secondary workload alone (for profiling) and print its JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the k-NN path would shard queries over a replicated tree)", file=sys.stderr)
         return 2
+    if args.only:
+        return _only(args)
 
     import gprt_hip
     from gprt_hip import datasets as D
