@@ -211,19 +211,68 @@ __global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __res
     }
 }
 
-// bitonic sort of x[0..m) (m <= SEG_LDS) through LDS, by (d64, id)
+// Bitonic sort of x[0..m) (m <= SEG_LDS) by (d64, id) in LDS.  Stages of
+// stride >= 8 take compare-exchange pairs from LDS, one pair per lane at a
+// time.  The strides 4, 2 and 1 of every size run in registers: lane t loads
+// elements [8t, 8t + 8), which hold every pair of those stages, and stores them
+// back once.  That saves LDS traffic and barriers on three stages per size.
+constexpr int SEG_E = 8;
+__device__ __forceinline__ void seg_cx(double& da, uint32_t& ia, double& db, uint32_t& ib, bool up) {
+    if (pair_gt(da, ia, db, ib) == up) {
+        const double td = da; da = db; db = td;
+        const uint32_t ti = ia; ia = ib; ib = ti;
+    }
+}
+// Register stages on lane t's eight elements.  first: sizes 2, 4 and 8 in full
+// (each 8-element chunk sorted in its bitonic direction); otherwise the
+// strides 4, 2 and 1 of `size`.
+__device__ __forceinline__ void seg_small_strides(double* sd, uint32_t* si, int P, int size, bool first) {
+    const int t = threadIdx.x, base = SEG_E * t;
+    if (base >= P) return;
+    double d[SEG_E];
+    uint32_t id[SEG_E];
+#pragma unroll
+    for (int e = 0; e < SEG_E; ++e) {
+        d[e] = sd[base + e];
+        id[e] = si[base + e];
+    }
+    if (first) {
+#pragma unroll
+        for (int sz = 2; sz <= SEG_E; sz <<= 1)
+#pragma unroll
+            for (int stride = sz >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+                for (int e = 0; e < SEG_E; ++e)
+                    if ((e & stride) == 0)
+                        seg_cx(d[e], id[e], d[e + stride], id[e + stride], ((base + e) & sz) == 0);
+    } else {
+        const bool up = (base & size) == 0;  // size >= 16: one direction for the eight
+#pragma unroll
+        for (int stride = SEG_E / 2; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int e = 0; e < SEG_E; ++e)
+                if ((e & stride) == 0) seg_cx(d[e], id[e], d[e + stride], id[e + stride], up);
+    }
+#pragma unroll
+    for (int e = 0; e < SEG_E; ++e) {
+        sd[base + e] = d[e];
+        si[base + e] = id[e];
+    }
+}
+
 __device__ void lds_bitonic(double* __restrict__ xd, uint32_t* __restrict__ xi, int m, double* sd, uint32_t* si) {
-    int P = 1;
+    int P = SEG_E;
     while (P < m) P <<= 1;
     for (int j = threadIdx.x; j < P; j += blockDim.x) {
         sd[j] = j < m ? xd[j] : INFINITY;
         si[j] = j < m ? xi[j] : 0xFFFFFFFFu;
     }
     __syncthreads();
-    // each lane takes compare-exchange pairs (j, j + stride) directly, so no
-    // lane idles on the upper element of a pair
-    for (int size = 2; size <= P; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+    // sizes 2, 4 and 8 entirely in registers
+    seg_small_strides(sd, si, P, SEG_E, true);
+    __syncthreads();
+    for (int size = 2 * SEG_E; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride >= SEG_E; stride >>= 1) {
             for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
                 const int j = 2 * t - (t & (stride - 1)), p = j + stride;
                 const bool up = (j & size) == 0;
@@ -236,6 +285,9 @@ __device__ void lds_bitonic(double* __restrict__ xd, uint32_t* __restrict__ xi, 
             }
             __syncthreads();
         }
+        seg_small_strides(sd, si, P, size, false);
+        __syncthreads();
+    }
     for (int j = threadIdx.x; j < m; j += blockDim.x) {
         xd[j] = sd[j];
         xi[j] = si[j];
