@@ -12,6 +12,7 @@
 // The result is written as a JSON superset of the reference's Schedule /
 // wisdom format (gprt-ir/src/lib.rs:58-89).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -205,7 +206,17 @@ extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_
         return true;
     };
     double d = 0.0;
-    if (num(v.get("radius_increment_mult"), d)) c.radius_mult = (float)d;
+    auto finite_f32 = [&](const char* key) {
+        if (!(std::fabs(d) <= 3.0e38)) {
+            fail(GPRT_ERR_ARG, std::string("schedule: ") + key + " out of range");
+            return false;
+        }
+        return true;
+    };
+    if (num(v.get("radius_increment_mult"), d)) {
+        if (!finite_f32("radius_increment_mult")) return GPRT_ERR_ARG;
+        c.radius_mult = (float)d;
+    }
     if (const JVal* m = v.get("use_morton_lbv")) {
         if (m->kind != JVal::Bool) return fail(GPRT_ERR_ARG, "schedule: use_morton_lbv must be a boolean");
         c.sort_queries = m->b ? 1 : 0;
@@ -214,9 +225,11 @@ extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_
         if (h->kind == JVal::Str && h->str == "SampledMax") {
             c.radius_heuristic = GPRT_RADIUS_MAX;
         } else if (h->kind == JVal::Obj && num(h->get("SampledPercentile"), d)) {
+            if (!finite_f32("SampledPercentile")) return GPRT_ERR_ARG;
             c.radius_heuristic = GPRT_RADIUS_PERCENTILE;
             c.radius_param = (float)d;
         } else if (h->kind == JVal::Obj && num(h->get("Fixed"), d)) {
+            if (!finite_f32("Fixed")) return GPRT_ERR_ARG;
             c.radius_heuristic = GPRT_RADIUS_FIXED;
             c.radius_param = (float)d;
         } else {
@@ -239,8 +252,14 @@ extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_
                       {"compact_rounds", &c.compact_rounds, nullptr}, {"leaf_size", &leaf, nullptr}};
         for (auto& f : fields)
             if (num(g->get(f.key), d)) {
-                if (f.ip) *f.ip = (int)d;
-                else *f.fp = (float)d;
+                if (f.ip) {
+                    if (!(d >= -2147483648.0 && d <= 2147483647.0) || d != (double)(int64_t)d)
+                        return fail(GPRT_ERR_ARG, std::string("schedule: ") + f.key + " must be an integer");
+                    *f.ip = (int)d;
+                } else {
+                    if (!(std::fabs(d) <= 3.0e38)) return fail(GPRT_ERR_ARG, std::string("schedule: ") + f.key + " out of range");
+                    *f.fp = (float)d;
+                }
             }
     }
     *cfg = c;

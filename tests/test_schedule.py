@@ -84,7 +84,9 @@ def test_schedule_extension_object_round_trip():
 
 @pytest.mark.parametrize("bad", [
     "", "[1, 2]", "{", '{"radius_heuristic": "Median"}', '{"radius_heuristic": {"Percentile": 0.5}}',
-    '{"use_morton_lbv": 1}', '{"gprt_hip": 3}',
+    '{"use_morton_lbv": 1}', '{"gprt_hip": 3}', '{"gprt_hip": {"max_rounds": 1e12}}',
+    '{"gprt_hip": {"stack_depth": 2.5}}', '{"radius_increment_mult": 1e300}',
+    '{"radius_heuristic": {"Fixed": -1e39}}',
 ])
 def test_schedule_rejects(bad):
     with pytest.raises(gprt_hip.GprtError) as e:
