@@ -63,7 +63,15 @@ struct JParser {
                     case '"': case '\\': case '/': out += e; break;
                     case 'u': {
                         if (i + 4 > n) return fail_at("bad \\u escape");
-                        unsigned v = (unsigned)strtoul(std::string(s + i, 4).c_str(), nullptr, 16);
+                        unsigned v = 0;
+                        for (int h = 0; h < 4; ++h) {
+                            const char x = s[i + h];
+                            const int dgt = x >= '0' && x <= '9' ? x - '0'
+                                            : x >= 'a' && x <= 'f' ? x - 'a' + 10
+                                            : x >= 'A' && x <= 'F' ? x - 'A' + 10 : -1;
+                            if (dgt < 0) return fail_at("bad \\u escape");
+                            v = v * 16 + (unsigned)dgt;
+                        }
                         i += 4;
                         if (v < 0x80) out += (char)v;
                         else return fail_at("non-ASCII \\u escape");
@@ -128,11 +136,20 @@ struct JParser {
         if (lit("false")) { v.kind = JVal::Bool; v.b = false; return true; }
         if (lit("null")) { v.kind = JVal::Null; return true; }
         if (c == '-' || (c >= '0' && c <= '9')) {
+            // the token is copied out first: the input need not be NUL-terminated
+            size_t j = i;
+            while (j < n && (s[j] == '-' || s[j] == '+' || s[j] == '.' || s[j] == 'e' || s[j] == 'E' ||
+                             (s[j] >= '0' && s[j] <= '9')))
+                ++j;
+            if (j - i > 64) return fail_at("number too long");
+            char tok[72];
+            memcpy(tok, s + i, j - i);
+            tok[j - i] = 0;
             char* end = nullptr;
             v.kind = JVal::Num;
-            v.num = strtod(s + i, &end);
-            if (end == s + i) return fail_at("bad number");
-            i = (size_t)(end - s);
+            v.num = strtod(tok, &end);
+            if (end != tok + (j - i)) return fail_at("bad number");
+            i = j;
             return true;
         }
         return fail_at("unexpected character");

@@ -92,3 +92,25 @@ def test_schedule_rejects(bad):
     with pytest.raises(gprt_hip.GprtError) as e:
         gprt_hip.parse_schedule(bad)
     assert e.value.code == gprt_hip.ERR_ARG
+
+
+def test_schedule_reads_only_the_given_bytes():
+    """The reader stops at the given size: the input need not be NUL-terminated,
+    and bytes after it (here digits) are not part of the number."""
+    import ctypes
+    text = b'{"radius_increment_mult": 2}' + b'5}'
+    cfg = gprt_hip.Config()
+    leaf = ctypes.c_int()
+    L = gprt_hip.lib()
+    assert L.gprt_hip_schedule_from_json(text, len(text) - 2, ctypes.byref(cfg), ctypes.byref(leaf)) == 0
+    assert cfg.radius_mult == 2.0
+    # cut inside the number: the object is unterminated
+    assert L.gprt_hip_schedule_from_json(text, len(text) - 3, ctypes.byref(cfg), ctypes.byref(leaf)) == \
+        gprt_hip.ERR_ARG
+
+
+def test_schedule_rejects_bad_escape():
+    with pytest.raises(gprt_hip.GprtError):
+        gprt_hip.parse_schedule('{"\\uZZZZ": 1}')
+    cfg, _ = gprt_hip.parse_schedule('{"\\u0072adius_increment_mult": 1.5}')  # r is "r"
+    assert cfg.radius_mult == 1.5
