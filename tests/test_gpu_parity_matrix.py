@@ -83,3 +83,26 @@ def test_parity_matrix(ctx, seed, kind, n, k, ci):
     assert len(bad) == 0, f"{kind} n={n} k={k} cfg={CONFIGS[ci]}: {len(bad)} queries differ, first {bad[:3]}"
     assert np.array_equal(t.k_nn_plain(q, k), ref)
     t.close()
+
+
+RANGE_CASES = []
+for _i in range(30):
+    RANGE_CASES.append((_i, _kinds[_i % len(_kinds)], [1, 300, 5_000, 40_000][(_i * 3) % 4],
+                        [0.0, 1e-3, 0.01, 0.05, 0.2][(_i * 7) % 5]))
+
+
+@pytest.mark.parametrize("seed,kind,n,rel", RANGE_CASES)
+def test_range_parity_matrix(ctx, seed, kind, n, rel):
+    """Range queries over the same distributions: every point with
+    d64 <= r*r, in (d64, id) order, CSR offsets equal to the oracle's."""
+    rng = np.random.default_rng(2000 + seed)
+    pts = _cloud(kind, n, rng)
+    q = _queries(pts, 800, rng)
+    span = float(np.ptp(pts, axis=0).max()) if len(pts) > 1 else 1.0
+    r = float(np.float32(rel * max(span, 1e-3)))
+    roff, rids = oracle.range_brute(pts, q, r)
+    t = ctx.build(pts, leaf_size=(4, 8, 16)[seed % 3])
+    off, ids = t.range_query(q, r)
+    assert np.array_equal(off, roff), f"{kind} n={n} r={r}: counts differ"
+    assert np.array_equal(ids, rids), f"{kind} n={n} r={r}: ids or order differ"
+    t.close()
