@@ -210,8 +210,14 @@ extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
         return fail(GPRT_ERR_HIP, "stream/event creation failed");
     }
     if (c.counters.reserve(sizeof(unsigned) * CNT_ALL) != GPRT_OK) {
-        delete h;
+        gprt_hip_ctx_destroy(h);
         return GPRT_ERR_ALLOC;
+    }
+    if (hipHostMalloc((void**)&c.hstage, sizeof(unsigned) * (CNT_ALL + 1), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c.hstage = nullptr;
+        gprt_hip_ctx_destroy(h);
+        return fail(GPRT_ERR_ALLOC, "pinned host allocation failed");
     }
     *out = h;
     return GPRT_OK;
@@ -227,6 +233,7 @@ extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
     for (auto& b : c.ws) b.release();
     for (auto& b : c.sort_tmp) b.release();
     c.counters.release();
+    if (c.hstage) (void)hipHostFree(c.hstage);
     drop_extra(&c);
     if (c.side) (void)hipStreamSynchronize(c.side);
     for (hipEvent_t e : {c.ev0, c.ev1, c.ev2, c.ev3, c.ev_fork, c.ev_join})
@@ -616,8 +623,15 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
 
 // d_radius: the sampled radius of cloud 0 on the device (batched k-NN), else null
 static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_radius = nullptr) {
+    // the counters and the radius come back by copies queued behind the work,
+    // so the call waits for the GPU once
+    if (read_counters)
+        GPRT_HIP_TRY(hipMemcpyAsync(c.hstage, c.counters.ptr, sizeof(unsigned) * CNT_ALL, hipMemcpyDeviceToHost,
+                                    c.stream));
+    if (d_radius)
+        GPRT_HIP_TRY(hipMemcpyAsync(c.hstage + CNT_ALL, d_radius, sizeof(float), hipMemcpyDeviceToHost, c.stream));
     GPRT_HIP_TRY(hipStreamSynchronize(c.stream));
-    if (d_radius) GPRT_HIP_TRY(hipMemcpy(&c.stats.initial_radius, d_radius, sizeof(float), hipMemcpyDeviceToHost));
+    if (d_radius) memcpy(&c.stats.initial_radius, c.hstage + CNT_ALL, sizeof(float));
     float ms = 0;
     if (hipEventElapsedTime(&ms, c.ev0, c.ev1) != hipSuccess) {
         (void)hipGetLastError();
@@ -627,8 +641,7 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_r
     c.stats.ms_total = ms;
     c.stats.queries = nq;
     if (read_counters) {
-        unsigned h[CNT_TOTAL + 2 * CNT_STATS64];
-        GPRT_HIP_TRY(hipMemcpy(h, c.counters.ptr, sizeof(h), hipMemcpyDeviceToHost));
+        const unsigned* h = c.hstage;
         unsigned long long v64[CNT_STATS64];
         memcpy(v64, h + CNT_TOTAL, sizeof(v64));
         c.stats.rerank_reads = (int64_t)v64[2];

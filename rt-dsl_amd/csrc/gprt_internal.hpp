@@ -197,6 +197,9 @@ struct Ctx {
     hipGraph_t graph = nullptr;
     uint64_t graph_key = 0;
     bool kernel_timed = false;
+    // pinned host copy of the counter buffer plus one float (the sampled
+    // initial radius), filled by copies enqueued behind each call's work
+    unsigned* hstage = nullptr;
 };
 
 }  // namespace gprt
