@@ -30,6 +30,36 @@ namespace {
 
 bool is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
 
+// Bytes of the White_Space character (Rust's char::is_whitespace) that starts
+// at b, or that ends at e; 0 if there is none.  The text is valid UTF-8.
+// White_Space: U+0009-000D, U+0020, U+0085, U+00A0, U+1680, U+2000-200A,
+// U+2028, U+2029, U+202F, U+205F, U+3000.
+int ws3(const unsigned char* p) {
+    if (p[0] == 0xE1) return p[1] == 0x9A && p[2] == 0x80;
+    if (p[0] == 0xE2 && p[1] == 0x80) return p[2] <= 0x8A || p[2] == 0xA8 || p[2] == 0xA9 || p[2] == 0xAF;
+    if (p[0] == 0xE2) return p[1] == 0x81 && p[2] == 0x9F;
+    return p[0] == 0xE3 && p[1] == 0x80 && p[2] == 0x80;
+}
+int ws_front(const char* b, const char* e) {
+    const unsigned char* p = (const unsigned char*)b;
+    if (b >= e) return 0;
+    if (p[0] < 0x80) return is_space(b[0]) ? 1 : 0;
+    if (e - b >= 2 && p[0] == 0xC2 && (p[1] == 0x85 || p[1] == 0xA0)) return 2;
+    return (e - b >= 3 && ws3(p)) ? 3 : 0;
+}
+int ws_back(const char* b, const char* e) {
+    if (b >= e) return 0;
+    const unsigned char* q = (const unsigned char*)e;
+    if (q[-1] < 0x80) return is_space(e[-1]) ? 1 : 0;
+    if (e - b >= 2 && q[-2] == 0xC2 && (q[-1] == 0x85 || q[-1] == 0xA0)) return 2;
+    return (e - b >= 3 && ws3(q - 3)) ? 3 : 0;
+}
+// str::trim: [b, e) without leading and trailing White_Space
+void trim(const char*& b, const char*& e) {
+    for (int n; (n = ws_front(b, e)) > 0;) b += n;
+    for (int n; (n = ws_back(b, e)) > 0;) e -= n;
+}
+
 // true if [b, e) is well-formed UTF-8
 bool utf8_ok(const unsigned char* b, const unsigned char* e) {
     while (b < e) {
@@ -53,13 +83,13 @@ bool utf8_ok(const unsigned char* b, const unsigned char* e) {
     return true;
 }
 
-// One trimmed field as a binary32; false if it is not a number in the forms
-// the file format allows (no hex floats, no "nan(...)" payloads).
+// One field, trimmed as str::trim does, as a binary32; false if it is not a
+// number in the forms Rust's f32 parser takes (no hex floats, no "nan(...)"
+// payloads, any length).
 bool parse_f32(const char* b, const char* e, float& out) {
-    while (b < e && is_space(*b)) ++b;
-    while (e > b && is_space(e[-1])) --e;
+    trim(b, e);
     const size_t n = (size_t)(e - b);
-    if (n == 0 || n > 4096) return false;
+    if (n == 0) return false;
     char small[80];
     std::string big;
     char* buf = small;
@@ -99,9 +129,10 @@ void parse_slice(const char* b, const char* e, int format, Slice& out) {
             continue;
         }
         if (format == GPRT_CSV_BODIES) {
-            const char* t = line;
-            while (t < le && is_space(*t)) ++t;
-            if (t == le) continue;
+            const char* tb = line;
+            const char* te = le;
+            trim(tb, te);
+            if (tb == te) continue;
         }
         // the first `width` fields
         const char* f[5];

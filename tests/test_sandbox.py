@@ -39,6 +39,23 @@ def test_read_points_rules(tmp_path):
     assert np.signbit(pts[3, 2])
 
 
+def test_read_points_unicode_trim_and_long_fields(tmp_path):
+    """Fields are trimmed as Rust's str::trim does (Unicode White_Space, not
+    only ASCII), and a number may be written with any number of digits."""
+    nbsp, em, ideo, nel = "\u00a0", "\u2003", "\u3000", "\u0085"
+    zeros = "0" * 5000
+    text = (f"{nbsp}1,{em}2{ideo},3{nel}\n"
+            f"\u2028{ideo}\n"                      # only White_Space: one empty field, dropped
+            f"4,5\u200b,6\n"                       # U+200B is not White_Space: dropped
+            f"0.{zeros}1,1{zeros}e-5000,-{zeros}7\n"
+            f"{em}8,\u1680 9 \u205f,10\u202f\n")
+    pts = gprt_hip.read_points(_write(tmp_path, "u.csv", text))
+    want = np.array([[1, 2, 3], [np.float32("1e-5001"), 1, -7], [8, 9, 10]], np.float32)
+    np.testing.assert_array_equal(pts, want)
+    b = gprt_hip.read_points(_write(tmp_path, "u.txt", f"{ideo}{nbsp}\n1,2,3,{em}4{nbsp}\n"), gprt_hip.CSV_BODIES)
+    np.testing.assert_array_equal(b, [[1, 2, 3, 4]])
+
+
 def test_read_points_not_utf8(tmp_path):
     path = _write(tmp_path, "p.csv", b"1,2,3\n4,5,6,\xff\xfe\n7,8,9\n")
     np.testing.assert_array_equal(gprt_hip.read_points(path), [[1, 2, 3], [7, 8, 9]])
