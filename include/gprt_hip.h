@@ -28,9 +28,14 @@
  * device pointer on the context's device (inputs already resident in HBM);
  * otherwise they are host pointers and the call copies through.
  *
- * Threading: a context owns one HIP stream; calls on one context must not
- * overlap (guard it with a mutex, as the Rust wrapper does for its caches,
- * gprt-optix/src/lib.rs:35-40).  Different contexts may run concurrently.
+ * Threading: a context owns one HIP stream and a mutex.  Every call that
+ * uses a context, directly, through one of its trees or through a legacy
+ * pipeline, holds that mutex until it returns, so calls on one context from
+ * several threads are safe and run one after another (the Rust wrapper keeps
+ * its caches behind a Mutex for the same reason, gprt-optix/src/lib.rs:35-40).
+ * Calls on different contexts run concurrently.  Destroying a context or a
+ * tree while another thread still uses it is an error.  Error messages are
+ * per thread.
  */
 #ifndef GPRT_HIP_H
 #define GPRT_HIP_H

@@ -246,12 +246,14 @@ extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
 
 extern "C" int gprt_hip_ctx_sync(gprt_hip_ctx* h) {
     if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    GPRT_CTX_LOCK(h);
     GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
     return GPRT_OK;
 }
 
 extern "C" int gprt_hip_device_alloc(gprt_hip_ctx* h, size_t bytes, void** out) {
     if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
+    GPRT_CTX_LOCK(h);
     *out = nullptr;
     GPRT_HIP_TRY(hipSetDevice(h->c.device));
     if (hipMalloc(out, std::max<size_t>(bytes, 1)) != hipSuccess) {
@@ -264,6 +266,7 @@ extern "C" int gprt_hip_device_alloc(gprt_hip_ctx* h, size_t bytes, void** out) 
 
 extern "C" int gprt_hip_device_free(gprt_hip_ctx* h, void* p) {
     if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    GPRT_CTX_LOCK(h);
     if (!p) return GPRT_OK;
     GPRT_HIP_TRY(hipSetDevice(h->c.device));
     GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
@@ -274,6 +277,7 @@ extern "C" int gprt_hip_device_free(gprt_hip_ctx* h, void* p) {
 extern "C" int gprt_hip_copy(gprt_hip_ctx* h, void* dst, const void* src, size_t bytes, int kind) {
     if (!h || (bytes && (!dst || !src))) return fail(GPRT_ERR_ARG, "null argument");
     if (kind < 0 || kind > 2) return fail(GPRT_ERR_ARG, "bad copy kind");
+    GPRT_CTX_LOCK(h);
     if (!bytes) return GPRT_OK;
     GPRT_HIP_TRY(hipSetDevice(h->c.device));
     hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
@@ -289,6 +293,7 @@ static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, i
                         int flags, gprt_hip_tree** out) {
     if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
     *out = nullptr;
+    GPRT_CTX_LOCK(h);
     if (nclouds <= 0) return fail(GPRT_ERR_ARG, "nclouds must be positive");
     if (leaf_size < 1 || leaf_size > 32) return fail(GPRT_ERR_ARG, "leaf_size must be in [1, 32]");
     for (int c = 0; c < nclouds; ++c)
@@ -343,7 +348,11 @@ extern "C" int gprt_hip_tree_build_batched(gprt_hip_ctx* h, const float* pts, co
 
 extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
     if (!t) return GPRT_OK;
-    if (t->owner) (void)hipSetDevice(t->owner->c.device);
+    std::unique_lock<std::recursive_mutex> lk;
+    if (t->owner) {
+        lk = std::unique_lock<std::recursive_mutex>(t->owner->mu);
+        (void)hipSetDevice(t->owner->c.device);
+    }
     Tree& T = t->t;
     for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.inv,
                       &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
@@ -366,6 +375,7 @@ extern "C" int gprt_hip_tree_info(const gprt_hip_tree* t, gprt_hip_tree_desc* ou
 extern "C" int gprt_hip_tree_export(gprt_hip_tree* t, uint64_t* keys, uint32_t* perm, float* nodes,
                                     int32_t* roots) {
     if (!t || !t->owner) return fail(GPRT_ERR_ARG, "null tree");
+    GPRT_CTX_LOCK(t->owner);
     Ctx& c = t->owner->c;
     Tree& T = t->t;
     hipStream_t s = c.stream;
@@ -677,6 +687,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
     if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "tree has several clouds: use gprt_hip_knn_batched");
+    GPRT_CTX_LOCK(h);
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));
     gprt_hip_config cf;
@@ -730,6 +741,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
 extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
                                   uint32_t* ids_out, double* d2_out, int flags) {
     if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
+    GPRT_CTX_LOCK(h);
     if (k < 1 || k > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
@@ -786,6 +798,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
                                     const int64_t* query_offsets, const int* k_per_cloud,
                                     const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int flags) {
     if (!h || !t || !query_offsets || !k_per_cloud) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
     const Tree& T = t->t;
     int nc = T.nclouds;
     if (query_offsets[0] != 0) return fail(GPRT_ERR_ARG, "query_offsets[0] must be 0");
@@ -856,6 +869,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
 
 extern "C" int gprt_hip_stats(gprt_hip_ctx* h, gprt_hip_run_stats* out) {
     if (!h || !out) return fail(GPRT_ERR_ARG, "null ctx/out");
+    GPRT_CTX_LOCK(h);
     *out = h->c.stats;
     return GPRT_OK;
 }
@@ -866,6 +880,7 @@ extern "C" int gprt_hip_stats(gprt_hip_ctx* h, gprt_hip_run_stats* out) {
 extern "C" int gprt_hip_morton_codes(gprt_hip_ctx* h, const float* pts, int64_t n, uint32_t* codes_out,
                                      float* bbox_out, int flags) {
     if (!h || (n > 0 && (!pts || !codes_out))) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
     if (n < 0 || n > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad n");
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));
@@ -894,6 +909,7 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
                                       int64_t nq, int sampler, int heuristic, float param, int k,
                                       float* radius_out, int flags) {
     if (!h || !radius_out) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
     if (n < 0 || nq < 0 || (n > 0 && !pts) || (nq > 0 && !queries)) return fail(GPRT_ERR_ARG, "bad arrays");
     if (sampler != GPRT_SAMPLER_LCG && sampler != GPRT_SAMPLER_STRIDED) return fail(GPRT_ERR_ARG, "bad sampler");
     if (heuristic < 0 || heuristic > 3) return fail(GPRT_ERR_ARG, "bad heuristic");
@@ -949,6 +965,7 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
 extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* vals, int64_t n, int begin_bit,
                                    int end_bit, int flags) {
     if (!h || (n > 0 && (!keys || !vals))) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
     if (n < 0 || n > 0x7FFFFFF0 || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
         return fail(GPRT_ERR_ARG, "bad size/bit range");
     Ctx& c = h->c;
@@ -984,6 +1001,7 @@ extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* va
 extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, float radius,
                               int64_t* offsets_out, uint32_t* ids_out, int64_t ids_capacity, int flags) {
     if (!h || !t || !offsets_out) return fail(GPRT_ERR_ARG, "null ctx/tree/offsets_out");
+    GPRT_CTX_LOCK(h);
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && !queries) return fail(GPRT_ERR_ARG, "null queries");
     if (!(radius >= 0.0f && radius <= 1e18f)) return fail(GPRT_ERR_ARG, "radius must be finite and >= 0");
@@ -1056,6 +1074,7 @@ extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* qu
 extern "C" int gprt_hip_bh_forces(gprt_hip_ctx* h, const float* pos_mass, int64_t n, float theta, float eps, float G,
                                   int leaf_size, float* acc_out, int flags) {
     if (!h) return fail(GPRT_ERR_ARG, "null ctx");
+    GPRT_CTX_LOCK(h);
     if (n < 0 || n > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad n");
     if (n > 0 && (!pos_mass || !acc_out)) return fail(GPRT_ERR_ARG, "null pos_mass/acc_out");
     if (!(theta >= 0.0f && theta <= 1e6f) || !(eps >= 0.0f && eps <= 1e18f) || !(fabsf(G) <= 1e30f))

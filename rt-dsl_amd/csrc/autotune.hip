@@ -70,6 +70,7 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
                                  int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                  const char* json_path, int flags) {
     if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
     if (n <= 0 || nq <= 0 || k < 1 || k > 64) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 64");
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));

@@ -10,6 +10,7 @@
 // for every point p inside B: pruning a box whose value exceeds a threshold
 // never drops a point whose own value is within it.
 #pragma once
+#include <mutex>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
@@ -204,11 +205,16 @@ struct Ctx {
 
 }  // namespace gprt
 
-// the opaque handles of the C ABI
+// the opaque handles of the C ABI; every call that uses a context (directly,
+// through one of its trees, or through a legacy pipeline) holds its mutex for
+// the whole call, so calls on one context from several threads run one at a
+// time; recursive because some entry points call others
 struct gprt_hip_ctx {
     gprt::Ctx c;
+    std::recursive_mutex mu;
 };
 struct gprt_hip_tree {
     gprt::Tree t;
     gprt_hip_ctx* owner = nullptr;
 };
+#define GPRT_CTX_LOCK(h) std::lock_guard<std::recursive_mutex> gprt_ctx_lock_((h)->mu)

@@ -333,6 +333,7 @@ extern "C" CGprtBvh* gprt_bvh_build(CGprtPipeline* p, const float* h_geom, const
         fail(GPRT_ERR_ARG, "gprt_bvh_build: bad pipeline or count");
         return nullptr;
     }
+    GPRT_CTX_LOCK(p->ctx);
     std::vector<float> xyz(3 * (size_t)count);
     float radius = 0.0f;
     if (h_geom && geom_bytes >= 16 * count) {
@@ -359,8 +360,9 @@ extern "C" CGprtBvh* gprt_bvh_build_triangles(CGprtPipeline* p, const float* h_v
 }
 
 extern "C" void gprt_bvh_refit(CGprtPipeline* p, CGprtBvh* b, float radius, int count) {
-    (void)p;
     if (!b) return;
+    std::unique_lock<std::recursive_mutex> lk;
+    if (p) lk = std::unique_lock<std::recursive_mutex>(p->ctx->mu);
     // the LBVH indexes point centres; a new search radius needs no refit
     b->radius = radius;
     (void)count;
@@ -377,6 +379,7 @@ extern "C" void gprt_register_array(CGprtPipeline* p, const char* name, unsigned
         fail(GPRT_ERR_ARG, "gprt_register_array: bad argument");
         return;
     }
+    GPRT_CTX_LOCK(p->ctx);
     (void)hipSetDevice(p->ctx->c.device);
     if (!p->arrays.count(name)) p->order.push_back(name);
     LegacyArray& a = p->arrays[name];
@@ -390,6 +393,7 @@ extern "C" void gprt_register_array(CGprtPipeline* p, const char* name, unsigned
 
 extern "C" void gprt_register_value(CGprtPipeline* p, const char* name) {
     if (!p || !name) return;
+    GPRT_CTX_LOCK(p->ctx);
     DevBuf& v = p->values[name];
     if (v.reserve(4) == GPRT_OK) (void)hipMemset(v.ptr, 0, 4);
 }
@@ -399,6 +403,7 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
         fail(GPRT_ERR_ARG, "gprt_execute: bad argument");
         return;
     }
+    GPRT_CTX_LOCK(p->ctx);
     if (p->program == PROG_AUTOROPE) {
         fail(GPRT_ERR_STATE, "gprt_execute: the autorope program runs through gprt_execute_autorope_soa_ffi");
         return;
@@ -512,6 +517,7 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
 extern "C" void gprt_retrieve_array_lengths(CGprtPipeline* p, const char* name, unsigned int* h_lengths,
                                             int num_queries) {
     if (!p || !name || !h_lengths) return;
+    GPRT_CTX_LOCK(p->ctx);
     auto it = p->arrays.find(name);
     if (it == p->arrays.end()) {
         fail(GPRT_ERR_ARG, std::string("unknown array ") + name);
@@ -523,6 +529,7 @@ extern "C" void gprt_retrieve_array_lengths(CGprtPipeline* p, const char* name, 
 
 extern "C" void gprt_retrieve_array_flat(CGprtPipeline* p, const char* name, unsigned int* h_out, int total) {
     if (!p || !name || !h_out) return;
+    GPRT_CTX_LOCK(p->ctx);
     auto it = p->arrays.find(name);
     if (it == p->arrays.end()) {
         fail(GPRT_ERR_ARG, std::string("unknown array ") + name);
@@ -534,6 +541,7 @@ extern "C" void gprt_retrieve_array_flat(CGprtPipeline* p, const char* name, uns
 
 extern "C" void gprt_retrieve_value(CGprtPipeline* p, const char* name, unsigned int* h_out) {
     if (!p || !name || !h_out) return;
+    GPRT_CTX_LOCK(p->ctx);
     auto it = p->values.find(name);
     if (it == p->values.end()) {
         fail(GPRT_ERR_ARG, std::string("unknown value ") + name);
@@ -548,6 +556,7 @@ extern "C" void gprt_execute_autorope_soa_ffi(CGprtPipeline* p, const void* h_bo
         fail(GPRT_ERR_ARG, "gprt_execute_autorope_soa_ffi: bad argument");
         return;
     }
+    GPRT_CTX_LOCK(p->ctx);
     Ctx& c = p->ctx->c;
     (void)hipSetDevice(c.device);
     hipStream_t s = c.stream;
