@@ -69,6 +69,10 @@ extern "C" {
  * per axis, gprt-core/src/morton.rs:4-48) instead of the default 21 bits per
  * axis.  Results are the same either way; only the tree shape changes. */
 #define GPRT_KEYS_REF30 8
+/* gprt_hip_bh_forces: the reference shader's force law
+ * G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over pairs with |d|^2 > 1e-10
+ * (gprt-macros/src/lib.rs:785; there eps = 1e-3) instead of Plummer softening */
+#define GPRT_BH_REFERENCE_LAW 16
 
 #define GPRT_ID_NONE 0xFFFFFFFFu
 
@@ -229,7 +233,9 @@ GPRT_HIP_API int gprt_hip_range(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const fl
  * with the Barnes-Hut approximation over an LBVH of the bodies: a node of
  * size s (largest box edge) at distance d from x_i to its centre of mass is
  * taken whole when s < theta * d.  FP32 traversal, compensated (Neumaier)
- * accumulation.  pos_mass: float4 (x, y, z, m); acc_out: 3 floats per body.
+ * accumulation.  Force law: Plummer, G m_j d / (|d|^2 + eps^2)^1.5, or with
+ * GPRT_BH_REFERENCE_LAW in ptr_flags the reference's; at theta = 0 the latter
+ * is what barnes_hut! returns as shipped (every node opened) divided by m_i.  pos_mass: float4 (x, y, z, m); acc_out: 3 floats per body.
  * leaf_size: bodies per leaf bucket (1..32).  A bucket that is opened is
  * summed body by body, so larger buckets trade time for accuracy: on a
  * 1M-body Plummer sphere at theta 0.5, leaf 8 gives a median relative error

@@ -371,8 +371,7 @@ EXPORT float oracle_sampler_strided(const float *pts, int64_t n, const float *qs
 /* pos_mass: float4 (x, y, z, m) per body.  For each sampled body i:
  * acc[i] = sum_{j} G m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^{3/2}
  * (the j == i term is zero).  This is the Plummer-softened force law of the
- * Barnes-Hut workload; the reference's shader uses G*m_i*m_j/(d^2 sqrt(d^2+1e-6))
- * (gprt-macros/src/lib.rs:785) and is deliberately not followed. */
+ * Barnes-Hut workload; the reference's own law is oracle_bh_direct_ref. */
 EXPORT int oracle_bh_direct(const float *pos_mass, int64_t n, const int64_t *sample, int64_t ns, double G,
                             double eps, double *acc_out, int nthreads) {
     double e2 = eps * eps;
@@ -391,6 +390,42 @@ EXPORT int oracle_bh_direct(const float *pos_mass, int64_t n, const int64_t *sam
             double r2 = (dx * dx + dy * dy) + dz * dz + e2;
             double inv = 1.0 / (r2 * sqrt(r2));
             double w = (double)pos_mass[4 * j + 3] * inv;
+            ax += w * dx;
+            ay += w * dy;
+            az += w * dz;
+        }
+        acc_out[3 * s] = G * ax;
+        acc_out[3 * s + 1] = G * ay;
+        acc_out[3 * s + 2] = G * az;
+    }
+    return 0;
+}
+
+/* The reference's force law (gprt-macros/src/lib.rs:785, 790, 799), FP64: for
+ * each sampled body i,
+ * acc[i] = sum_{j : d2 > 1e-10} G m_j (x_j - x_i) / (d2 sqrt(d2 + eps^2)),
+ * d2 = |x_j - x_i|^2 (the shader divides G m_i m_j by d2 sqrt(d2 + 1e-6), i.e.
+ * eps = 1e-3, and skips pairs with d2 <= 1e-10, the body itself included).
+ * As shipped, barnes_hut! opens every node (a null traversable), so its output
+ * is this sum in FP32 times m_i. */
+EXPORT int oracle_bh_direct_ref(const float *pos_mass, int64_t n, const int64_t *sample, int64_t ns, double G,
+                                double eps, double *acc_out, int nthreads) {
+    double e2 = eps * eps;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t s = 0; s < ns; ++s) {
+        int64_t i = sample ? sample[s] : s;
+        double xi = pos_mass[4 * i], yi = pos_mass[4 * i + 1], zi = pos_mass[4 * i + 2];
+        double ax = 0, ay = 0, az = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            double dx = (double)pos_mass[4 * j] - xi;
+            double dy = (double)pos_mass[4 * j + 1] - yi;
+            double dz = (double)pos_mass[4 * j + 2] - zi;
+            double d2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(d2 > 1e-10)) continue;
+            double w = (double)pos_mass[4 * j + 3] / (d2 * sqrt(d2 + e2));
             ax += w * dx;
             ay += w * dy;
             az += w * dz;

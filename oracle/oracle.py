@@ -51,6 +51,7 @@ def lib():
         L.oracle_sampler_strided.restype = ctypes.c_float
         L.oracle_bh_direct.argtypes = [_f32p, _i64, ctypes.c_void_p, _i64, ctypes.c_double, ctypes.c_double,
                                        _f64p, _int]
+        L.oracle_bh_direct_ref.argtypes = L.oracle_bh_direct.argtypes
         L.oracle_kdtree_knn.argtypes = [_f32p, _i64, _f32p, _i64, _int, _u32p, ctypes.c_void_p, _int,
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_num_threads.restype = _int
@@ -141,8 +142,13 @@ def sampler_strided(pts, qs, k, heuristic=HEUR_PERCENTILE, param=0.10):
     return float(np.float32(lib().oracle_sampler_strided(pts, len(pts), qs, len(qs), k, heuristic, param)))
 
 
-def bh_direct(pos_mass, sample=None, G=1.0, eps=1e-3, nthreads=0):
-    """FP64 direct-sum accelerations for the sampled bodies (all when None)."""
+def bh_direct(pos_mass, sample=None, G=1.0, eps=1e-3, nthreads=0, law="plummer"):
+    """FP64 direct-sum accelerations for the sampled bodies (all when None).
+    law "plummer": G m_j d / (|d|^2 + eps^2)^1.5; law "reference": the
+    reference shader's G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over pairs with
+    |d|^2 > 1e-10 (gprt-macros/src/lib.rs:785, 790, 799)."""
+    if law not in ("plummer", "reference"):
+        raise ValueError("law is 'plummer' or 'reference'")
     pm = np.ascontiguousarray(pos_mass, np.float32)
     assert pm.ndim == 2 and pm.shape[1] == 4
     if sample is None:
@@ -151,7 +157,8 @@ def bh_direct(pos_mass, sample=None, G=1.0, eps=1e-3, nthreads=0):
         sample = np.ascontiguousarray(sample, np.int64)
         ns, sp = len(sample), sample.ctypes.data
     acc = np.empty((ns, 3), np.float64)
-    lib().oracle_bh_direct(pm, len(pm), sp, ns, G, eps, acc, nthreads)
+    fn = lib().oracle_bh_direct if law == "plummer" else lib().oracle_bh_direct_ref
+    fn(pm, len(pm), sp, ns, G, eps, acc, nthreads)
     return acc
 
 

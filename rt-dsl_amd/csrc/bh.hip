@@ -7,8 +7,12 @@
 // the GPU LBVH; each internal node additionally stores, per child, the child's
 // mass and centre of mass (a second bottom-up pass with the chunked refit).
 //
-// Force law (the workload's Plummer softening, G and eps as parameters):
+// Force law, Plummer softening (the workload's; G and eps as parameters):
 //   a_i = sum_j G m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^{3/2}
+// or, with GPRT_BH_REFERENCE_LAW, the reference shader's law
+// (gprt-macros/src/lib.rs:785, 790, 799; there eps^2 = 1e-6):
+//   a_i = sum_{j : d2 > 1e-10} G m_j (x_j - x_i) / (d2 sqrt(d2 + eps^2)),
+//   d2 = |x_j - x_i|^2, applied alike to bodies and to centres of mass.
 // Opening rule for a child of size s (largest box edge) whose centre of mass is
 // at distance d from x_i: take it whole when s < theta * d and x_i lies outside
 // the child's box; otherwise open it (leaf buckets are summed body by body).
@@ -103,21 +107,29 @@ struct KSum {
     __device__ __forceinline__ float value() const { return s + c; }
 };
 
+template <bool REFLAW>
 struct BHAcc {
     KSum ax, ay, az;
     float x, y, z, eps2, G;
     __device__ __forceinline__ void add_mass(float mx, float my, float mz, float m) {
         const float dx = mx - x, dy = my - y, dz = mz - z;
-        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-        const float inv = 1.0f / sqrtf(r2);
-        const float w = G * m * inv * inv * inv;
+        float w;
+        if (REFLAW) {
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            if (!(d2 > 1e-10f)) return;
+            w = G * m / (d2 * sqrtf(d2 + eps2));
+        } else {
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+            const float inv = 1.0f / sqrtf(r2);
+            w = G * m * inv * inv * inv;
+        }
         ax.add(w * dx);
         ay.add(w * dy);
         az.add(w * dz);
     }
 };
 
-template <bool COUNT>
+template <bool COUNT, bool REFLAW>
 __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ nodes, const BHNode* __restrict__ bh,
                                                        const float4* __restrict__ pts, const float* __restrict__ mass,
                                                        const uint32_t* __restrict__ perm, int n, int2 root,
@@ -127,7 +139,7 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 me = pts[i];
-    BHAcc A;
+    BHAcc<REFLAW> A;
     A.x = me.x; A.y = me.y; A.z = me.z;
     A.eps2 = eps * eps;
     A.G = G;
@@ -191,7 +203,7 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
 }
 
 int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps, float G, float* d_acc,
-           DevBuf& mass_buf, bool count) {
+           DevBuf& mass_buf, bool count, bool ref_law) {
     hipStream_t s = ctx->stream;
     unsigned* counters = ctx->counters.as<unsigned>();
     GPRT_TRY(T.bh.reserve(sizeof(BHNode) * std::max(n - 1, 1)));
@@ -212,16 +224,19 @@ int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps,
     GPRT_HIP_TRY(hipStreamSynchronize(s));
     const float theta2 = theta * theta;
     GPRT_HIP_TRY(hipEventRecord(ctx->ev0, s));
-    if (count)
-        hipLaunchKernelGGL(bh_force_kernel<true>, dim3((n + 255) / 256), dim3(256), 0, s, T.nodes.as<Node>(),
-                           T.bh.as<BHNode>(), T.pts4.as<float4>(), mass_buf.as<float>(), T.perm.as<uint32_t>(), n,
-                           root, theta2, eps, G, d_acc, T.n + 64, counters + CNT_ERROR,
-                           (unsigned long long*)(counters + CNT_TOTAL));
-    else
-        hipLaunchKernelGGL(bh_force_kernel<false>, dim3((n + 255) / 256), dim3(256), 0, s, T.nodes.as<Node>(),
-                           T.bh.as<BHNode>(), T.pts4.as<float4>(), mass_buf.as<float>(), T.perm.as<uint32_t>(), n,
-                           root, theta2, eps, G, d_acc, T.n + 64, counters + CNT_ERROR,
-                           (unsigned long long*)(counters + CNT_TOTAL));
+#define GPRT_BH_LAUNCH(CNT, REF)                                                                               \
+    hipLaunchKernelGGL((bh_force_kernel<CNT, REF>), dim3((n + 255) / 256), dim3(256), 0, s, T.nodes.as<Node>(), \
+                       T.bh.as<BHNode>(), T.pts4.as<float4>(), mass_buf.as<float>(), T.perm.as<uint32_t>(), n,   \
+                       root, theta2, eps, G, d_acc, T.n + 64, counters + CNT_ERROR,                              \
+                       (unsigned long long*)(counters + CNT_TOTAL))
+    if (count) {
+        if (ref_law) GPRT_BH_LAUNCH(true, true);
+        else GPRT_BH_LAUNCH(true, false);
+    } else {
+        if (ref_law) GPRT_BH_LAUNCH(false, true);
+        else GPRT_BH_LAUNCH(false, false);
+    }
+#undef GPRT_BH_LAUNCH
     GPRT_HIP_TRY(hipEventRecord(ctx->ev1, s));
     GPRT_HIP_TRY(hipGetLastError());
     return GPRT_OK;

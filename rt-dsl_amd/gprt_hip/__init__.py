@@ -21,7 +21,7 @@ LIB_PATH = os.environ.get("GPRT_HIP_LIBRARY", LIB_PATH)
 
 OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
-PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30 = 0, 1, 2, 4, 8
+PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30, BH_REFERENCE_LAW = 0, 1, 2, 4, 8, 16
 ID_NONE = 0xFFFFFFFF
 RADIUS_MAX, RADIUS_PERCENTILE, RADIUS_FIXED, RADIUS_INFINITE = 0, 1, 2, 3
 SAMPLER_LCG, SAMPLER_STRIDED = 0, 1
@@ -327,15 +327,20 @@ class Context:
         return keys, vals
 
     # ---- Barnes-Hut ----
-    def barnes_hut(self, pos_mass, theta=0.5, eps=1e-3, G=1.0, leaf_size=8, out=None):
+    def barnes_hut(self, pos_mass, theta=0.5, eps=1e-3, G=1.0, leaf_size=8, out=None, law="plummer"):
+        """Accelerations (n, 3) of float4 bodies (x, y, z, m).  law "plummer":
+        G m_j d / (|d|^2 + eps^2)^1.5; law "reference": the reference shader's
+        G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over pairs with |d|^2 > 1e-10."""
+        if law not in ("plummer", "reference"):
+            raise ValueError("law is 'plummer' or 'reference'")
         pm = _points(pos_mass, 4)
         p, dev = _ptr(pm)
         n = len(pm)
         if out is None:
             out = DeviceArray(self, (n, 3), np.float32) if dev else np.empty((n, 3), np.float32)
         o, _ = _ptr(out)
-        _check(lib().gprt_hip_bh_forces(self.h, p, n, theta, eps, G, leaf_size, o,
-                                        PTR_DEVICE if dev else PTR_HOST))
+        flags = (PTR_DEVICE if dev else PTR_HOST) | (BH_REFERENCE_LAW if law == "reference" else 0)
+        _check(lib().gprt_hip_bh_forces(self.h, p, n, theta, eps, G, leaf_size, o, flags))
         return out
 
     def autotune(self, pts, queries, k, budget_trials=24, json_path=None):
@@ -544,6 +549,7 @@ def range_query(ctx, data, queries, radius, leaf_size=8):
         t.close()
 
 
-def barnes_hut(ctx, pos_mass, theta=0.5, eps=1e-3, G=1.0):
-    """barnes_hut!(bodies, theta, G, out): accelerations (n, 3)."""
-    return ctx.barnes_hut(pos_mass, theta, eps, G)
+def barnes_hut(ctx, pos_mass, theta=0.5, G=1.0, eps=1e-3, law="reference"):
+    """barnes_hut!(bodies, theta, G, out): accelerations (n, 3), by default
+    with the reference's force law (gprt-macros/src/lib.rs:785)."""
+    return ctx.barnes_hut(pos_mass, theta, eps, G, law=law)

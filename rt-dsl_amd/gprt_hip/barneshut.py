@@ -6,11 +6,11 @@ Follows gprt-sandbox/src/bin/barneshut.rs:36-70: reads "x,y,z,m" lines
 (gprt_hip.read_points with CSV_BODIES, the rules of load_galaxy,
 barneshut.rs:8-34), times the force computation (tree build, walk, copies in
 and out), and prints the sum over bodies of |F_i| and a report block.  F_i is
-m_i times the library's acceleration with G = 6.674e-11 and softening
-eps = 1e-3, i.e. G m_i m_j (x_j - x_i) / (|x_j - x_i|^2 + eps^2)^1.5 summed
-with the Barnes-Hut approximation; the reference's law divides by
-d^2 sqrt(d^2 + 1e-6) instead (gprt-macros/src/lib.rs:785), which differs only
-for bodies closer than about eps.
+m_i times the library's acceleration under the reference's force law
+(gprt-macros/src/lib.rs:785, 790, 799): G m_i m_j (x_j - x_i) /
+(d^2 sqrt(d^2 + 1e-6)) over pairs with d^2 > 1e-10, G = 6.674e-11, summed
+with the Barnes-Hut approximation at the given theta (theta = 0 sums every
+pair, which is what the reference's launch does as shipped).
 """
 import math
 import sys
@@ -50,7 +50,8 @@ def run(argv, out=print):
     ctx = gprt_hip.Context(0)
     try:
         t0 = time.perf_counter()
-        acc = ctx.barnes_hut(bodies, theta=theta, eps=EPS, G=G_NEWTON) if len(bodies) else np.zeros((0, 3), np.float32)
+        acc = (ctx.barnes_hut(bodies, theta=theta, eps=EPS, G=G_NEWTON, law="reference") if len(bodies)
+               else np.zeros((0, 3), np.float32))
         ms = (time.perf_counter() - t0) * 1e3
     finally:
         ctx.close()

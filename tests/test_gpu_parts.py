@@ -263,6 +263,27 @@ def test_bh_error_vs_direct(ctx, theta, tol_med, tol_p99):
     assert np.median(err) < tol_med and np.quantile(err, 0.99) < tol_p99
 
 
+def test_bh_reference_law(ctx):
+    """GPRT_BH_REFERENCE_LAW: the reference shader's law.  At theta = 0 it is
+    the FP64 direct sum of that law (close pairs and a coincident pair
+    included), and it differs from the Plummer result where pairs are close."""
+    pm = D.plummer(20_000, seed=9)
+    pm[1, :3] = pm[0, :3] + np.float32([1e-3, 0, 0])
+    pm[2, :3] = pm[0, :3] + np.float32([0, 2e-4, 0])
+    pm[4, :3] = pm[3, :3]
+    sample = np.concatenate([np.arange(5), np.arange(5, 20_000, 97)])
+    acc = ctx.barnes_hut(pm, theta=0.0, eps=1e-3, G=1.0, law="reference")
+    ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3, law="reference")
+    assert np.max(_rel_err(acc[sample].astype(np.float64), ref)) < 2e-5
+    pl = oracle.bh_direct(pm, np.arange(3), G=1.0, eps=1e-3)
+    assert np.min(_rel_err(acc[:3].astype(np.float64), pl)) > 0.05
+    acc5 = ctx.barnes_hut(pm, theta=0.5, eps=1e-3, G=1.0, law="reference")
+    err = _rel_err(acc5[sample].astype(np.float64), ref)
+    assert np.median(err) < 5e-3 and np.quantile(err, 0.99) < 5e-2
+    with pytest.raises(ValueError):
+        ctx.barnes_hut(pm, law="newton")
+
+
 def test_bh_two_bodies_and_rejects_negative_mass(ctx):
     pm = np.array([[0, 0, 0, 2.0], [3, 4, 0, 5.0]], np.float32)
     acc = ctx.barnes_hut(pm, theta=0.5, eps=0.5, G=1.5)

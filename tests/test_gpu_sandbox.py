@@ -89,6 +89,10 @@ def test_barneshut_sandbox(tmp_path):
     rng = np.random.default_rng(24)
     n = 4_000
     pm = np.concatenate([rng.normal(size=(n, 3)), rng.random((n, 1)) + 0.5], axis=1).astype(np.float32)
+    # a pair 1e-3 apart (where the reference's law is twice the Plummer one)
+    # and a coincident duplicate (a pair the reference's law skips)
+    pm[1, :3] = pm[0, :3] + np.float32([1e-3, 0, 0])
+    pm[2] = pm[3]
     path = tmp_path / "galaxy.txt"
     with open(path, "w") as f:
         f.write("0,0,0,0\n")  # the dummy line the reader skips
@@ -99,11 +103,13 @@ def test_barneshut_sandbox(tmp_path):
                          "Compiling and launching RT-BarnesHut via GPRT DSL..."]
     assert lines[3].startswith("Total Force Magnitude: ")
     assert f"Forces Computed: {n}" in lines
-    # theta = 0 visits every body: the FP64 direct sum of the softened law
-    acc = oracle.bh_direct(pm, G=barneshut.G_NEWTON, eps=barneshut.EPS)
+    # theta = 0 visits every body: the FP64 direct sum of the reference's law
+    acc = oracle.bh_direct(pm, G=barneshut.G_NEWTON, eps=barneshut.EPS, law="reference")
     want = acc * pm[:, 3:4].astype(np.float64)
     err = np.linalg.norm(forces - want, axis=1) / np.linalg.norm(want, axis=1)
     assert err.max() < 2e-5
+    plummer = oracle.bh_direct(pm[:2], G=barneshut.G_NEWTON, eps=barneshut.EPS) * pm[:2, 3:4]
+    assert np.linalg.norm(forces[:2] - plummer, axis=1).min() > 0.1 * np.linalg.norm(plummer, axis=1).min()
     total = np.linalg.norm(want, axis=1).sum()
     got = float(lines[3].split(": ")[1])
     assert abs(got - total) / total < 1e-3

@@ -346,6 +346,39 @@ def test_bh_direct_matches_numpy_and_conserves_momentum():
     assert np.array_equal(oracle.bh_direct(pm, sample, G=1.0, eps=1e-3), acc[sample])
 
 
+def test_bh_direct_reference_law_properties():
+    """The reference shader's law, G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over
+    pairs with |d|^2 > 1e-10: equal to the Plummer law far away, exactly twice
+    it at |d| = eps, pairs inside the cut-off contribute nothing (coincident
+    bodies included), and forces are antisymmetric (momentum is conserved)."""
+    eps = 0.5
+    far = np.array([[0, 0, 0, 2.0], [600, 800, 0, 5.0]], np.float32)
+    a_ref = oracle.bh_direct(far, G=1.5, eps=eps, law="reference")
+    a_pl = oracle.bh_direct(far, G=1.5, eps=eps)
+    assert np.allclose(a_ref, a_pl, rtol=1e-6) and not np.array_equal(a_ref, a_pl)
+    assert np.allclose(a_ref[0], 1.5 * 5.0 * np.array([0.6, 0.8, 0]) / 1000.0 ** 2, rtol=1e-6)
+    at_eps = np.array([[0, 0, 0, 1.0], [0.5, 0, 0, 3.0]], np.float32)
+    ratio = oracle.bh_direct(at_eps, eps=eps, law="reference")[:, 0] / oracle.bh_direct(at_eps, eps=eps)[:, 0]
+    assert np.allclose(ratio, 2.0, rtol=1e-12)
+    # cut-off: d2 = 0 and d2 = 1e-12 skipped, d2 = 4e-10 kept
+    pm = np.array([[1, 1, 1, 1.0], [1, 1, 1, 2.0], [1 + 1e-6, 1, 1, 4.0]], np.float32)
+    d = np.float64(np.float32(1 + 1e-6)) - 1.0
+    acc = oracle.bh_direct(pm, G=1.0, eps=1e-3, law="reference")
+    assert np.all(np.isfinite(acc)) and np.all(acc[:, 1:] == 0)
+    assert d * d < 1e-10 and np.all(acc == 0)
+    pm[2, 0] = np.float32(1 + 2e-5)
+    d = np.float64(pm[2, 0]) - 1.0
+    acc = oracle.bh_direct(pm, G=1.0, eps=1e-3, law="reference")
+    assert d * d > 1e-10 and np.isclose(acc[0, 0], 4.0 * d / (d * d * np.sqrt(d * d + 1e-6)), rtol=1e-9)
+    rng = np.random.default_rng(10)
+    pm = np.concatenate([rng.normal(size=(300, 3)), rng.uniform(0.5, 1.5, (300, 1))], 1).astype(np.float32)
+    acc = oracle.bh_direct(pm, G=1.0, eps=1e-3, law="reference")
+    m = pm[:, 3].astype(np.float64)
+    assert np.abs((m[:, None] * acc).sum(0)).max() < 1e-9 * np.abs(m[:, None] * acc).sum()
+    with pytest.raises(ValueError):
+        oracle.bh_direct(pm, law="newton")
+
+
 def test_d2_contract_is_binary64_without_fma():
     q = np.array([0.1, 0.2, 0.3], np.float32)
     p = np.array([0.7, -0.4, 1e-3], np.float32)
