@@ -221,7 +221,7 @@ def main():
                     help="run one secondary workload alone (for profiling) and print its JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the k-NN path would shard queries over a replicated tree)", file=sys.stderr)
         return 2
     if args.only:
