@@ -92,7 +92,7 @@ static void drop_extra(Ctx* c) {
                 b->release();
             Tree& T = e->bh_tree;
             for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm,
-                              &T.inv, &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
+                              &T.orig4, &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
                 b->release();
             delete e;
             g_extra.erase(g_extra.begin() + i);
@@ -354,7 +354,7 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
         (void)hipSetDevice(t->owner->c.device);
     }
     Tree& T = t->t;
-    for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.inv,
+    for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.orig4,
                       &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
         b->release();
     delete t;
@@ -492,16 +492,20 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     int nclasses = 0;
     for (int cl = 0; cl < N_KCLASS; ++cl) nclasses += p.class_size[cl] > 0;
     // the radius sampler runs on the side stream, overlapping the query sort;
-    // the rounds wait for both.  The sort, the longer branch, is enqueued
-    // first (a captured graph launches its branches in that order).
+    // the rounds wait for both.  The longer branch is enqueued first (a
+    // captured graph launches its branches in that order): the sort for one
+    // cloud, the sampler (100 x 5000 draws per cloud) for several.
     GPRT_HIP_TRY(hipEventRecord(c.ev_fork, s));
-    GPRT_TRY(launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
-                               perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
-                               p.d_kc, p.k, nclasses > 1));
+    auto sort = [&]() {
+        return launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
+                                 perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
+                                 p.d_kc, p.k, nclasses > 1);
+    };
+    if (p.nclouds == 1) GPRT_TRY(sort());
     GPRT_HIP_TRY(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-    GPRT_TRY(launch_sampler(c.side, cfg.sampler, T.pts4.as<float4>(), T.inv.as<uint32_t>(),
-                            T.cloud_off.as<int64_t>(), p.d_q, p.d_qoff, p.nclouds, p.kmax, cfg.radius_heuristic,
-                            cfg.radius_param, kth, radius));
+    GPRT_TRY(launch_sampler(c.side, cfg.sampler, T.orig4.as<float4>(), T.cloud_off.as<int64_t>(), p.d_q, p.d_qoff,
+                            p.nclouds, p.kmax, cfg.radius_heuristic, cfg.radius_param, kth, radius));
+    if (p.nclouds > 1) GPRT_TRY(sort());
     GPRT_HIP_TRY(hipEventRecord(c.ev_join, c.side));
     GPRT_HIP_TRY(hipStreamWaitEvent(s, c.ev_join, 0));
     int grid = cfg.grid_blocks > 0 ? cfg.grid_blocks : 0;
@@ -594,7 +598,7 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
         // everything the captured kernels point at: a rebuilt tree or a grown
         // workspace changes a pointer and forces a new capture
         const Tree& T = *p.tree;
-        const void* ptrs[] = {T.nodes.ptr, T.nodes4.ptr, T.pts4.ptr, T.roots.ptr, T.inv.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
+        const void* ptrs[] = {T.nodes.ptr, T.nodes4.ptr, T.pts4.ptr, T.roots.ptr, T.orig4.ptr, T.key_bbox.ptr, T.cloud_off.ptr,
                               c.ws[0].ptr, c.ws[1].ptr, c.ws[2].ptr, c.ws[3].ptr, c.ws[4].ptr, c.ws[5].ptr,
                               c.ws[6].ptr, c.sort_tmp[0].ptr, c.sort_tmp[1].ptr, c.sort_tmp[2].ptr,
                               c.counters.ptr, extra(&c)->qcloud.ptr, extra(&c)->spill.ptr};
@@ -937,11 +941,10 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
         float* rad = kth + 100;
         (void)hipMemcpyAsync(x->qoff.ptr, qo, 16, hipMemcpyHostToDevice, s);
         if (n > 0 && nq > 0)
-            rc = launch_sampler(s, sampler, t->t.pts4.as<float4>(), t->t.inv.as<uint32_t>(),
-                                t->t.cloud_off.as<int64_t>(), d_q, x->qoff.as<int64_t>(), 1, k, heuristic, param,
-                                kth, rad);
+            rc = launch_sampler(s, sampler, t->t.orig4.as<float4>(), t->t.cloud_off.as<int64_t>(), d_q,
+                                x->qoff.as<int64_t>(), 1, k, heuristic, param, kth, rad);
         else
-            rc = launch_sampler(s, sampler, nullptr, nullptr, nullptr, nullptr, nullptr, 1, k, GPRT_RADIUS_INFINITE,
+            rc = launch_sampler(s, sampler, nullptr, nullptr, nullptr, nullptr, 1, k, GPRT_RADIUS_INFINITE,
                                 param, kth, rad);
         float r = 0;
         if (rc == GPRT_OK && hipMemcpyAsync(&r, rad, 4, hipMemcpyDeviceToHost, s) == hipSuccess &&

@@ -92,7 +92,7 @@ struct Tree {
     DevBuf cloud_bbox;  // float[6*nclouds]: per-cloud (min xyz, max xyz)
     DevBuf key_bbox;    // float[6*nclouds]: box the sort keys are normalised to
     DevBuf perm;        // uint32[n]: sorted position -> input index
-    DevBuf inv;         // uint32[n]: input index -> sorted position
+    DevBuf orig4;       // float4[n]: the points in input order (x, y, z, 0), read by the radius samplers
     DevBuf codes;       // uint64[n]: sorted keys (cloud << 3*axis_bits | morton)
     DevBuf parent;      // int[n-1 + n]: parent of internal nodes, then of points
     DevBuf flags;       // int[n-1]: refit arrival counters

@@ -77,8 +77,47 @@ __device__ __forceinline__ void ins4(float* d, float v) {
 // the point loads of its (at most five) draws together, then keeps its four
 // smallest values; lanes, then waves, merge their lists.
 constexpr int SAMPLER_THREADS = 1024;
-__global__ __launch_bounds__(SAMPLER_THREADS) void sampler_lcg_kernel(const float4* __restrict__ pts4,
-                                                                      const uint32_t* __restrict__ inv,
+constexpr int SAMPLER_PER = 5;  // draws per lane: 1024 x 5 >= 5000
+
+// The LCG advanced by SAMPLER_PER * t steps is state -> a[t] * state + c[t]
+// (mod 2^64); the table is built at compile time, so lane t jumps to its
+// share of a slot's draws with one multiply-add.
+struct LcgJumpTable {
+    uint64_t a[SAMPLER_THREADS], c[SAMPLER_THREADS];
+    constexpr LcgJumpTable() : a(), c() {
+        uint64_t ap = 1, cp = 0;  // one lane's stride: SAMPLER_PER steps
+        for (int i = 0; i < SAMPLER_PER; ++i) {
+            cp = cp * LCG_A + LCG_C;
+            ap *= LCG_A;
+        }
+        uint64_t am = 1, cm = 0;
+        for (int t = 0; t < SAMPLER_THREADS; ++t) {
+            a[t] = am;
+            c[t] = cm;
+            cm = cm * ap + cp;
+            am *= ap;
+        }
+    }
+};
+__constant__ LcgJumpTable g_lcg_jump = LcgJumpTable();
+
+// x mod n for 0 <= x, 0 < n < 2^31, without an integer division: the FP64
+// product x * fl(1/n) has relative error below 2^-51, so it is within
+// 2^-21 / n of x / n; a non-integer x / n lies at least 1 / n from the
+// integers around it, so the truncated product is floor(x / n), or one less
+// when x / n is an integer, which the single correction step fixes.
+struct ModN {
+    uint32_t n;
+    double r;
+    __device__ explicit ModN(uint32_t n_) : n(n_), r(1.0 / (double)n_) {}
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const {
+        const uint32_t q = (uint32_t)((double)x * r);
+        uint32_t m = x - q * n;
+        if (m >= n) m -= n;
+        return m;
+    }
+};
+__global__ __launch_bounds__(SAMPLER_THREADS) void sampler_lcg_kernel(const float4* __restrict__ orig4,
                                                                       const int64_t* __restrict__ poff,
                                                                       const float* __restrict__ qs,
                                                                       const int64_t* __restrict__ qoff, float* kth) {
@@ -96,25 +135,22 @@ __global__ __launch_bounds__(SAMPLER_THREADS) void sampler_lcg_kernel(const floa
     const uint64_t qstate = lcg_jump(LCG_SEED, qstep);
     const int64_t qi = (int64_t)((uint32_t)(qstate >> 33) % (uint32_t)nq);
     float q[3] = {qs[3 * (qoff[c] + qi)], qs[3 * (qoff[c] + qi) + 1], qs[3 * (qoff[c] + qi) + 2]};
-    constexpr int PER = 5;  // 1024 lanes x 5 >= 5000 draws
+    constexpr int PER = SAMPLER_PER;
     const int64_t t0 = (int64_t)threadIdx.x * PER;
     float d[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
     if (t0 < S) {
-        uint64_t st = lcg_jump(qstate, (uint64_t)t0);
+        uint64_t st = g_lcg_jump.a[threadIdx.x] * qstate + g_lcg_jump.c[threadIdx.x];
+        const ModN mod((uint32_t)n);
         uint32_t idx[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             st = st * LCG_A + LCG_C;
-            idx[u] = (uint32_t)(st >> 33) % (uint32_t)n;
+            idx[u] = mod((uint32_t)(st >> 33));
         }
-        uint32_t sp[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u)
-            if (t0 + u < S) sp[u] = inv[poff[c] + idx[u]];
         float4 p4[PER];
 #pragma unroll
         for (int u = 0; u < PER; ++u)
-            if (t0 + u < S) p4[u] = pts4[sp[u]];
+            if (t0 + u < S) p4[u] = orig4[poff[c] + idx[u]];
 #pragma unroll
         for (int u = 0; u < PER; ++u)
             if (t0 + u < S) {
@@ -144,8 +180,7 @@ __global__ __launch_bounds__(SAMPLER_THREADS) void sampler_lcg_kernel(const floa
 // grid (100, nclouds), 256 threads.  Strided sampler slot s: query s*qstep
 // against every pstep-th point (at most 1000), k smallest d^2 with
 // 1e-10 < d^2 (< f32::MAX); kth = sqrt(k-th) or -1.
-__global__ __launch_bounds__(256) void sampler_strided_kernel(const float4* __restrict__ pts4,
-                                                              const uint32_t* __restrict__ inv,
+__global__ __launch_bounds__(256) void sampler_strided_kernel(const float4* __restrict__ orig4,
                                                               const int64_t* __restrict__ poff,
                                                               const float* __restrict__ qs,
                                                               const int64_t* __restrict__ qoff, int k,
@@ -165,7 +200,7 @@ __global__ __launch_bounds__(256) void sampler_strided_kernel(const float4* __re
         int64_t pi = (int64_t)j * pstep;
         float dd = INFINITY;
         if (j < 1000 && pi < n) {
-            const float4 p4 = pts4[inv[poff[c] + pi]];
+            const float4 p4 = orig4[poff[c] + pi];
             const float p[3] = {p4.x, p4.y, p4.z};
             dd = d2_sampler(q, p);
             if (!(dd > 1e-10f && dd < FLT_MAX)) dd = INFINITY;
@@ -421,16 +456,16 @@ __global__ __launch_bounds__(256) void knn_exact_kernel(ExactArgs a) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_t* inv, const int64_t* poff,
+int launch_sampler(hipStream_t s, int sampler, const float4* orig4, const int64_t* poff,
                    const float* qs,
                    const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth,
                    float* radius) {
     if (heuristic != GPRT_RADIUS_INFINITE && !(sampler == GPRT_SAMPLER_STRIDED && heuristic != GPRT_RADIUS_PERCENTILE)) {
         if (sampler == GPRT_SAMPLER_LCG)
-            hipLaunchKernelGGL(sampler_lcg_kernel, dim3(100, nclouds), dim3(SAMPLER_THREADS), 0, s, pts4, inv, poff,
+            hipLaunchKernelGGL(sampler_lcg_kernel, dim3(100, nclouds), dim3(SAMPLER_THREADS), 0, s, orig4, poff,
                                qs, qoff, kth);
         else
-            hipLaunchKernelGGL(sampler_strided_kernel, dim3(100, nclouds), dim3(256), 0, s, pts4, inv, poff, qs, qoff,
+            hipLaunchKernelGGL(sampler_strided_kernel, dim3(100, nclouds), dim3(256), 0, s, orig4, poff, qs, qoff,
                                k, kth);
     } else {
         hipLaunchKernelGGL(fill_u32_kernel, dim3((100 * nclouds + 255) / 256), dim3(256), 0, s, (unsigned*)kth,

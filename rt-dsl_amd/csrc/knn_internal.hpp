@@ -74,7 +74,7 @@ struct ExactArgs {
     int max_steps;
 };
 
-int launch_sampler(hipStream_t s, int sampler, const float4* pts4, const uint32_t* inv, const int64_t* poff,
+int launch_sampler(hipStream_t s, int sampler, const float4* orig4, const int64_t* poff,
                    const float* qs,
                    const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth, float* radius);
 // queries -> Morton-bucket order (float4 with the input index in .w); keys is

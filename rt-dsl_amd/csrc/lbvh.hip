@@ -304,9 +304,11 @@ __global__ void roots_small_kernel(const int64_t* __restrict__ off, int nclouds,
     }
 }
 
-__global__ void invert_perm_kernel(const uint32_t* __restrict__ perm, uint32_t* __restrict__ inv, int n) {
+// the points in input order as float4 (x, y, z, 0): the radius samplers draw
+// input indices, and one 16-byte load per draw serves them
+__global__ void input_points_kernel(const float* __restrict__ pts, float4* __restrict__ out, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inv[perm[i]] = (uint32_t)i;
+    if (i < n) out[i] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], 0.0f);
 }
 
 __global__ void fill_int_kernel(int* p, int n, int v) {
@@ -348,7 +350,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     GPRT_TRY(t->nodes.reserve(sizeof(Node) * std::max(n - 1, 1)));
     if (t->build_wide) GPRT_TRY(t->nodes4.reserve(sizeof(Node4) * std::max(n - 1, 1)));
     GPRT_TRY(t->perm.reserve(sizeof(uint32_t) * std::max(n, 1)));
-    GPRT_TRY(t->inv.reserve(sizeof(uint32_t) * std::max(n, 1)));
+    GPRT_TRY(t->orig4.reserve(sizeof(float4) * std::max(n, 1)));
     GPRT_TRY(t->codes.reserve(sizeof(uint64_t) * std::max(n, 1)));
     GPRT_TRY(t->parent.reserve(sizeof(int) * std::max(2 * n - 1, 1)));
     GPRT_TRY(t->flags.reserve(sizeof(int) * std::max(n - 1, 1)));
@@ -379,8 +381,8 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
         hipLaunchKernelGGL(gather_points_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts,
                            t->perm.as<uint32_t>(), t->codes.as<uint64_t>(), 3 * b, t->cloud_off.as<int64_t>(),
                            t->pts4.as<float4>(), n);
-        hipLaunchKernelGGL(invert_perm_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, t->perm.as<uint32_t>(),
-                           t->inv.as<uint32_t>(), n);
+        hipLaunchKernelGGL(input_points_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts, t->orig4.as<float4>(),
+                           n);
     }
     if (n > 1) {
         hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->flags.as<int>(), n - 1, 0);

@@ -72,6 +72,20 @@ def test_knn_uses_sampled_radius(ctx, heur, param):
             assert np.float32(ctx.stats()["initial_radius"]) == np.float32(want)
 
 
+@pytest.mark.parametrize("n,nq", [(1, 1), (2, 3), (7, 1000), (4_999, 99), (5_000, 100), (5_001, 101),
+                                  (65_537, 7), (262_147, 50_000), (1_048_573, 123_457)])
+def test_sampler_lcg_sizes(ctx, n, nq):
+    """The LCG sampler's draws (lane jumps from the compile-time table, indices
+    reduced mod n without integer division) across cloud and query sizes,
+    prime and power-of-two neighbours included, bit for bit."""
+    rng = np.random.default_rng(n + nq)
+    pts = rng.random((n, 3), dtype=np.float32)
+    q = rng.random((nq, 3), dtype=np.float32)
+    for heur, param in ((gprt_hip.RADIUS_PERCENTILE, 0.95), (gprt_hip.RADIUS_MAX, 0.0)):
+        assert ctx.sample_radius(pts, q, gprt_hip.SAMPLER_LCG, heur, param, 5) == \
+            oracle.sampler_lcg(pts, q, heur, param)
+
+
 def test_sampler_small_inputs(ctx):
     pts = np.zeros((3, 3), np.float32)
     q = np.ones((5, 3), np.float32)
