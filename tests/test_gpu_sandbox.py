@@ -113,3 +113,34 @@ def test_barneshut_sandbox(tmp_path):
     total = np.linalg.norm(want, axis=1).sum()
     got = float(lines[3].split(": ")[1])
     assert abs(got - total) / total < 1e-3
+
+
+def test_barneshut_reference_physics_check(tmp_path):
+    """The reference's own Barnes-Hut check (gprt-sandbox/verify_physics.py:5-48):
+    10,000 bodies from RandomState(42), masses U(0.5, 1.5) drawn first, then
+    positions U(-100, 100)^3, written as "x,y,z,m" lines; `barneshut <file> 0.5`
+    must print a total |F| within 2% of the FP64 O(N^2) Newtonian sum (pairs
+    closer than 1e-6 skipped).  Per-body errors are checked here as well."""
+    n = 10_000
+    rs = np.random.RandomState(42)
+    mass = rs.uniform(0.5, 1.5, n)
+    pos = rs.uniform(-100, 100, (n, 3))
+    path = tmp_path / "small_test.txt"
+    path.write_text("".join(f"{x},{y},{z},{m}\n" for (x, y, z), m in zip(pos.tolist(), mass.tolist())))
+    lines = []
+    forces = barneshut.run(["barneshut", str(path), "0.5"], lines.append)
+    # FP64 ground truth from the same (float64) values, 250 bodies at a time
+    exact = np.empty((n, 3))
+    for i0 in range(0, n, 250):
+        d = pos[None, :, :] - pos[i0:i0 + 250, None, :]
+        r = np.sqrt((d * d).sum(-1))
+        w = np.where(r > 1e-6, barneshut.G_NEWTON * mass[i0:i0 + 250, None] * mass[None, :] /
+                     np.maximum(r, 1e-6) ** 3, 0.0)
+        exact[i0:i0 + 250] = (d * w[..., None]).sum(1)
+    mag_exact = np.linalg.norm(exact, axis=1).sum()
+    total = float(lines[3].split(": ")[1])
+    assert abs(total - mag_exact) / mag_exact < 0.02
+    err = np.linalg.norm(forces - exact, axis=1) / np.linalg.norm(exact, axis=1)
+    print(f"total |F| rel. error {abs(total - mag_exact) / mag_exact:.2e}; per body median {np.median(err):.2e} "
+          f"p99 {np.quantile(err, 0.99):.2e}")
+    assert np.median(err) < 5e-3 and np.quantile(err, 0.99) < 5e-2
