@@ -187,6 +187,10 @@ GPRT_HIP_API int gprt_hip_tree_build(gprt_hip_ctx* ctx, const float* pts, int64_
  * Each cloud gets its own Morton normalisation and its own subtree. */
 GPRT_HIP_API int gprt_hip_tree_build_batched(gprt_hip_ctx* ctx, const float* pts, const int64_t* cloud_offsets,
                                              int nclouds, int leaf_size, int ptr_flags, gprt_hip_tree** out);
+/* Waits for the device, then hands the tree's device buffers to the context
+ * that built it, whose next build reuses them (growing any that are too
+ * small); a context holds at most one destroyed tree's buffers, freed by
+ * gprt_hip_ctx_destroy. */
 GPRT_HIP_API int gprt_hip_tree_destroy(gprt_hip_tree* tree);
 GPRT_HIP_API int gprt_hip_tree_info(const gprt_hip_tree* tree, gprt_hip_tree_desc* out);
 /* Host copies of the tree for invariant checks: sorted keys (n), permutation

@@ -6,6 +6,7 @@
 // rounds.  The whole sequence is enqueued on the context stream and, when
 // cfg.use_graph is set, captured once into a hipGraph and replayed.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -233,6 +234,7 @@ extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
     for (auto& b : c.ws) b.release();
     for (auto& b : c.sort_tmp) b.release();
     c.counters.release();
+    for (auto& b : c.tree_pool) b.release();
     if (c.hstage) (void)hipHostFree(c.hstage);
     drop_extra(&c);
     if (c.side) (void)hipStreamSynchronize(c.side);
@@ -318,6 +320,13 @@ static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, i
     if (bad) return fail(GPRT_ERR_INPUT, "points contain NaN/Inf or |coordinate| > 1e18");
     auto* t = new gprt_hip_tree();
     t->owner = h;
+    static std::atomic<uint64_t> builds{0};
+    t->t.gen = ++builds;
+    {
+        // the buffers of the context's last destroyed tree (grown below if small)
+        auto bufs = tree_bufs(t->t);
+        for (int i = 0; i < TREE_BUFS; ++i) std::swap(*bufs[i], c.tree_pool[i]);
+    }
     GPRT_HIP_TRY(hipEventRecord(c.ev2, c.stream));
     int rc = tree_build(&c, d_pts, off, nclouds, leaf_size, &t->t, (flags & GPRT_KEYS_REF30) ? 10 : 0);
     if (rc != GPRT_OK) {
@@ -353,10 +362,20 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
         lk = std::unique_lock<std::recursive_mutex>(t->owner->mu);
         (void)hipSetDevice(t->owner->c.device);
     }
-    Tree& T = t->t;
-    for (DevBuf* b : {&T.pts4, &T.nodes, &T.nodes4, &T.roots, &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm, &T.orig4,
-                      &T.codes, &T.parent, &T.flags, &T.ranges, &T.ready, &T.bh})
-        b->release();
+    auto bufs = tree_bufs(t->t);
+    if (t->owner) {
+        // kept for the owner's next build.  Work that still reads the tree may
+        // sit on any context's stream, so wait for the device first, as
+        // freeing the buffers would
+        (void)hipDeviceSynchronize();
+        DevBuf* pool = t->owner->c.tree_pool;
+        for (int i = 0; i < TREE_BUFS; ++i) {
+            pool[i].release();
+            std::swap(*bufs[i], pool[i]);
+        }
+    } else {
+        for (DevBuf* b : bufs) b->release();
+    }
     delete t;
     return GPRT_OK;
 }
@@ -606,6 +625,7 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
         key = fnv(key, &p, sizeof(KnnPlan));
         key = fnv(key, ptrs, sizeof(ptrs));
         key = fnv(key, &T.n, sizeof(int));
+        key = fnv(key, &T.gen, sizeof(T.gen));
         if (!c.graph_exec || c.graph_key != key) {
             if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
             if (c.graph) (void)hipGraphDestroy(c.graph);

@@ -10,6 +10,7 @@
 // for every point p inside B: pruning a box whose value exceeds a threshold
 // never drops a point whose own value is within it.
 #pragma once
+#include <array>
 #include <mutex>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -105,7 +106,16 @@ struct Tree {
     int key_bits = 30;  // significant bits of the sort key
     int axis_bits = 10; // Morton bits per axis
     int max_depth_bound = 64;
+    uint64_t gen = 0;   // build number, different for every build in the process
 };
+
+// the device buffers of a tree, in a fixed order (a context's pool of reusable
+// tree buffers follows it)
+constexpr int TREE_BUFS = 15;
+inline std::array<DevBuf*, TREE_BUFS> tree_bufs(Tree& T) {
+    return {&T.pts4, &T.nodes,  &T.nodes4, &T.roots,  &T.cloud_off, &T.cloud_bbox, &T.key_bbox, &T.perm,
+            &T.orig4, &T.codes, &T.parent, &T.flags,  &T.ranges,    &T.ready,      &T.bh};
+}
 
 struct Ctx;
 
@@ -201,6 +211,10 @@ struct Ctx {
     // pinned host copy of the counter buffer plus one float (the sampled
     // initial radius), filled by copies enqueued behind each call's work
     unsigned* hstage = nullptr;
+    // the device buffers of the last tree destroyed on this context, taken
+    // over by the next build (rebuilding a tree per frame allocates nothing
+    // once the sizes settle)
+    DevBuf tree_pool[TREE_BUFS];
 };
 
 }  // namespace gprt
