@@ -161,6 +161,9 @@ GPRT_HIP_API const char* gprt_hip_last_error(void);
 GPRT_HIP_API void gprt_hip_clear_error(void);
 GPRT_HIP_API void gprt_hip_config_default(gprt_hip_config* cfg);
 GPRT_HIP_API int gprt_hip_ctx_create(int device, gprt_hip_ctx** out);
+/* Trees built on the context may outlive it: they stay usable with another
+ * context on the same device (gprt_hip_tree_export excepted) and free their
+ * own buffers when destroyed. */
 GPRT_HIP_API int gprt_hip_ctx_destroy(gprt_hip_ctx* ctx);
 GPRT_HIP_API int gprt_hip_ctx_sync(gprt_hip_ctx* ctx);
 GPRT_HIP_API int gprt_hip_stats(gprt_hip_ctx* ctx, gprt_hip_run_stats* out);

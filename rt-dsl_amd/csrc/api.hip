@@ -226,6 +226,13 @@ extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
 
 extern "C" int gprt_hip_ctx_destroy(gprt_hip_ctx* h) {
     if (!h) return GPRT_OK;
+    {
+        // trees that outlive their context keep working and free their own
+        // buffers when destroyed
+        GPRT_CTX_LOCK(h);
+        for (gprt_hip_tree* t : h->trees) t->owner = nullptr;
+        h->trees.clear();
+    }
     Ctx& c = h->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
@@ -320,6 +327,7 @@ static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, i
     if (bad) return fail(GPRT_ERR_INPUT, "points contain NaN/Inf or |coordinate| > 1e18");
     auto* t = new gprt_hip_tree();
     t->owner = h;
+    h->trees.push_back(t);
     static std::atomic<uint64_t> builds{0};
     t->t.gen = ++builds;
     {
@@ -373,6 +381,8 @@ extern "C" int gprt_hip_tree_destroy(gprt_hip_tree* t) {
             pool[i].release();
             std::swap(*bufs[i], pool[i]);
         }
+        auto& live = t->owner->trees;
+        live.erase(std::remove(live.begin(), live.end(), t), live.end());
     } else {
         for (DevBuf* b : bufs) b->release();
     }
@@ -393,7 +403,8 @@ extern "C" int gprt_hip_tree_info(const gprt_hip_tree* t, gprt_hip_tree_desc* ou
 
 extern "C" int gprt_hip_tree_export(gprt_hip_tree* t, uint64_t* keys, uint32_t* perm, float* nodes,
                                     int32_t* roots) {
-    if (!t || !t->owner) return fail(GPRT_ERR_ARG, "null tree");
+    if (!t) return fail(GPRT_ERR_ARG, "null tree");
+    if (!t->owner) return fail(GPRT_ERR_STATE, "the tree's context was destroyed");
     GPRT_CTX_LOCK(t->owner);
     Ctx& c = t->owner->c;
     Tree& T = t->t;

@@ -223,9 +223,14 @@ struct Ctx {
 // through one of its trees, or through a legacy pipeline) holds its mutex for
 // the whole call, so calls on one context from several threads run one at a
 // time; recursive because some entry points call others
+struct gprt_hip_tree;
 struct gprt_hip_ctx {
     gprt::Ctx c;
     std::recursive_mutex mu;
+    // the trees built on this context and not yet destroyed; destroying the
+    // context first leaves them without an owner (they then free their own
+    // buffers)
+    std::vector<gprt_hip_tree*> trees;
 };
 struct gprt_hip_tree {
     gprt::Tree t;

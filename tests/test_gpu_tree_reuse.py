@@ -34,3 +34,28 @@ def test_rebuilds_reuse_buffers(ctx):
             keep = ctx.build(keep_pts)
     assert np.array_equal(keep.k_nn(q, 4, cfg), oracle.knn_brute(keep_pts, q, 4))
     keep.close()
+
+
+def test_tree_outlives_its_context():
+    """A context destroyed before its trees: they keep working through another
+    context and free their own buffers (no use of the destroyed context)."""
+    rng = np.random.default_rng(78)
+    pts = rng.random((20_000, 3), dtype=np.float32)
+    q = rng.random((1_000, 3), dtype=np.float32)
+    c1 = gprt_hip.Context(0)
+    t = c1.build(pts)
+    dead = c1.build(rng.random((3_000, 3), dtype=np.float32))
+    c1.close()
+    c2 = gprt_hip.Context(0)
+    try:
+        t.ctx = c2
+        assert np.array_equal(t.k_nn(q, 6), oracle.knn_brute(pts, q, 6))
+        with pytest.raises(gprt_hip.GprtError):
+            t.export()
+        dead.close()
+        t.close()
+        t2 = c2.build(pts)
+        assert np.array_equal(t2.k_nn(q, 6), oracle.knn_brute(pts, q, 6))
+        t2.close()
+    finally:
+        c2.close()
