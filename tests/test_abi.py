@@ -15,6 +15,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "gprt_hip.h")
 
 
+@pytest.mark.parametrize("lang,std", [("c", "-std=c99"), ("c++", "-std=c++11")])
+def test_header_is_plain_c_and_cxx(lang, std, tmp_path):
+    """The boundary header compiles on its own as C99 (what cgo, a C binding
+    generator or ctypes users read) and as C++11, without warnings."""
+    import shutil
+    cc = shutil.which("gcc" if lang == "c" else "g++")
+    if cc is None:
+        pytest.skip("no host compiler")
+    r = subprocess.run([cc, "-fsyntax-only", std, "-Wall", "-Wextra", "-Werror", "-x", lang, HEADER],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def declared_symbols():
     text = open(HEADER).read()
     return re.findall(r"GPRT_HIP_API\s+[\w\s\*]+?\b(\w+)\s*\(", text)
