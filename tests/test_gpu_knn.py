@@ -113,6 +113,37 @@ def test_knn_ties_lattice(ctx):
     assert ctx.stats()["queries"] == len(q)
 
 
+@pytest.mark.xfail(reason="open defect: on integer lattices with exact distance ties at large k the batched "
+                          "form returns one id outside the cloud (with a plausible small distance) for a few "
+                          "queries when many fall back to the FP64 kernel; the plain form is exact. Cause not "
+                          "found.", strict=False)
+@pytest.mark.parametrize("case", ["n1437", "n5000"])
+def test_knn_lattice_ties_large_k(ctx, case):
+    """Cases a randomized 400-case sweep found: an integer lattice of 1437 points
+    with k = 55 (rounds retried inline with radius_mult 1.1), and the 5000-point
+    lattice of the parity matrix at k = 60, 61 and 63.  Both forms must equal the
+    oracle."""
+    from test_gpu_parity_matrix import _cloud, _queries
+    if case == "n1437":
+        rng = np.random.default_rng(90063)
+        rng.integers(6), rng.uniform(0, np.log(100_000)), rng.integers(1, 65), rng.choice(6), rng.integers(10)
+        pts = _cloud("lattice", 1437, rng)
+        q = _queries(pts, 500, rng)
+        t, ks, cfg = ctx.build(pts, leaf_size=32), (55,), gprt_hip.default_config(radius_mult=1.1, max_rounds=3)
+    else:
+        rng = np.random.default_rng(1064)
+        pts = _cloud("lattice", 5000, rng)
+        q = _queries(pts, 1500, rng)
+        t, ks, cfg = ctx.build(pts, leaf_size=8), (60, 61, 63), gprt_hip.default_config()
+    try:
+        for k in ks:
+            ref = oracle.knn_brute(pts, q, k)
+            _check_knn(t.k_nn_plain(q, k), ref, label=f"lattice plain k={k}")
+            _check_knn(t.k_nn(q, k, cfg), ref, label=f"lattice k={k}")
+    finally:
+        t.close()
+
+
 def test_knn_duplicates(ctx):
     rng = np.random.default_rng(3)
     base = rng.random((200, 3), dtype=np.float32)
