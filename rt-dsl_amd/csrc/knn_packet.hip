@@ -173,7 +173,6 @@ void launch_packet_round(hipStream_t s, const RoundArgs& a, int cls, int grid, b
         case 3: GPRT_LP(32, GUARD) break;
         case 4: GPRT_LP(40, GUARD) break;
         case 5: GPRT_LP(48, GUARD) break;
-        case 6: GPRT_LP(64, GUARD) break;
         default: GPRT_LP(67, GUARD) break;
     }
 #undef GPRT_LP

@@ -262,7 +262,6 @@ int wide_grid(int cls, int ring, int grid_req) {
         case 3: return grid_cs<32, GUARD, 8>(grid_req);
         case 4: return grid_cs<40, GUARD, 8>(grid_req);
         case 5: return grid_cs<48, GUARD, 8>(grid_req);
-        case 6: return grid_cs<64, GUARD, 8>(grid_req);
         default: return grid_cs<67, GUARD, 8>(grid_req);
     }
 }
@@ -298,7 +297,6 @@ void launch_wide_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int
         case 3: launch_cs<32, GUARD, 8>(s, a, grid_req, count); break;
         case 4: launch_cs<40, GUARD, 8>(s, a, grid_req, count); break;
         case 5: launch_cs<48, GUARD, 8>(s, a, grid_req, count); break;
-        case 6: launch_cs<64, GUARD, 8>(s, a, grid_req, count); break;
         default: launch_cs<67, GUARD, 8>(s, a, grid_req, count); break;
     }
 }
