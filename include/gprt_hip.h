@@ -56,7 +56,11 @@ extern "C" {
 #define GPRT_ERR_HIP 3      /* HIP runtime failure */
 #define GPRT_ERR_ALLOC 4    /* device allocation failed */
 #define GPRT_ERR_CAPACITY 5 /* output capacity too small (range query) */
-#define GPRT_ERR_STATE 6    /* object in the wrong state */
+#define GPRT_ERR_STATE 6    /* object in the wrong state, or a device consistency guard tripped */
+
+/* largest k of every k-NN entry point (the reference's k_nn! register heap of
+ * 2k slots has no bound, gprt-macros/src/lib.rs:470-471) */
+#define GPRT_HIP_K_MAX 4096
 
 #define GPRT_PTR_HOST 0
 #define GPRT_PTR_DEVICE 1
@@ -208,15 +212,18 @@ GPRT_HIP_API int gprt_hip_tree_export(gprt_hip_tree* tree, uint64_t* keys, uint3
  * initial radius, Morton-ordered queries, FP32 traversal keeping a guard band
  * of candidates, FP64 re-rank by (d64, id), on-device compaction of unresolved
  * queries between rounds, queries whose band overflows re-ranked by the exact
- * kernel.  ids_out: nq*k; d2_out (may be NULL): nq*k binary64 d64 values. */
+ * kernel.  k = 1..GPRT_HIP_K_MAX; k > 45 runs one query per wavefront in
+ * binary64 with the candidate list spread over the wavefront's lanes.
+ * ids_out: nq*k; d2_out (may be NULL): nq*k binary64 d64 values. */
 GPRT_HIP_API int gprt_hip_knn(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq, int k,
                               const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int ptr_flags);
 /* Plain single-launch form: one query per lane in input order, FP64 distance
- * arithmetic throughout, initial radius +inf, no compaction, no graph. */
+ * arithmetic throughout, initial radius +inf, no compaction, no graph.
+ * k = 1..GPRT_HIP_K_MAX; k > 64 runs one query per wavefront. */
 GPRT_HIP_API int gprt_hip_knn_plain(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq,
                                     int k, uint32_t* ids_out, double* d2_out, int ptr_flags);
 /* Uneven batches: query set c (rows [query_offsets[c], query_offsets[c+1]))
- * searches cloud c of a batched tree with its own k = k_per_cloud[c] (1..64).
+ * searches cloud c of a batched tree with its own k = k_per_cloud[c] (1..GPRT_HIP_K_MAX).
  * Results of query set c start at out_offsets[c] = sum_{c'<c} nq_c' * k_c'.
  * query_offsets, k_per_cloud: host arrays. */
 GPRT_HIP_API int gprt_hip_knn_batched(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries,

@@ -474,7 +474,7 @@ static int knn_reserve(Ctx& c, const KnnPlan& p) {
     // HBM part of the per-lane traversal stacks of the persistent rounds
     if (p.cfg.traversal != GPRT_TRAV_WAVE) {
         size_t bytes = 0;
-        for (int cl = 0; cl < N_KCLASS; ++cl)
+        for (int cl = 0; cl < N_LANE_CLASSES; ++cl)
             if (p.class_size[cl] > 0) {
                 const size_t lanes = (size_t)256 * lane_round_grid(cl, p.cfg.lds_stack, p.cfg.grid_blocks,
                                                                    p.cfg.traversal);
@@ -561,6 +561,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     ra.qoff = p.d_qoff;
     ra.max_steps = T.n + 64;
     ra.spill = x->spill.as<int2>();
+    ra.poff = T.cloud_off.as<int64_t>();
     ExactArgs ea{};
     ea.nq = p.nq;
     ea.qs = qs4;
@@ -586,6 +587,16 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
         if (nclasses > 1)
             hipLaunchKernelGGL(class_init_kernel, dim3(1), dim3(64), 0, s, counters, (unsigned)size);
         ra.base = p.class_begin[cl];
+        if (cl == QWAVE_CLASS) {
+            // large k: one query per wavefront, exact in binary64, rounds retried
+            // inline; nothing falls back
+            ra.round = 0;
+            ra.rounds = R;
+            ra.last = R == 1;
+            ra.inline_rounds = 1;
+            GPRT_TRY(launch_qwave(s, ra, p.class_kmax[cl], grid, cfg.collect_counters != 0));
+            continue;
+        }
         ra.spill_depth = lane_spill_depth(cfg.stack_depth, cl, cfg.lds_stack);
         // later rounds inside the first launch (a lane retries its unresolved
         // query with the next radius) unless the schedule asks for one launch
@@ -708,6 +719,9 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_r
         c.stats.point_tests = (int64_t)v64[1];
         if (h[CNT_TOTAL - 1]) return fail(GPRT_ERR_INPUT, "queries contain NaN/Inf or |coordinate| > 1e18");
         if (h[CNT_ERROR]) return fail(GPRT_ERR_STATE, "traversal step guard tripped: corrupt tree");
+        if (h[CNT_BADID])
+            return fail(GPRT_ERR_STATE, "k-NN id guard tripped: " + std::to_string(h[CNT_BADID]) +
+                                            " queries re-ranked a candidate outside their cloud");
     }
     return GPRT_OK;
 }
@@ -718,7 +732,7 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_r
 extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
                             const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out, int flags) {
     if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
-    if (k < 1 || k > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
+    if (k < 1 || k > GPRT_HIP_K_MAX) return fail(GPRT_ERR_ARG, "k must be in [1, " + std::to_string(GPRT_HIP_K_MAX) + "]");
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
     if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "tree has several clouds: use gprt_hip_knn_batched");
@@ -777,7 +791,7 @@ extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float
                                   uint32_t* ids_out, double* d2_out, int flags) {
     if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
     GPRT_CTX_LOCK(h);
-    if (k < 1 || k > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
+    if (k < 1 || k > GPRT_HIP_K_MAX) return fail(GPRT_ERR_ARG, "k must be in [1, " + std::to_string(GPRT_HIP_K_MAX) + "]");
     if (nq < 0 || nq > 0x7FFFFFF0) return fail(GPRT_ERR_ARG, "bad nq");
     if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
     if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "plain form takes a single-cloud tree");
@@ -819,7 +833,27 @@ extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float
     GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
     launch_validate(s, d_q, 3 * nq, counters + CNT_TOTAL - 1);
     GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
-    GPRT_TRY(launch_exact(s, ea, k, (int)nq, count));
+    if (k <= 64) {
+        GPRT_TRY(launch_exact(s, ea, k, (int)nq, count));
+    } else {
+        // beyond the per-lane list: one query per wavefront, same FP64 order
+        RoundArgs ra{};
+        ra.nodes = ea.nodes;
+        ra.pts = ea.pts;
+        ra.roots = ea.roots;
+        ra.k = k;
+        ra.last = 1;
+        ra.rounds = 1;
+        ra.counters = counters;
+        ra.stats64 = ea.stats64;
+        ra.ids_out = d_ids;
+        ra.d2_out = d_d2;
+        ra.max_steps = ea.max_steps;
+        ra.poff = T.cloud_off.as<int64_t>();
+        ra.q_xyz = d_q;
+        ra.nq_plain = nq;
+        GPRT_TRY(launch_qwave(s, ra, k, 0, count));
+    }
     GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
@@ -842,7 +876,8 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     int64_t maxq = 0;
     for (int c = 0; c < nc; ++c) {
         if (query_offsets[c + 1] < query_offsets[c]) return fail(GPRT_ERR_ARG, "query_offsets must be non-decreasing");
-        if (k_per_cloud[c] < 1 || k_per_cloud[c] > 64) return fail(GPRT_ERR_ARG, "k must be in [1, 64]");
+        if (k_per_cloud[c] < 1 || k_per_cloud[c] > GPRT_HIP_K_MAX)
+            return fail(GPRT_ERR_ARG, "k must be in [1, " + std::to_string(GPRT_HIP_K_MAX) + "]");
         kmax = std::max(kmax, k_per_cloud[c]);
         outbase[c + 1] = outbase[c] + (query_offsets[c + 1] - query_offsets[c]) * k_per_cloud[c];
         maxq = std::max(maxq, query_offsets[c + 1] - query_offsets[c]);

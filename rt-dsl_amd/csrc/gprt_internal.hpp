@@ -134,6 +134,15 @@ __device__ __forceinline__ double d2d(float qx, float qy, float qz, float px, fl
     return (dx * dx + dy * dy) + dz * dz;
 }
 
+// largest float <= v for a squared distance v >= 0 (one ulp down when the
+// conversion rounded up): a box bound kept in FP32 never prunes more than its
+// FP64 value would
+__device__ __forceinline__ float f64_lower_f32(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = __int_as_float(__float_as_int(f) - 1);
+    return f;
+}
+
 // clamp to [lo, hi] for lo <= hi (every box here is built from points): the
 // median of the three, one v_med3_f32
 __device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }

@@ -330,12 +330,6 @@ struct ExactList {
     }
 };
 
-__device__ __forceinline__ float f64_lower_f32(double v) {
-    // v >= 0 (a squared distance): step one ulp down when rounding went up
-    float f = (float)v;
-    if ((double)f > v) f = __int_as_float(__float_as_int(f) - 1);
-    return f;
-}
 
 // limit: a known upper bound on the k-th d64 (+inf when none); points and
 // boxes beyond it cannot hold a result and are skipped

@@ -148,6 +148,10 @@ __device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
 
 // FP64 re-rank of the guard band in (d64, id) order and output of the first k.
 // Candidates are the list entries with 0 <= d32 <= tf (sentinels are -inf).
+// Guard: every candidate's sorted position must lie in its cloud's range
+// [poff[c], poff[c+1]) and its id below the cloud size; a violation (which no
+// correct walk produces) is counted in CNT_BADID, which fails the call, so a
+// wrong id is never returned silently.
 template <int C>
 __device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float* ld, const int* ls, float qx,
                                                  float qy, float qz, float tf, int g, int c, int k,
@@ -155,6 +159,8 @@ __device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float*
     double dd[C];
     uint32_t id[C];
     int reads = 0;
+    const int64_t lo = a.poff[c], hi = a.poff[c + 1];
+    bool bad = false;
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         const bool in = ld[j] >= 0.0f && ld[j] <= tf && ls[j] >= 0;
@@ -162,11 +168,14 @@ __device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float*
         id[j] = GPRT_ID_NONE;
         reads += in;
         if (in) {
-            const float4 p = pts[ls[j]];
+            const bool out = ls[j] < lo || ls[j] >= hi;
+            const float4 p = pts[out ? lo : ls[j]];
             dd[j] = d2d(qx, qy, qz, p.x, p.y, p.z);
             id[j] = (uint32_t)__float_as_int(p.w);
+            bad |= out || (int64_t)id[j] >= hi - lo;
         }
     }
+    if (bad) atomicAdd(&a.counters[CNT_BADID], 1u);
     bool swapped = true;
     while (swapped) {
         swapped = false;

@@ -11,6 +11,7 @@ constexpr int CNT_WORK = 20;      // [r]: work-queue head of round r
 constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
 constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
+constexpr int CNT_BADID = 43;     // re-ranked candidates outside their query's cloud (must stay 0)
 constexpr int CNT_TOTAL = 48;
 constexpr int CNT_STATS64 = 6;   // u64 statistics after the counters: visits, tests, re-rank reads,
                                  // (spare), wave steps, lane steps
@@ -49,6 +50,9 @@ struct RoundArgs {
     int inline_rounds;       // 1: an unresolved query retries with the next radius in the same launch
     int2* spill;             // stack entries below the LDS ring: [level][grid lanes] of (ref, d32)
     int spill_depth;         // levels of spill per lane
+    const int64_t* poff;     // per cloud first sorted point (cloud_off of the tree): the id guard
+    const float* q_xyz;      // wave-per-query plain form: the caller's xyz queries in input order
+    int64_t nq_plain;        // and their number
 };
 
 struct ExactArgs {
@@ -84,23 +88,20 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
                       int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc = nullptr,
                       int k_uniform = 1, int sort_classes = 0);
-// k classes: queries of one class share a candidate list of capacity C with
-// G = 3 guard slots, the smallest C of {8, 16, 24, 32, 40, 48, 64, 67} with
-// k + G <= C (registers hold the list, so a tighter class runs at a higher
-// occupancy).
+// k classes: queries of one class share a per-lane candidate list of
+// capacity C with G = 3 guard slots, the smallest C of {8, 16, 24, 32, 40, 48}
+// with k + G <= C (registers hold the list, so a tighter class runs at a
+// higher occupancy).  Larger k (class QWAVE_CLASS) run one query per wavefront
+// with the list spread over its lanes (knn_qwave.hip).
 constexpr int N_KCLASS = 8;  // the query sort key holds the class in 3 bits
+constexpr int N_LANE_CLASSES = 6;
+constexpr int QWAVE_CLASS = 6;
 __host__ __device__ inline int k_class_cap(int cls) {
-    return cls == 0 ? 8
-                    : (cls == 1 ? 16
-                                : (cls == 2 ? 24
-                                            : (cls == 3 ? 32 : (cls == 4 ? 40 : (cls == 5 ? 48 : 67)))));
+    return cls == 0 ? 8 : (cls == 1 ? 16 : (cls == 2 ? 24 : (cls == 3 ? 32 : (cls == 4 ? 40 : 48))));
 }
-// Class 6 (k = 46..64) runs the 67-slot list: with a 64-slot list a
-// randomized 400-case parity sweep found 5 cases with wrong ids (k = 47..57),
-// against none at 67 slots; class 7 is kept for the key layout.
 __host__ __device__ inline int k_class(int k) {
     int c = 0;
-    while (c + 1 < N_KCLASS && k + 3 > k_class_cap(c)) ++c;
+    while (c < N_LANE_CLASSES && k + 3 > k_class_cap(c)) ++c;
     return c;
 }
 size_t query_prep_scratch_bytes(int nclouds, int use_morton);
@@ -120,5 +121,9 @@ void launch_wide_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int
 void launch_packet_round(hipStream_t s, const RoundArgs& a, int cls, int grid, bool count);
 int launch_round(hipStream_t s, const RoundArgs& a, int cls, int grid, int trav, int ring, bool count);
 int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, bool count);
+// the wave-per-query kernel (knn_qwave.hip): over one k class of the batched
+// form, or over the caller's queries in input order when a.q_xyz is set
+int launch_qwave(hipStream_t s, const RoundArgs& a, int kmax, int grid_req, bool count);
+int qwave_lanes_m(int kmax);
 
 }  // namespace gprt

@@ -243,8 +243,7 @@ int lane_grid(int cls, int ring, int grid_req) {
         case 2: return grid_cs<24, GUARD, 8>(grid_req);
         case 3: return grid_cs<32, GUARD, 8>(grid_req);
         case 4: return grid_cs<40, GUARD, 8>(grid_req);
-        case 5: return grid_cs<48, GUARD, 8>(grid_req);
-        default: return grid_cs<67, GUARD, 8>(grid_req);
+        default: return grid_cs<48, GUARD, 8>(grid_req);  // class 5 (k class 6 never runs here)
     }
 }
 
@@ -278,8 +277,7 @@ void launch_lane_round(hipStream_t s, const RoundArgs& a, int cls, int ring, int
         case 2: launch_cs<24, GUARD, 8>(s, a, grid_req, count); break;
         case 3: launch_cs<32, GUARD, 8>(s, a, grid_req, count); break;
         case 4: launch_cs<40, GUARD, 8>(s, a, grid_req, count); break;
-        case 5: launch_cs<48, GUARD, 8>(s, a, grid_req, count); break;
-        default: launch_cs<67, GUARD, 8>(s, a, grid_req, count); break;
+        default: launch_cs<48, GUARD, 8>(s, a, grid_req, count); break;  // class 5
     }
 }
 

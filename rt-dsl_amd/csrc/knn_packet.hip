@@ -172,8 +172,7 @@ void launch_packet_round(hipStream_t s, const RoundArgs& a, int cls, int grid, b
         case 2: GPRT_LP(24, GUARD) break;
         case 3: GPRT_LP(32, GUARD) break;
         case 4: GPRT_LP(40, GUARD) break;
-        case 5: GPRT_LP(48, GUARD) break;
-        default: GPRT_LP(67, GUARD) break;
+        default: GPRT_LP(48, GUARD) break;  // class 5
     }
 #undef GPRT_LP
 }

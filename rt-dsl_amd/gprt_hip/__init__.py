@@ -23,6 +23,7 @@ OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
 PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30, BH_REFERENCE_LAW = 0, 1, 2, 4, 8, 16
 ID_NONE = 0xFFFFFFFF
+K_MAX = 4096  # GPRT_HIP_K_MAX: largest k of the k-NN entry points
 RADIUS_MAX, RADIUS_PERCENTILE, RADIUS_FIXED, RADIUS_INFINITE = 0, 1, 2, 3
 SAMPLER_LCG, SAMPLER_STRIDED = 0, 1
 TRAV_THREAD, TRAV_WAVE, TRAV_WIDE = 0, 1, 2

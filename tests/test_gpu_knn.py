@@ -113,10 +113,6 @@ def test_knn_ties_lattice(ctx):
     assert ctx.stats()["queries"] == len(q)
 
 
-@pytest.mark.xfail(reason="open defect: on integer lattices with exact distance ties at large k the batched "
-                          "form returns one id outside the cloud (with a plausible small distance) for a few "
-                          "queries when many fall back to the FP64 kernel; the plain form is exact. Cause not "
-                          "found.", strict=False)
 @pytest.mark.parametrize("case", ["n1437", "n5000"])
 def test_knn_lattice_ties_large_k(ctx, case):
     """Cases a randomized 400-case sweep found: an integer lattice of 1437 points
@@ -142,6 +138,44 @@ def test_knn_lattice_ties_large_k(ctx, case):
             _check_knn(t.k_nn(q, k, cfg), ref, label=f"lattice k={k}")
     finally:
         t.close()
+
+
+@pytest.mark.parametrize("k", [46, 64, 65, 100, 256, 1000, 4096])
+def test_knn_large_k(ctx, k):
+    """k beyond the per-lane list (k + 3 > 48): one query per wavefront with the
+    list spread over its lanes, in both forms, on a uniform cloud and on an
+    integer lattice full of exact distance ties.  Clouds of fewer than k points
+    pad with 0xFFFFFFFF."""
+    rng = np.random.default_rng(700 + k)
+    pts = rng.random((20_000, 3), dtype=np.float32)
+    q = np.concatenate([rng.random((300, 3), dtype=np.float32), pts[:100]])
+    g = np.arange(20, dtype=np.float32)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).copy()
+    ql = np.concatenate([lat[rng.integers(0, len(lat), 200)],
+                         lat[rng.integers(0, len(lat), 100)] + np.float32(0.5)]).astype(np.float32)
+    small = rng.random((k // 2 + 1, 3), dtype=np.float32)
+    for label, P, Q in (("uniform", pts, q), ("lattice", lat, ql), ("n<k", small, q[:50])):
+        ref = oracle.knn_brute(P, Q, k)
+        t = ctx.build(P, leaf_size=8)
+        try:
+            _check_knn(t.k_nn(Q, k), ref, label=f"{label} batched k={k}")
+            _check_knn(t.k_nn_plain(Q, k), ref, label=f"{label} plain k={k}")
+            if k <= 256:
+                cfg = gprt_hip.default_config(radius_heuristic=gprt_hip.RADIUS_FIXED, radius_param=1e-3,
+                                              max_rounds=6, radius_mult=2.0)
+                _check_knn(t.k_nn(Q, k, cfg), ref, label=f"{label} tiny radius k={k}")
+        finally:
+            t.close()
+
+
+def test_knn_k_limit(ctx):
+    pts = np.random.default_rng(0).random((100, 3), dtype=np.float32)
+    t = ctx.build(pts)
+    with pytest.raises(gprt_hip.GprtError):
+        t.k_nn(pts, gprt_hip.K_MAX + 1)
+    with pytest.raises(gprt_hip.GprtError):
+        t.k_nn_plain(pts, gprt_hip.K_MAX + 1)
+    t.close()
 
 
 def test_knn_duplicates(ctx):

@@ -64,11 +64,13 @@ CONFIGS = [
 CASES = []
 _kinds = ["uniform", "gauss", "clusters", "plane", "lattice", "line"]
 _sizes = [1, 7, 300, 5_000, 60_000]
-_OPEN = pytest.mark.xfail(reason="open defect: integer lattice with exact ties at large k (batched form)",
-                          strict=False)
 for _i in range(96):
-    _c = (_i, _kinds[_i % len(_kinds)], _sizes[(_i * 7) % len(_sizes)], KS[(_i * 5) % len(KS)], _i % len(CONFIGS))
-    CASES.append(pytest.param(*_c, marks=_OPEN) if _c[1] == "lattice" and _c[3] > 45 else _c)
+    CASES.append((_i, _kinds[_i % len(_kinds)], _sizes[(_i * 7) % len(_sizes)], KS[(_i * 5) % len(KS)],
+                  _i % len(CONFIGS)))
+# k beyond 64 (one query per wavefront): every distribution, sizes around k
+for _i in range(24):
+    CASES.append((200 + _i, _kinds[_i % len(_kinds)], [1, 90, 300, 5_000][_i % 4], [65, 100, 200, 777][(_i * 3) % 4],
+                  _i % len(CONFIGS)))
 
 
 @pytest.mark.parametrize("seed,kind,n,k,ci", CASES)
