@@ -113,8 +113,8 @@ typedef struct {
     int traversal;        /* GPRT_TRAV_*, default THREAD */
     int stack_depth;      /* traversal-stack entries per query, LDS + HBM spill (default 64);
                              a deeper walk is finished by the exact FP64 kernel */
-    int guard_slots;      /* spare candidate slots beyond k: read and written with schedules, but
-                             the candidate lists always carry 3 (the band re-rank needs them) */
+    int guard_slots;      /* spare candidate slots beyond k: must be 3 (the per-lane lists carry
+                             exactly 3; other values are rejected); kept in the schedule format */
     int sort_queries;     /* process queries in Morton order (default 1) */
     int use_graph;        /* replay the pipeline as a captured hipGraph (default 1) */
     int grid_blocks;      /* persistent grid size, 0 = automatic */

@@ -2,7 +2,9 @@
 
     python rt-dsl_amd/build.py [-j N]
 
-Objects go to rt-dsl_amd/_build/, the library to rt-dsl_amd/gprt_hip/.
+Objects go to rt-dsl_amd/_build/, the library to rt-dsl_amd/gprt_hip/, together
+with libgprt_host.so (the k-NN walk built for the host, for CPU tests).
+--sanitize builds both with host-side ASan + UBSan into rt-dsl_amd/_build_asan/.
 """
 import argparse
 import concurrent.futures as cf
@@ -31,10 +33,18 @@ SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hi
            "bh.hip", "legacy.hip", "autotune.hip", "codegen.hip", "io.hip"]
 
 
-def compile_one(src, build_dir=BUILD, defines=()):
+HOST_OUT = os.path.join(HERE, "gprt_hip", "libgprt_host.so")
+# host-side AddressSanitizer + UndefinedBehaviorSanitizer (GPU code is built as
+# usual; every -fsanitize applies to the host pass only)
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined", "-g"]
+SAN_DIR = os.path.join(HERE, "_build_asan")
+
+
+def compile_one(src, build_dir=BUILD, defines=(), extra=()):
     t0 = time.time()
     obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIPFLAGS + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + HIPFLAGS + list(extra) + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -42,25 +52,65 @@ def compile_one(src, build_dir=BUILD, defines=()):
     return obj, err + f"\n[{src}: {time.time() - t0:.0f} s]"
 
 
-def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=()):
+def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=(), sanitize=False):
     """Compile every translation unit and link the library.  `defines` are
     extra -D options (tuning constants of the kernels), for builds of the
-    same library into another `out` / `build_dir`."""
+    same library into another `out` / `build_dir`.  sanitize=True builds the
+    host code with ASan + UBSan (load it with the shared ASan runtime
+    preloaded)."""
     os.makedirs(build_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
+    extra = SANITIZE if sanitize else []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(lambda src: compile_one(src, build_dir, defines), srcs))
+        results = list(ex.map(lambda src: compile_one(src, build_dir, defines, extra), srcs))
     for obj, err in results:
         if err.strip() and verbose:
             print(err, file=sys.stderr)
     tmp = out + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in results] + ["-lhiprtc"]
+    link_san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libsan"] \
+        if sanitize else []
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + link_san + [o for o, _ in results] + \
+        ["-lhiprtc"]
     subprocess.check_call(cmd)
     os.replace(tmp, out)
     if verbose:
         print("built", out)
     return out
+
+
+def build_host(out=HOST_OUT, sanitize=False):
+    """libgprt_host.so: the per-lane k-NN walk and the LBVH layout compiled for
+    the host only (host_walk.cpp), for CPU tests of the device code's logic."""
+    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+           "-fvisibility=hidden", "-Wall"]
+    if sanitize:
+        cmd += ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Xarch_host",
+                "-fno-sanitize-recover=undefined", "-shared-libsan", "-g"]
+    tmp = out + ".tmp"
+    r = subprocess.run(cmd + [os.path.join(CSRC, "host_walk.cpp"), "-o", tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n" + r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
+def asan_runtime():
+    """Path of the shared ASan runtime of the HIP toolchain (to LD_PRELOAD)."""
+    clang = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "bin", "clang++")
+    if not os.path.exists(clang):
+        clang = "/opt/rocm/llvm/bin/clang++"
+    return subprocess.run([clang, "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True,
+                          text=True).stdout.strip()
+
+
+def build_sanitized(jobs=None):
+    """The library and the host walk with ASan + UBSan on the host side, into
+    rt-dsl_amd/_build_asan/."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    lib = build(jobs, verbose=False, out=os.path.join(SAN_DIR, "libgprt_hip.so"), build_dir=SAN_DIR, sanitize=True)
+    host = build_host(out=os.path.join(SAN_DIR, "libgprt_host.so"), sanitize=True)
+    return lib, host
 
 
 if __name__ == "__main__":
@@ -69,5 +119,10 @@ if __name__ == "__main__":
     ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor define")
     ap.add_argument("--out", default=OUT)
     ap.add_argument("--build-dir", default=BUILD)
+    ap.add_argument("--sanitize", action="store_true", help="ASan + UBSan host build into _build_asan/")
     a = ap.parse_args()
-    build(a.j, out=a.out, build_dir=a.build_dir, defines=a.defines)
+    if a.sanitize:
+        print(build_sanitized(a.j))
+    else:
+        build(a.j, out=a.out, build_dir=a.build_dir, defines=a.defines)
+        build_host()

@@ -494,6 +494,7 @@ static int check_config(const gprt_hip_config& cf) {
     if (cf.traversal < GPRT_TRAV_THREAD || cf.traversal > GPRT_TRAV_WIDE) return fail(GPRT_ERR_ARG, "bad traversal");
     if (cf.stack_depth < 1 || cf.stack_depth > 4096) return fail(GPRT_ERR_ARG, "stack_depth must be in [1, 4096]");
     if (cf.lds_stack < 1 || cf.lds_stack > 32) return fail(GPRT_ERR_ARG, "lds_stack must be in [1, 32]");
+    if (cf.guard_slots != 3) return fail(GPRT_ERR_ARG, "guard_slots must be 3 (the lists carry exactly 3)");
     if (cf.grid_blocks < 0 || cf.grid_blocks > (1 << 20)) return fail(GPRT_ERR_ARG, "bad grid_blocks");
     if (cf.sampler != GPRT_SAMPLER_LCG && cf.sampler != GPRT_SAMPLER_STRIDED) return fail(GPRT_ERR_ARG, "bad sampler");
     if (cf.radius_heuristic < GPRT_RADIUS_MAX || cf.radius_heuristic > GPRT_RADIUS_INFINITE)

@@ -120,14 +120,30 @@ inline std::array<DevBuf*, TREE_BUFS> tree_bufs(Tree& T) {
 struct Ctx;
 
 // ---------------------------------------------------------------------------
-// numeric helpers
+// numeric helpers.  The ones the per-lane k-NN walk uses are __host__
+// __device__, so the walk also compiles for the host (a CPU build of the same
+// code, with bounds-checked memory, for tests).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float d2f(float qx, float qy, float qz, float px, float py, float pz) {
+#define GPRT_HD __host__ __device__ __forceinline__
+
+GPRT_HD int as_int(float f) { return __builtin_bit_cast(int, f); }
+GPRT_HD float as_float(int i) { return __builtin_bit_cast(float, i); }
+
+// median of three (one v_med3_f32 on the device)
+GPRT_HD float med3f(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(a, b, c);
+#else
+    return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+#endif
+}
+
+GPRT_HD float d2f(float qx, float qy, float qz, float px, float py, float pz) {
     float dx = qx - px, dy = qy - py, dz = qz - pz;
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-__device__ __forceinline__ double d2d(float qx, float qy, float qz, float px, float py, float pz) {
+GPRT_HD double d2d(float qx, float qy, float qz, float px, float py, float pz) {
     double dx = (double)qx - (double)px;
     double dy = (double)qy - (double)py;
     double dz = (double)qz - (double)pz;
@@ -145,13 +161,13 @@ __device__ __forceinline__ float f64_lower_f32(double v) {
 
 // clamp to [lo, hi] for lo <= hi (every box here is built from points): the
 // median of the three, one v_med3_f32
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
+GPRT_HD float clampf(float v, float lo, float hi) { return med3f(v, lo, hi); }
 
-__device__ __forceinline__ float box2f(float qx, float qy, float qz, const float4& lo, const float4& hi) {
+GPRT_HD float box2f(float qx, float qy, float qz, const float4& lo, const float4& hi) {
     return d2f(qx, qy, qz, clampf(qx, lo.x, hi.x), clampf(qy, lo.y, hi.y), clampf(qz, lo.z, hi.z));
 }
 
-__device__ __forceinline__ double box2d(float qx, float qy, float qz, const float4& lo, const float4& hi) {
+GPRT_HD double box2d(float qx, float qy, float qz, const float4& lo, const float4& hi) {
     return d2d(qx, qy, qz, clampf(qx, lo.x, hi.x), clampf(qy, lo.y, hi.y), clampf(qz, lo.z, hi.z));
 }
 
@@ -162,7 +178,7 @@ __device__ __forceinline__ double box2d(float qx, float qy, float qz, const floa
 // additive term covers flushed subnormal squares (each < FLT_MIN).
 #define GPRT_BAND 1.0000019073486328125f
 #define GPRT_BAND_ABS 1e-36f
-__device__ __forceinline__ float band(float t) { return __builtin_fmaf(t, GPRT_BAND, GPRT_BAND_ABS); }
+GPRT_HD float band(float t) { return __builtin_fmaf(t, GPRT_BAND, GPRT_BAND_ABS); }
 
 // wave helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -1,0 +1,315 @@
+// host_walk.cpp -- a CPU build of the per-lane FP32 k-NN walk (lane_walk.hpp),
+// for tests: libgprt_host.so, compiled host-only (no GPU code, no GPU needed).
+//
+// It rebuilds the LBVH exactly as lbvh.hip lays it out (same keys, stable key
+// order, Karras hierarchy with the index tie-break, leaf collapse, min/max
+// boxes), then runs lane_walk and rerank_sort for each query with every read
+// and write of the points, nodes, LDS ring and spill column bounds-checked.
+// The list capacity C is a parameter, so capacities the GPU build no longer
+// uses can be exercised too.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lane_walk.hpp"
+
+namespace {
+
+using namespace gprt;
+
+// ---------------------------------------------------------------------------
+// LBVH, host restatement of lbvh.hip (bbox_reduce, key_box, morton_keys with b
+// bits per axis, stable sort, gather, karras, refit)
+// ---------------------------------------------------------------------------
+struct HostTree {
+    int n = 0, leaf = 8, axis_bits = 21;
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> perm;
+    std::vector<float4> pts4;
+    std::vector<Node> nodes;
+    int2 root{0, 0};
+};
+
+uint32_t quantize_b(float p, float mn, float ext, float scale, uint32_t qmax) {
+    const float t = (p - mn) / ext * scale;
+    const uint32_t u = !(t > 0.0f) ? 0u : (t >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)t);
+    return u < qmax ? u : qmax;
+}
+
+uint64_t spread21(uint32_t v) {  // bit i of v to bit 3i
+    uint64_t x = 0;
+    for (int i = 0; i < 21; ++i) x |= (uint64_t)((v >> i) & 1u) << (3 * i);
+    return x;
+}
+
+int delta(const std::vector<uint64_t>& k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    if (k[i] == k[j]) return 64 + __builtin_clz((unsigned)(i ^ j));
+    return __builtin_clzll(k[i] ^ k[j]);
+}
+
+void build_tree(const float* p, int n, int leaf, HostTree& T) {
+    T.n = n;
+    T.leaf = leaf;
+    const int b = 21;  // one cloud: 21 bits per axis
+    float mn[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+    float mx[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], p[3 * i + a]);
+            mx[a] = fmaxf(mx[a], p[3 * i + a]);
+        }
+    const float e = fmaxf(mx[0] - mn[0], fmaxf(mx[1] - mn[1], mx[2] - mn[2]));
+    float ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = fmaxf((mn[a] + e) - mn[a], 1e-6f);
+    const uint32_t qmax = (1u << b) - 1u;
+    const float scale = (float)qmax;
+    T.keys.resize(n);
+    T.perm.resize(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t m = 0;
+        for (int a = 0; a < 3; ++a) m = (m << 1) | spread21(quantize_b(p[3 * i + a], mn[a], ext[a], scale, qmax));
+        T.keys[i] = m;
+        T.perm[i] = (uint32_t)i;
+    }
+    std::stable_sort(T.perm.begin(), T.perm.end(), [&](uint32_t x, uint32_t y) { return T.keys[x] < T.keys[y]; });
+    std::vector<uint64_t> sk(n);
+    T.pts4.resize(std::max(n, 1));
+    for (int i = 0; i < n; ++i) {
+        const uint32_t s = T.perm[i];
+        sk[i] = T.keys[s];
+        T.pts4[i] = make_float4(p[3 * s], p[3 * s + 1], p[3 * s + 2], as_float((int)s));
+    }
+    T.keys = sk;
+    // a cloud of at most `leaf` points is one bucket (its nodes are still built)
+    T.root = n > leaf ? make_int2(0, n) : make_int2(~0, n);
+    if (n < 2) return;
+    T.nodes.assign(n - 1, Node{});
+    std::vector<int2> ranges(n - 1);
+    std::vector<int> parent(2 * n - 1, -1);
+    for (int i = 0; i < n - 1; ++i) {
+        const int dr = delta(T.keys, n, i, i + 1), dl = delta(T.keys, n, i, i - 1);
+        const int d = dr > dl ? 1 : -1;
+        const int dmin = std::min(dr, dl);
+        int lmax = 2;
+        while (delta(T.keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(T.keys, n, i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(T.keys, n, i, j);
+        int s = 0;
+        for (int div = 2;; div <<= 1) {
+            const int t = (l + div - 1) / div;
+            if (delta(T.keys, n, i, i + (s + t) * d) > dnode) s += t;
+            if (t <= 1) break;
+        }
+        const int gamma = i + s * d + std::min(d, 0);
+        const int lo = std::min(i, j), hi = std::max(i, j);
+        ranges[i] = make_int2(lo, hi);
+        const int cA = gamma - lo + 1, cB = hi - gamma;
+        const int rA = (lo == gamma || cA <= leaf) ? ~lo : gamma;
+        const int rB = (hi == gamma + 1 || cB <= leaf) ? ~(gamma + 1) : gamma + 1;
+        Node& nd = T.nodes[i];
+        nd.alo.w = as_float(rA);
+        nd.ahi.w = as_float(cA);
+        nd.blo.w = as_float(rB);
+        nd.bhi.w = as_float(cB);
+        parent[(lo == gamma ? (n - 1) : 0) + gamma] = (i << 1) | 0;
+        parent[(hi == gamma + 1 ? (n - 1) : 0) + gamma + 1] = (i << 1) | 1;
+    }
+    // boxes: every child box is the min/max of the points of its range
+    for (int i = 0; i < n - 1; ++i) {
+        Node& nd = T.nodes[i];
+        const int lo = ranges[i].x, hi = ranges[i].y, cA = as_int(nd.ahi.w);
+        for (int side = 0; side < 2; ++side) {
+            const int f = side ? lo + cA : lo, l = side ? hi : lo + cA - 1;
+            float4 bl = make_float4(3.402823466e38f, 3.402823466e38f, 3.402823466e38f, 0.f);
+            float4 bh = make_float4(-3.402823466e38f, -3.402823466e38f, -3.402823466e38f, 0.f);
+            for (int t = f; t <= l; ++t) {
+                const float4 q = T.pts4[t];
+                bl.x = fminf(bl.x, q.x); bl.y = fminf(bl.y, q.y); bl.z = fminf(bl.z, q.z);
+                bh.x = fmaxf(bh.x, q.x); bh.y = fmaxf(bh.y, q.y); bh.z = fmaxf(bh.z, q.z);
+            }
+            float4& L = side ? nd.blo : nd.alo;
+            float4& H = side ? nd.bhi : nd.ahi;
+            L.x = bl.x; L.y = bl.y; L.z = bl.z;
+            H.x = bh.x; H.y = bh.y; H.z = bh.z;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bounds-checked memory of one lane
+// ---------------------------------------------------------------------------
+[[noreturn]] void oob(const char* what, long long idx, long long size) {
+    throw std::out_of_range(std::string(what) + " index " + std::to_string(idx) + " outside [0, " +
+                            std::to_string(size) + ")");
+}
+
+template <int S>
+struct LaneMemHost {
+    const HostTree* T;
+    int* ring_r;
+    float* ring_dd;
+    std::vector<int2>* spill;
+    int spill_depth;
+    void fetch4(bool scanning, int first, int node, int last, float4* v) const {
+        for (int u = 0; u < 4; ++u) {
+            const int o = std::min(u, last);
+            if (scanning) {
+                const long long i = (long long)first + o;
+                if (i < 0 || i >= T->n) oob("point", i, T->n);
+                v[u] = T->pts4[i];
+            } else {
+                if (node < 0 || node >= (int)T->nodes.size()) oob("node", node, (long long)T->nodes.size());
+                const Node& nd = T->nodes[node];
+                v[u] = o == 0 ? nd.alo : (o == 1 ? nd.ahi : (o == 2 ? nd.blo : nd.bhi));
+            }
+        }
+    }
+    int ring_ref(int j) const {
+        if (j < 0 || j >= S) oob("ring", j, S);
+        return ring_r[j];
+    }
+    float ring_d(int j) const {
+        if (j < 0 || j >= S) oob("ring", j, S);
+        return ring_dd[j];
+    }
+    void ring_put(int j, int r, float d) const {
+        if (j < 0 || j >= S) oob("ring", j, S);
+        ring_r[j] = r;
+        ring_dd[j] = d;
+    }
+    int2 spill_get(int l) const {
+        if (l < 0 || l >= spill_depth) oob("spill", l, spill_depth);
+        return (*spill)[l];
+    }
+    void spill_put(int l, int2 e) const {
+        if (l < 0 || l >= spill_depth) oob("spill", l, spill_depth);
+        (*spill)[l] = e;
+    }
+};
+
+struct PtHost {
+    const HostTree* T;
+    float4 operator()(int64_t i) const {
+        if (i < 0 || i >= T->n) oob("re-rank point", i, T->n);
+        return T->pts4[i];
+    }
+};
+
+// One query through the batched form's per-lane path: inline rounds with radius
+// R0 * mult^r (the last unbounded), then the guard-band test and the re-rank.
+// status: 0 written, 1 handed to the exact kernel (band overflow or stack), 2
+// the id guard tripped.
+template <int C>
+int run_query(const HostTree& T, const float* q, int k, float R0, float mult, int rounds, int stack_depth,
+              uint32_t* ids, int64_t* stats) {
+    constexpr int G = GUARD, S = 8;
+    if (k + G > C) throw std::invalid_argument("k + 3 exceeds the list capacity");
+    int ring_r[S] = {0};
+    float ring_d[S] = {0};
+    std::vector<int2> spill(std::max(stack_depth - S, 0));
+    const LaneMemHost<S> mem{&T, ring_r, ring_d, &spill, (int)spill.size()};
+    const int max_depth = S + (int)spill.size();
+    F32Query<C, G> Q;
+    bool overflow = false, resolved = false;
+    float R = R0;
+    uint64_t visits = 0, tests = 0;
+    for (int round = 0;; ++round) {
+        const bool last = round >= rounds - 1;
+        const float T0 = last ? INFINITY : R * R;
+        Q.start(q[0], q[1], q[2], T0, k, true);
+        overflow = false;
+        if (T.root.y > 0) stats[2] += lane_walk<C, G, S, true>(Q, T.root, mem, T.n + 64, max_depth, overflow, visits,
+                                                               tests);
+        resolved = last || (Q.kth() < INFINITY && Q.kth() <= T0);
+        if (resolved || overflow) break;
+        R *= mult;
+    }
+    stats[0] += (int64_t)visits;
+    stats[1] += (int64_t)tests;
+    const float kth = Q.kth();
+    const float tf = band(kth);
+    if (!resolved || overflow || Q.ev_min <= tf) return 1;
+    double dd[C];
+    uint32_t id[C];
+    bool bad = false;
+    rerank_sort<C>(Q.L.d, Q.L.s, Q.qx, Q.qy, Q.qz, tf, 0, T.n, PtHost{&T}, dd, id, bad);
+    for (int j = 0; j < k; ++j) ids[j] = id[j];
+    return bad ? 2 : 0;
+}
+
+thread_local std::string g_msg;
+
+}  // namespace
+
+extern "C" {
+
+__attribute__((visibility("default"))) const char* gprt_host_last_error(void) { return g_msg.c_str(); }
+
+// The LBVH of one cloud as lbvh.hip builds it: keys (n, sorted), perm (n),
+// nodes (16 floats per internal node), root (2 ints).  Any output may be NULL.
+__attribute__((visibility("default"))) int gprt_host_build_tree(const float* pts, int64_t n, int leaf_size,
+                                                                uint64_t* keys, uint32_t* perm, float* nodes,
+                                                                int32_t* root) {
+    if (n < 0 || n > 0x7FFFFFF0 || leaf_size < 1 || leaf_size > 32) return 1;
+    HostTree T;
+    build_tree(pts, (int)n, leaf_size, T);
+    if (keys) std::memcpy(keys, T.keys.data(), 8 * (size_t)n);
+    if (perm) std::memcpy(perm, T.perm.data(), 4 * (size_t)n);
+    if (nodes && !T.nodes.empty()) std::memcpy(nodes, T.nodes.data(), sizeof(Node) * T.nodes.size());
+    if (root) {
+        root[0] = T.root.x;
+        root[1] = T.root.y;
+    }
+    return 0;
+}
+
+// The batched form's per-lane path for each query (see run_query) with a list
+// of `cap` slots (8, 16, 24, 32, 40, 48, 64 or 67).  ids_out: nq*k (rows of
+// status != 0 are left untouched); status_out: nq; stats_out[3]: node visits,
+// point tests, walk trips.  Returns 0, or 1 with gprt_host_last_error() naming
+// the first out-of-bounds access or bad argument.
+__attribute__((visibility("default"))) int gprt_host_lane_knn(const float* pts, int64_t n, int leaf_size,
+                                                              const float* qs, int64_t nq, int k, int cap,
+                                                              float radius0, float mult, int rounds, int stack_depth,
+                                                              uint32_t* ids_out, int32_t* status_out,
+                                                              int64_t* stats_out) {
+    g_msg.clear();
+    try {
+        if (n < 0 || n > 0x7FFFFFF0 || nq < 0 || k < 1 || rounds < 1 || stack_depth < 8 || !(mult > 1.0f))
+            throw std::invalid_argument("bad argument");
+        HostTree T;
+        build_tree(pts, (int)n, leaf_size, T);
+        int64_t st[3] = {0, 0, 0};
+        for (int64_t i = 0; i < nq; ++i) {
+            const float* q = qs + 3 * i;
+            uint32_t* o = ids_out + i * k;
+            int r;
+            switch (cap) {
+                case 8: r = run_query<8>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 16: r = run_query<16>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 24: r = run_query<24>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 32: r = run_query<32>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 40: r = run_query<40>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 48: r = run_query<48>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 64: r = run_query<64>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                case 67: r = run_query<67>(T, q, k, radius0, mult, rounds, stack_depth, o, st); break;
+                default: throw std::invalid_argument("cap must be 8, 16, 24, 32, 40, 48, 64 or 67");
+            }
+            status_out[i] = r;
+        }
+        if (stats_out) std::memcpy(stats_out, st, sizeof(st));
+    } catch (const std::exception& e) {
+        g_msg = e.what();
+        return 1;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -6,11 +6,9 @@
 
 #include "gprt_internal.hpp"
 #include "knn_internal.hpp"
+#include "lane_walk.hpp"
 
 namespace gprt {
-
-// spare candidate slots beyond k (the guard band)
-constexpr int GUARD = 3;
 
 // LDS ring sizes the per-lane kernels are built for
 __host__ __device__ inline int ring_size(int ring) { return ring <= 8 ? 8 : (ring <= 16 ? 16 : 32); }
@@ -36,79 +34,6 @@ inline int resident_grid(F kernel, std::atomic<int>* cache) {
     slot.store(g, std::memory_order_relaxed);
     return g;
 }
-
-// ---------------------------------------------------------------------------
-// FP32 candidate list (ascending d32), capacity C, in registers.
-//
-// The list holds C = S + k + G slots: the first S = C - k - G are -inf
-// sentinels that never move, then the k nearest candidates, then G guard slots
-// for candidates inside the band of the k-th.  The k-th candidate therefore sits
-// at the fixed index C - 1 - G whatever k is, so no slot is ever addressed with
-// a runtime index (which would push the whole list to scratch memory).
-// ---------------------------------------------------------------------------
-template <int C>
-struct CandList {
-    float d[C];
-    int s[C];
-    __device__ __forceinline__ void init(int sentinels) {
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            d[j] = j < sentinels ? -INFINITY : INFINITY;
-            s[j] = -1;
-        }
-    }
-    // insert (dd, ss), dd >= 0; returns the value pushed out of the last slot
-    // (INF if that slot was empty).  Caller guarantees dd < d[C-1].
-    // The new slot j value of a sorted list is the median of the old d[j-1],
-    // d[j] and dd (one v_med3_f32); ids follow with two selects.  An entry
-    // equal to existing ones goes after them.
-    __device__ __forceinline__ float insert(float dd, int ss) {
-        const float ev = d[C - 1];
-#pragma unroll
-        for (int j = C - 1; j >= 1; --j) {
-            const bool shift = d[j - 1] > dd;
-            const bool here = d[j] > dd;
-            s[j] = shift ? s[j - 1] : (here ? ss : s[j]);
-            d[j] = __builtin_amdgcn_fmed3f(d[j - 1], d[j], dd);
-        }
-        s[0] = d[0] > dd ? ss : s[0];
-        d[0] = fminf(d[0], dd);
-        return ev;
-    }
-};
-
-template <int C, int G>
-struct F32Query {
-    float qx, qy, qz;
-    float T0;      // round bound (radius^2), INF in the unbounded round
-    float T, tau;  // current k-th bound and guard threshold
-    float ev_min;  // smallest d32 pushed out of a full list
-    CandList<C> L;
-
-    __device__ __forceinline__ void start(float x, float y, float z, float t0, int k, bool active) {
-        qx = x; qy = y; qz = z;
-        T0 = t0;
-        T = t0;
-        tau = active ? band(t0) : -1.0f;  // inactive lanes accept nothing
-        ev_min = INFINITY;
-        L.init(C - k - G);
-    }
-    // One branch per point (inside the band or not).  Within the band a point
-    // that does not beat the last slot (rare: the guard slots are full) is
-    // inserted as +inf, which leaves the sorted list unchanged, rather than
-    // taking a second branch.
-    __device__ __forceinline__ void visit_point(float4 p, int s) {
-        const float dd = d2f(qx, qy, qz, p.x, p.y, p.z);
-        if (dd <= tau) {
-            const bool ins = dd < L.d[C - 1];
-            const float ev = L.insert(ins ? dd : INFINITY, s);
-            ev_min = fminf(ev_min, ins ? ev : dd);
-            T = fminf(T0, L.d[C - 1 - G]);
-            tau = band(T);
-        }
-    }
-    __device__ __forceinline__ float kth() const { return L.d[C - 1 - G]; }
-};
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -147,47 +72,22 @@ __device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
 }
 
 // FP64 re-rank of the guard band in (d64, id) order and output of the first k.
-// Candidates are the list entries with 0 <= d32 <= tf (sentinels are -inf).
-// Guard: every candidate's sorted position must lie in its cloud's range
-// [poff[c], poff[c+1]) and its id below the cloud size; a violation (which no
-// correct walk produces) is counted in CNT_BADID, which fails the call, so a
-// wrong id is never returned silently.
+// A candidate outside the query's cloud (which no correct walk produces) is
+// counted in CNT_BADID, which fails the call, so a wrong id is never returned
+// silently.
+struct PtLoad {
+    const float4* __restrict__ pts;
+    __device__ __forceinline__ float4 operator()(int64_t i) const { return pts[i]; }
+};
 template <int C>
 __device__ __forceinline__ int rerank_and_write(const RoundArgs& a, const float* ld, const int* ls, float qx,
                                                  float qy, float qz, float tf, int g, int c, int k,
                                                  const float4* __restrict__ pts) {
     double dd[C];
     uint32_t id[C];
-    int reads = 0;
-    const int64_t lo = a.poff[c], hi = a.poff[c + 1];
     bool bad = false;
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-        const bool in = ld[j] >= 0.0f && ld[j] <= tf && ls[j] >= 0;
-        dd[j] = INFINITY;
-        id[j] = GPRT_ID_NONE;
-        reads += in;
-        if (in) {
-            const bool out = ls[j] < lo || ls[j] >= hi;
-            const float4 p = pts[out ? lo : ls[j]];
-            dd[j] = d2d(qx, qy, qz, p.x, p.y, p.z);
-            id[j] = (uint32_t)__float_as_int(p.w);
-            bad |= out || (int64_t)id[j] >= hi - lo;
-        }
-    }
+    const int reads = rerank_sort<C>(ld, ls, qx, qy, qz, tf, a.poff[c], a.poff[c + 1], PtLoad{pts}, dd, id, bad);
     if (bad) atomicAdd(&a.counters[CNT_BADID], 1u);
-    bool swapped = true;
-    while (swapped) {
-        swapped = false;
-#pragma unroll
-        for (int j = 0; j + 1 < C; ++j) {
-            const bool gt = dd[j] > dd[j + 1] || (dd[j] == dd[j + 1] && id[j] > id[j + 1]);
-            const double lo = gt ? dd[j + 1] : dd[j], hi = gt ? dd[j] : dd[j + 1];
-            const uint32_t li = gt ? id[j + 1] : id[j], hiid = gt ? id[j] : id[j + 1];
-            dd[j] = lo; dd[j + 1] = hi; id[j] = li; id[j + 1] = hiid;
-            swapped |= gt;
-        }
-    }
     const int64_t ob = a.out_base ? a.out_base[c] + (int64_t)(g - a.qoff[c]) * k : (int64_t)g * k;
 #pragma unroll
     for (int j = 0; j < C; ++j)

@@ -125,5 +125,8 @@ int launch_exact(hipStream_t s, const ExactArgs& a, int kmax, int grid_threads, 
 // form, or over the caller's queries in input order when a.q_xyz is set
 int launch_qwave(hipStream_t s, const RoundArgs& a, int kmax, int grid_req, bool count);
 int qwave_lanes_m(int kmax);
+// the register-heap program for k > 64 (knn_qwave.hip)
+int launch_heap_wave(hipStream_t s, const Node* nodes, const float4* pts, int2 root, const float4* q4, int nq, int k,
+                     unsigned* heap, unsigned* lens, int max_steps, unsigned* err);
 
 }  // namespace gprt

@@ -14,6 +14,34 @@ namespace gprt {
 #endif
 
 
+// The lane's memory for lane_walk: nodes and points in HBM, its column of the
+// block's LDS ring ([slot][lane], so lanes hit distinct banks) and its column
+// of the HBM spill array ([level][grid lanes]).
+#define GPRT_LDS __attribute__((address_space(3)))
+template <int S>
+struct LaneMemDev {
+    const float4* __restrict__ nodes4;
+    const float4* __restrict__ pts;
+    GPRT_LDS int (*sref)[256];      // typed as LDS so every ring access is a ds_read / ds_write
+    GPRT_LDS float (*sdist)[256];
+    int tid;
+    int2* __restrict__ spill;
+    size_t gstride;
+    __device__ __forceinline__ void fetch4(bool scanning, int first, int node, int last, float4* v) const {
+        const float4* src = scanning ? pts + first : nodes4 + 4 * (size_t)node;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = src[min(u, last)];
+    }
+    __device__ __forceinline__ int ring_ref(int j) const { return sref[j][tid]; }
+    __device__ __forceinline__ float ring_d(int j) const { return sdist[j][tid]; }
+    __device__ __forceinline__ void ring_put(int j, int r, float d) const {
+        sref[j][tid] = r;
+        sdist[j][tid] = d;
+    }
+    __device__ __forceinline__ int2 spill_get(int level) const { return spill[level * gstride]; }
+    __device__ __forceinline__ void spill_put(int level, int2 e) const { spill[level * gstride] = e; }
+};
+
 // ---------------------------------------------------------------------------
 // The same round with one independent walk per lane (closest child first).
 //
@@ -46,10 +74,14 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
     const int tid = threadIdx.x, lane = tid & 63;
     const unsigned n_in = a.counters[CNT_ACTIVE + a.round];
     if (n_in == 0) return;
-    const float4* __restrict__ nodes4 = (const float4*)a.nodes;
     const float4* __restrict__ pts = a.pts;
-    const size_t gstride = (size_t)gridDim.x * 256;
-    int2* __restrict__ spill = a.spill + (size_t)blockIdx.x * 256 + tid;
+    const LaneMemDev<S> mem{(const float4*)a.nodes,
+                            pts,
+                            (GPRT_LDS int(*)[256])sref,
+                            (GPRT_LDS float(*)[256])sdist,
+                            tid,
+                            a.spill + (size_t)blockIdx.x * 256 + tid,
+                            (size_t)gridDim.x * 256};
     const int max_depth = S + a.spill_depth;
     uint64_t visits = 0, tests = 0, reranks = 0, lane_steps = 0, wave_steps = 0;
     while (true) {
@@ -82,93 +114,8 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
             T0 = last ? INFINITY : R * R;
             Q.start(q.x, q.y, q.z, T0, k, true);
             overflow = false;
-            if (root.y > 0) {
-                int sp = 0, lo = 0;  // stack depth; entries [0, lo) are in HBM
-                int steps = 0, trips = 0;
-                // walk state: the node to load next, or a leaf range [sf, sf + sm)
-                // whose points from sf + st on are still to be scanned
-                bool scanning = root.x < 0;
-                int node = root.x;
-                int sf = scanning ? ~root.x : 0, sm = scanning ? root.y : 0, st = 0;
-                // children of the last node: box distances and refs (negative
-                // refs are leaf buckets, never descended into)
-                float cdA = INFINITY, cdB = INFINITY;
-                int crA = -1, crB = -1;
-                while (true) {
-                    ++trips;
-                    const int avail = scanning ? sm - st : 4;
-                    float4 v[4];
-                    // four loads without branches (so they issue back to back
-                    // whichever lanes scan): past the end of a leaf range the index
-                    // is clamped to its last point, which is loaded but not visited
-                    const float4* src = scanning ? pts + (sf + st) : nodes4 + 4 * (size_t)node;
-                    const int last = scanning ? avail - 1 : 3;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) v[u] = src[min(u, last)];
-                    if (!scanning) {
-                        if (++steps > a.max_steps) { overflow = true; break; }
-                        if (COUNT) ++visits;
-                        // v = child A's box (lo, hi), child B's box (lo, hi)
-                        cdA = box2f(Q.qx, Q.qy, Q.qz, v[0], v[1]);
-                        cdB = box2f(Q.qx, Q.qy, Q.qz, v[2], v[3]);
-                        crA = __float_as_int(v[0].w);
-                        crB = __float_as_int(v[2].w);
-                        const bool sA = crA < 0 && cdA <= Q.tau, sB = crB < 0 && cdB <= Q.tau;
-                        sf = sA ? ~crA : ~crB;
-                        sm = (sA ? __float_as_int(v[1].w) : 0) + (sB ? __float_as_int(v[3].w) : 0);
-                        st = 0;
-                        if (sm > 0) {
-                            if (COUNT) tests += sm;
-                            scanning = true;
-                            continue;
-                        }
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (u < avail) Q.visit_point(v[u], sf + st + u);
-                        st += 4;
-                        if (st < sm) continue;
-                        scanning = false;
-                    }
-                    // the last node's successor, with the threshold its buckets left
-                    const bool gA = crA >= 0 && cdA <= Q.tau, gB = crB >= 0 && cdB <= Q.tau;
-                    const bool bfirst = cdB < cdA;
-                    if (gA && gB) {
-                        if (sp >= max_depth) { overflow = true; break; }
-                        if (sp - lo == S) {  // ring full: its oldest entry goes to HBM
-                            const int j = lo & (S - 1);
-                            spill[lo * gstride] = make_int2(sref[j][tid], __float_as_int(sdist[j][tid]));
-                            ++lo;
-                        }
-                        sref[sp & (S - 1)][tid] = bfirst ? crA : crB;
-                        sdist[sp & (S - 1)][tid] = bfirst ? cdA : cdB;
-                        ++sp;
-                        node = bfirst ? crB : crA;
-                        continue;
-                    }
-                    if (gA) { node = crA; continue; }
-                    if (gB) { node = crB; continue; }
-                    // pop the nearest-pushed entry still inside the threshold
-                    node = -1;
-                    while (sp > 0) {
-                        --sp;
-                        int r;
-                        float d;
-                        if (sp >= lo) {
-                            r = sref[sp & (S - 1)][tid];
-                            d = sdist[sp & (S - 1)][tid];
-                        } else {
-                            const int2 e = spill[sp * gstride];
-                            lo = sp;
-                            r = e.x;
-                            d = __int_as_float(e.y);
-                        }
-                        if (d <= Q.tau) { node = r; break; }
-                    }
-                    if (node < 0) break;
-                }
-                task_steps += trips;
-            }
+            if (root.y > 0) task_steps += lane_walk<C, G, S, COUNT>(Q, root, mem, a.max_steps, max_depth, overflow,
+                                                                   visits, tests);
             resolved = last || (Q.kth() < INFINITY && Q.kth() <= T0);
             if (resolved || overflow || !a.inline_rounds) break;
             ++round;

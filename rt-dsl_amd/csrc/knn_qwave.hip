@@ -63,13 +63,14 @@ template <int M>
 struct WaveList {
     double d[M];
     uint32_t id[M];
-    __device__ __forceinline__ void init(int k, int lane) {
+    // live slots start at (live_d, live_id): only a candidate below that pair enters
+    __device__ __forceinline__ void init(int k, int lane, double live_d = INFINITY, uint32_t live_id = GPRT_ID_NONE) {
         const int s0 = 64 * M - k;
 #pragma unroll
         for (int p = 0; p < M; ++p) {
             const bool sent = lane * M + p < s0;
-            d[p] = sent ? -INFINITY : INFINITY;
-            id[p] = sent ? 0u : GPRT_ID_NONE;
+            d[p] = sent ? -INFINITY : live_d;
+            id[p] = sent ? 0u : live_id;
         }
     }
     __device__ __forceinline__ void worst(double& wd, uint32_t& wi) const {
@@ -280,18 +281,75 @@ __global__ __launch_bounds__(256) void knn_qwave_kernel(RoundArgs a) {
     }
 }
 
-template <int M>
-static int qwave_grid(int grid_req) {
-    if (grid_req > 0) return grid_req;
-    static std::atomic<int> cache[64];
-    return resident_grid(knn_qwave_kernel<M, false>, cache);
-}
-
 // list slots per lane for a largest k
 int qwave_lanes_m(int kmax) {
     int m = 1;
     while (64 * m < kmax) m <<= 1;
     return m;
+}
+
+// The reference's register-heap program (gprt-macros/src/lib.rs:305-377) for
+// k > 64: per query (input order, radius in q.w) the k nearest points with
+// d64 < r^2 (strict, as its AnyHit insert), written as ids[0..k) then the
+// distances [k..2k) as float bits; empty slots hold 0xFFFFFFFF and fl32(r*r);
+// lens = number found.  Live slots start at (r^2, 0), so a point enters only
+// below r^2.
+template <int M>
+__global__ __launch_bounds__(256) void heap_wave_kernel(const Node* __restrict__ nodes_, const float4* __restrict__ pts,
+                                                        int2 root, const float4* __restrict__ q4, int nq, int k,
+                                                        unsigned* __restrict__ heap, unsigned* __restrict__ lens,
+                                                        int max_steps, unsigned* err) {
+    const int lane = threadIdx.x & 63;
+    const GPRT_AS4 Node* nodes = const_view(nodes_);
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    uint64_t visits = 0;
+    for (int qi = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6)); qi < nq; qi += nwaves) {
+        const float4 q = q4[qi];
+        const double r2 = (double)q.w * (double)q.w;
+        WaveQuery<M> Q;
+        Q.qx = q.x; Q.qy = q.y; Q.qz = q.z;
+        Q.T0 = INFINITY;
+        Q.tests = 0;
+        Q.L.init(k, lane, r2, 0u);
+        Q.L.worst(Q.wd, Q.wi);
+        if (!qwave_walk<M>(Q, nodes, pts, root, lane, max_steps, visits) && lane == 0) atomicAdd(err, 1u);
+        const float rsq = q.w * q.w;
+        unsigned* h = heap + (size_t)qi * 2 * k;
+        const int s0 = 64 * M - k;
+        unsigned cnt = 0;
+#pragma unroll
+        for (int p = 0; p < M; ++p) {
+            const int o = lane * M + p - s0;
+            const bool real = o >= 0 && Q.L.d[p] < r2;
+            if (o >= 0) {
+                h[o] = real ? Q.L.id[p] : 0xFFFFFFFFu;
+                h[k + o] = real ? __float_as_uint((float)Q.L.d[p]) : __float_as_uint(rsq);
+            }
+            cnt += real;
+        }
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if (lane == 0) lens[qi] = cnt;
+    }
+}
+
+int launch_heap_wave(hipStream_t s, const Node* nodes, const float4* pts, int2 root, const float4* q4, int nq, int k,
+                     unsigned* heap, unsigned* lens, int max_steps, unsigned* err) {
+    if (k > GPRT_HIP_K_MAX) return fail(GPRT_ERR_ARG, "k too large for the wave-per-query kernel");
+    const dim3 grid((nq + 3) / 4);  // one wavefront per query
+    switch (qwave_lanes_m(k)) {
+#define GPRT_HW(MM)                                                                                                     case MM:                                                                                                                hipLaunchKernelGGL(heap_wave_kernel<MM>, grid, dim3(256), 0, s, nodes, pts, root, q4, nq, k, heap, lens,                             max_steps, err);                                                                                 break;
+        GPRT_HW(1) GPRT_HW(2) GPRT_HW(4) GPRT_HW(8) GPRT_HW(16) GPRT_HW(32)
+        default: GPRT_HW(64)
+#undef GPRT_HW
+    }
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "heap program launch failed");
+}
+
+template <int M>
+static int qwave_grid(int grid_req) {
+    if (grid_req > 0) return grid_req;
+    static std::atomic<int> cache[64];
+    return resident_grid(knn_qwave_kernel<M, false>, cache);
 }
 
 int launch_qwave(hipStream_t s, const RoundArgs& a, int kmax, int grid_req, bool count) {

@@ -462,9 +462,17 @@ extern "C" void gprt_execute(CGprtPipeline* p, CGprtBvh* b, const float* h_queri
         if (p->program == PROG_GENERATED || (a.nq < count && !flat) || count == 0) continue;
         if (p->program == PROG_KNN_HEAP) {
             const int k = (int)(a.cap / 2);
-            if (k < 1 || k > 64) {
-                fail(GPRT_ERR_ARG, "gprt_execute: register-heap capacity must be 2k with 1 <= k <= 64");
+            if (k < 1 || k > GPRT_HIP_K_MAX) {
+                fail(GPRT_ERR_ARG, "gprt_execute: register-heap capacity must be 2k with 1 <= k <= " +
+                                       std::to_string(GPRT_HIP_K_MAX));
                 return;
+            }
+            if (k > 64) {  // one query per wavefront, the list spread over its lanes
+                if (launch_heap_wave(s, T.nodes.as<Node>(), T.pts4.as<float4>(), root, p->q4.as<float4>(), count, k,
+                                     a.data.as<unsigned>(), a.lens.as<unsigned>(), T.n + 64,
+                                     counters + CNT_ERROR) != GPRT_OK)
+                    return;
+                continue;
             }
             dim3 grid((count + 255) / 256);
 #define HEAPK(KM)                                                                                              \

@@ -1,0 +1,67 @@
+"""ctypes front end of libgprt_host.so: the per-lane FP32 k-NN walk, its
+candidate list and FP64 re-rank (lane_walk.hpp) and the LBVH layout of
+lbvh.hip, compiled for the host with every memory access bounds-checked.
+
+For tests of the device code's logic on the CPU; nothing in the GPU path uses
+it.  GPRT_HOST_LIBRARY selects another build (e.g. the ASan + UBSan one).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PATH = os.environ.get("GPRT_HOST_LIBRARY",
+                       os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgprt_host.so"))
+_lib = None
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+
+CAPACITIES = (8, 16, 24, 32, 40, 48, 64, 67)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError(f"{_PATH} is not built (python rt-dsl_amd/build.py)")
+        L = ctypes.CDLL(_PATH)
+        L.gprt_host_last_error.restype = ctypes.c_char_p
+        L.gprt_host_build_tree.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, _u64p, _u32p, _f32p, _i32p]
+        L.gprt_host_lane_knn.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, _f32p, ctypes.c_int64, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                         _u32p, _i32p, _i64p]
+        _lib = L
+    return _lib
+
+
+def build_tree(pts, leaf_size=8):
+    """(keys, perm, nodes (n-1, 16) float32, root (2,)) as lbvh.hip lays them out."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = len(pts)
+    keys = np.zeros(n, np.uint64)
+    perm = np.zeros(n, np.uint32)
+    nodes = np.zeros((max(n - 1, 1), 16), np.float32)
+    root = np.zeros(2, np.int32)
+    rc = lib().gprt_host_build_tree(pts, n, leaf_size, keys, perm, nodes, root)
+    assert rc == 0
+    return keys, perm, nodes[: max(n - 1, 0)], root
+
+
+def lane_knn(pts, queries, k, cap, leaf_size=8, radius0=1.0, mult=3.0, rounds=3, stack_depth=64):
+    """The batched form's per-lane path, one query at a time.  Returns (ids
+    (nq, k), status (nq,): 0 written by the re-rank, 1 sent to the exact kernel,
+    2 id guard tripped, stats: node visits, point tests, trips).  Raises on an
+    out-of-bounds access."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    q = np.ascontiguousarray(queries, np.float32)
+    ids = np.full((len(q), k), 0xFFFFFFFF, np.uint32)
+    st = np.zeros(len(q), np.int32)
+    stats = np.zeros(3, np.int64)
+    rc = lib().gprt_host_lane_knn(pts, len(pts), leaf_size, q, len(q), k, cap, radius0, mult, rounds, stack_depth,
+                                  ids, st, stats)
+    if rc:
+        raise RuntimeError(lib().gprt_host_last_error().decode())
+    return ids, st, stats

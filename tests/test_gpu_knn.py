@@ -70,7 +70,6 @@ def test_knn_leaf_sizes(ctx, leaf):
     dict(sort_queries=0),
     dict(use_graph=0),
     dict(stack_depth=32, grid_blocks=7),
-    dict(guard_slots=1),
     dict(lds_stack=8, stack_depth=24),   # most walks spill to HBM
     dict(lds_stack=8, stack_depth=9),    # walks overflow: exact fallback
     dict(lds_stack=32, stack_depth=32),  # no spill
@@ -166,6 +165,17 @@ def test_knn_large_k(ctx, k):
                 _check_knn(t.k_nn(Q, k, cfg), ref, label=f"{label} tiny radius k={k}")
         finally:
             t.close()
+
+
+def test_knn_guard_slots_fixed(ctx):
+    """The lists carry exactly three guard slots; a schedule asking for another
+    number is rejected rather than ignored."""
+    pts = np.random.default_rng(0).random((100, 3), dtype=np.float32)
+    t = ctx.build(pts)
+    for g in (0, 1, 2, 4):
+        with pytest.raises(gprt_hip.GprtError):
+            t.k_nn(pts, 5, gprt_hip.default_config(guard_slots=g))
+    t.close()
 
 
 def test_knn_k_limit(ctx):

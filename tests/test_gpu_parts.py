@@ -318,16 +318,18 @@ def _legacy_pipe(program):
     return L, p
 
 
-def test_legacy_register_heap_knn():
+@pytest.mark.parametrize("k,r", [(5, 0.02), (64, 0.05), (65, 0.05), (300, 0.06), (1000, 0.08)])
+def test_legacy_register_heap_knn(k, r):
+    """The register-heap program: the k nearest with d64 < r^2, ids then float
+    distances, padding, lengths; k > 64 runs one query per wavefront."""
     L, p = _legacy_pipe("gprt-hip:knn-heap")
     try:
-        pts, q = D.uniform_cloud(20_000, 2_000, seed=12)
-        r = np.float32(0.02)
+        pts, q = D.uniform_cloud(20_000, 2_000 if k <= 64 else 400, seed=12)
+        r = np.float32(r)
         geom = np.concatenate([pts, np.full((len(pts), 1), r, np.float32)], 1)
         aabbs = np.concatenate([pts - r, pts + r], 1).astype(np.float32)
         bvh = L.gprt_bvh_build(p, geom.ctypes.data, aabbs.ctypes.data, len(pts), geom.nbytes)
         assert bvh
-        k = 5
         L.gprt_register_array(p, b"out", 2 * k, len(q))
         qq = np.concatenate([q, np.full((len(q), 1), r, np.float32)], 1)
         L.gprt_execute(p, bvh, qq.ctypes.data, len(q))

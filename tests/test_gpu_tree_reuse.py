@@ -59,3 +59,20 @@ def test_tree_outlives_its_context():
         t2.close()
     finally:
         c2.close()
+
+
+@pytest.mark.parametrize("kind,n,leaf", [("uniform", 50_000, 8), ("lattice", 5000, 4), ("plane", 3000, 16),
+                                         ("clusters", 20_000, 1), ("line", 7, 8)])
+def test_gpu_tree_equals_host_build(ctx, kind, n, leaf):
+    """The GPU LBVH (keys, permutation, every node's child boxes, refs and
+    counts, root) equals the host build of the same layout bit for bit."""
+    from gprt_hip import hostwalk as H
+    from test_gpu_parity_matrix import _cloud
+    pts = _cloud(kind, n, np.random.default_rng(n))
+    t = ctx.build(pts, leaf_size=leaf)
+    keys, perm, nodes, roots = t.export()
+    hk, hp, hn, hr = H.build_tree(pts, leaf)
+    t.close()
+    assert np.array_equal(keys, hk) and np.array_equal(perm, hp)
+    assert np.array_equal(nodes.view(np.uint32), hn.view(np.uint32))
+    assert np.array_equal(roots[0], hr)
