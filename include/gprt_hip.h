@@ -289,6 +289,16 @@ GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t 
  * other thread calls the library.  Results never depend on it; a captured
  * graph keeps the ordering it was captured with (use_graph = 0 to compare). */
 GPRT_HIP_API void gprt_hip_debug_set_query_sort_bits(int bits);
+/* Device allocations the library made for its own buffers since load (calls),
+ * and those still held (buffers, bytes); any pointer may be NULL.  No
+ * counterpart in the reference (which exits on any failure, main.cpp:12-28):
+ * for leak checks. */
+GPRT_HIP_API void gprt_hip_debug_alloc_stats(int64_t* calls, int64_t* live_buffers, int64_t* live_bytes);
+/* Fault injection for tests: the next `after` device allocations of the
+ * library succeed, every later one fails with GPRT_ERR_ALLOC, until called
+ * again; after < 0 turns it off.  The environment variable GPRT_HIP_FAULT=alloc
+ * (or alloc:N) sets it at load.  Process-wide. */
+GPRT_HIP_API void gprt_hip_debug_fail_alloc(int64_t after);
 
 /* ---------------- sandbox inputs ---------------- */
 #define GPRT_CSV_POINTS 0 /* "x,y,z[,...]" per line (gprt-sandbox/src/io.rs:8-30) */

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -28,11 +29,25 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+// Device allocations of the library's buffers: counted (live buffers and
+// bytes, for leak checks), and failed on purpose when a test asks for it:
+// GPRT_HIP_FAULT=alloc makes every one fail, GPRT_HIP_FAULT=alloc:N lets N
+// succeed first (also settable with gprt_hip_debug_fail_alloc).
+static std::atomic<int64_t> g_alloc_calls{0}, g_live_bufs{0}, g_live_bytes{0};
+static std::atomic<int64_t> g_alloc_budget{[] {
+    const char* f = std::getenv("GPRT_HIP_FAULT");
+    if (!f || std::strncmp(f, "alloc", 5) != 0) return (int64_t)-1;
+    return f[5] == ':' ? (int64_t)std::max(0LL, std::atoll(f + 6)) : (int64_t)0;
+}()};
+
 int DevBuf::reserve(size_t need) {
     if (need <= bytes && ptr) return GPRT_OK;
     release();
     size_t b = std::max<size_t>(need, 256);
     b = (b + 255) & ~(size_t)255;
+    int64_t left = g_alloc_budget.load();
+    while (left > 0 && !g_alloc_budget.compare_exchange_weak(left, left - 1)) {}
+    if (left == 0) return fail(GPRT_ERR_ALLOC, "hipMalloc of " + std::to_string(b) + " bytes failed (injected)");
     hipError_t e = hipMalloc(&ptr, b);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -41,10 +56,17 @@ int DevBuf::reserve(size_t need) {
         return fail(GPRT_ERR_ALLOC, "hipMalloc of " + std::to_string(b) + " bytes failed");
     }
     bytes = b;
+    ++g_alloc_calls;
+    ++g_live_bufs;
+    g_live_bytes += (int64_t)b;
     return GPRT_OK;
 }
 void DevBuf::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) {
+        (void)hipFree(ptr);
+        --g_live_bufs;
+        g_live_bytes -= (int64_t)bytes;
+    }
     ptr = nullptr;
     bytes = 0;
 }
@@ -150,6 +172,14 @@ using namespace gprt;
 // library / context
 // ===========================================================================
 extern "C" int gprt_hip_version(void) { return 10000; }
+
+extern "C" void gprt_hip_debug_alloc_stats(int64_t* calls, int64_t* live_buffers, int64_t* live_bytes) {
+    if (calls) *calls = g_alloc_calls.load();
+    if (live_buffers) *live_buffers = g_live_bufs.load();
+    if (live_bytes) *live_bytes = g_live_bytes.load();
+}
+
+extern "C" void gprt_hip_debug_fail_alloc(int64_t after) { g_alloc_budget.store(after < 0 ? -1 : after); }
 
 extern "C" const char* gprt_hip_last_error(void) { return g_err_msg.c_str(); }
 extern "C" void gprt_hip_clear_error(void) { g_err_msg.clear(); }

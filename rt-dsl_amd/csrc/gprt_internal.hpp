@@ -54,6 +54,7 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+
 // ---------------------------------------------------------------------------
 // LBVH layout in HBM
 // ---------------------------------------------------------------------------

@@ -101,6 +101,8 @@ SIGNATURES = [
     ("gprt_hip_last_error", ctypes.c_char_p, []),
     ("gprt_hip_clear_error", None, []),
     ("gprt_hip_debug_set_query_sort_bits", None, [_int]),
+    ("gprt_hip_debug_alloc_stats", None, [ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    ("gprt_hip_debug_fail_alloc", None, [_i64]),
     ("gprt_hip_config_default", None, [ctypes.POINTER(Config)]),
     ("gprt_hip_ctx_create", _int, [_int, ctypes.POINTER(_vp)]),
     ("gprt_hip_ctx_destroy", _int, [_vp]),
