@@ -231,6 +231,21 @@ GPRT_HIP_API int gprt_hip_knn_batched(gprt_hip_ctx* ctx, gprt_hip_tree* tree, co
                                       const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out,
                                       int ptr_flags);
 
+/* Allocation-free variant of gprt_hip_knn, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Queries, ids_out
+ * and d2_out (may be NULL) are device pointers; the tree is built beforehand.
+ * Every per-call workspace lives in one 32 MiB device array per GPU, held
+ * until the process ends; the traversal stack is the LDS ring only (deeper
+ * walks finish in the exact kernel) and no graph is captured.  Same results as
+ * gprt_hip_knn.  nq above gprt_hip_knn_noalloc_max_queries() is rejected with
+ * GPRT_ERR_CAPACITY.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (from several contexts or threads of different contexts) race.
+ * Synchronous. */
+GPRT_HIP_API int gprt_hip_knn_noalloc(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq,
+                                      int k, const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out);
+GPRT_HIP_API int64_t gprt_hip_knn_noalloc_max_queries(void);
+
 /* ---------------- fixed-radius range query ---------------- */
 /* r_nn! / range_query! (gprt-macros/src/lib.rs:729-750, which as shipped
  * returns one flat id list without per-query split).  Phase 1: ids_out ==

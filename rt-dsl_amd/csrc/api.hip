@@ -39,9 +39,14 @@ static std::atomic<int64_t> g_alloc_budget{[] {
     if (!f || std::strncmp(f, "alloc", 5) != 0) return (int64_t)-1;
     return f[5] == ':' ? (int64_t)std::max(0LL, std::atoll(f + 6)) : (int64_t)0;
 }()};
+static thread_local bool t_no_alloc = false;
+void set_no_alloc(bool on) { t_no_alloc = on; }
 
 int DevBuf::reserve(size_t need) {
     if (need <= bytes && ptr) return GPRT_OK;
+    if (borrowed || t_no_alloc)
+        return fail(GPRT_ERR_CAPACITY, "allocation-free call: a workspace of " + std::to_string(need) +
+                                           " bytes does not fit its slot of the scratch array");
     release();
     size_t b = std::max<size_t>(need, 256);
     b = (b + 255) & ~(size_t)255;
@@ -62,13 +67,14 @@ int DevBuf::reserve(size_t need) {
     return GPRT_OK;
 }
 void DevBuf::release() {
-    if (ptr) {
+    if (ptr && !borrowed) {
         (void)hipFree(ptr);
         --g_live_bufs;
         g_live_bytes -= (int64_t)bytes;
     }
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
 // workspace slots of a context
@@ -216,6 +222,32 @@ extern "C" int gprt_hip_num_devices(int* out) {
     return GPRT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free k-NN entry point (gprt_hip_knn_noalloc): a
+// file-scope device array, one copy per GPU, held until the process ends.  Its
+// address on each device is looked up once, when the first context on that
+// device is created, and only that address is handed to kernels and copies.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+// with `device` current
+static int scratch_setup(int device) {
+    if (device < 0 || device >= 64) return fail(GPRT_ERR_ARG, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GPRT_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return fail(GPRT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = ptr;
+    return GPRT_OK;
+}
+
 extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
     if (!out) return fail(GPRT_ERR_ARG, "null out");
     *out = nullptr;
@@ -250,6 +282,11 @@ extern "C" int gprt_hip_ctx_create(int device, gprt_hip_ctx** out) {
         gprt_hip_ctx_destroy(h);
         return fail(GPRT_ERR_ALLOC, "pinned host allocation failed");
     }
+    if (int rc = scratch_setup(device); rc != GPRT_OK) {
+        gprt_hip_ctx_destroy(h);
+        return rc;
+    }
+    (void)extra(&c);  // the per-context extras exist before any allocation-free call
     *out = h;
     return GPRT_OK;
 }
@@ -816,6 +853,105 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
     }
     if (flags & GPRT_NO_SYNC) return GPRT_OK;
     return finish_stats(c, nq, true, c.ws[WS_MISC].as<float>() + 100 * (p.nclouds));
+}
+
+// ---------------------------------------------------------------------------
+// gprt_hip_knn without device allocation: every per-call workspace of the
+// batched pipeline (query keys, order and copies, work lists, fallback list,
+// sort temporaries, radius slots) is a slot of the scratch array, lent to the
+// context's buffers for the duration of the call; the traversal stack is the
+// LDS ring only (stack_depth 8, no HBM spill; deeper walks finish in the exact
+// kernel) and no graph is captured.  A reservation that would still allocate
+// fails the call instead (GPRT_ERR_CAPACITY).
+// ---------------------------------------------------------------------------
+namespace {
+struct ScratchPlan {
+    size_t off[12];
+    size_t size[12];
+    size_t total;
+};
+// slots: 7 workspaces (WS_QKEYS .. WS_MISC), 3 sort temporaries, qoff
+ScratchPlan scratch_plan(int64_t nq) {
+    const size_t q = (size_t)std::max<int64_t>(nq, 1);
+    const size_t sz[11] = {8 * q, 4 * q, 16 * q, 4 * q, 4 * q, 8 * q, sizeof(float) * 101 + 1024,
+                           8 * q, 4 * q, radix_sort_hist_bytes((int)q), 16};
+    ScratchPlan p{};
+    size_t o = 0;
+    for (int i = 0; i < 11; ++i) {
+        p.off[i] = o;
+        p.size[i] = (sz[i] + 255) & ~(size_t)255;
+        o += p.size[i];
+    }
+    p.total = o;
+    return p;
+}
+}  // namespace
+
+extern "C" int64_t gprt_hip_knn_noalloc_max_queries(void) {
+    int64_t lo = 0, hi = (int64_t)NOALLOC_SCRATCH_BYTES;
+    while (lo < hi) {
+        const int64_t m = (lo + hi + 1) / 2;
+        if (scratch_plan(m).total <= NOALLOC_SCRATCH_BYTES) lo = m; else hi = m - 1;
+    }
+    return lo;
+}
+
+extern "C" int gprt_hip_knn_noalloc(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,
+                                    const gprt_hip_config* cfg, uint32_t* ids_out, double* d2_out) {
+    if (!h || !t) return fail(GPRT_ERR_ARG, "null ctx/tree");
+    if (k < 1 || k > GPRT_HIP_K_MAX) return fail(GPRT_ERR_ARG, "k must be in [1, " + std::to_string(GPRT_HIP_K_MAX) + "]");
+    if (nq < 0) return fail(GPRT_ERR_ARG, "bad nq");
+    if (nq > gprt_hip_knn_noalloc_max_queries())
+        return fail(GPRT_ERR_CAPACITY, "gprt_hip_knn_noalloc takes at most " +
+                                           std::to_string(gprt_hip_knn_noalloc_max_queries()) + " queries (" +
+                                           std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB scratch array)");
+    if (nq > 0 && (!queries || !ids_out)) return fail(GPRT_ERR_ARG, "null queries/ids_out");
+    if (t->t.nclouds != 1) return fail(GPRT_ERR_ARG, "tree has several clouds");
+    GPRT_CTX_LOCK(h);
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    gprt_hip_config cf;
+    if (cfg) cf = *cfg; else gprt_hip_config_default(&cf);
+    cf.use_graph = 0;
+    cf.stack_depth = 8;  // no HBM spill: the LDS ring is the whole stack
+    GPRT_TRY(check_config(cf));
+    c.stats = gprt_hip_run_stats{};
+    if (nq == 0) return GPRT_OK;
+    unsigned char* base = (unsigned char*)g_scratch_ptr[c.device];
+    if (!base) return fail(GPRT_ERR_STATE, "scratch array not set up on this device");
+    CtxExtra* x = extra(&c);
+    DevBuf* slots[11] = {&c.ws[WS_QKEYS], &c.ws[WS_QPERM], &c.ws[WS_QS4], &c.ws[WS_LISTA], &c.ws[WS_LISTB],
+                         &c.ws[WS_FB], &c.ws[WS_MISC], &c.sort_tmp[0], &c.sort_tmp[1], &c.sort_tmp[2], &x->qoff};
+    DevBuf saved[11];
+    const ScratchPlan sp = scratch_plan(nq);
+    for (int i = 0; i < 11; ++i) {
+        saved[i] = *slots[i];
+        slots[i]->ptr = base + sp.off[i];
+        slots[i]->bytes = sp.size[i];
+        slots[i]->borrowed = true;
+    }
+    set_no_alloc(true);
+    KnnPlan p{};
+    p.tree = &t->t;
+    p.d_q = queries;
+    p.nq = nq;
+    p.nclouds = 1;
+    p.kmax = k;
+    p.k = k;
+    p.d_qoff = x->qoff.as<int64_t>();
+    p.max_q_per_cloud = nq;
+    p.cfg = cf;
+    p.d_ids = ids_out;
+    p.d_d2 = d2_out;
+    {
+        int64_t qo[2] = {0, nq};
+        plan_classes(p, nullptr, qo, 1, k);
+    }
+    int rc = knn_run(c, p);
+    if (rc == GPRT_OK) rc = finish_stats(c, nq, true, c.ws[WS_MISC].as<float>() + 100);
+    set_no_alloc(false);
+    for (int i = 0; i < 11; ++i) *slots[i] = saved[i];
+    return rc;
 }
 
 extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float* queries, int64_t nq, int k,

@@ -49,11 +49,14 @@ int fail(int code, const std::string& msg);
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;  // a view into memory the buffer does not own (never freed, never grown)
     int reserve(size_t need);
     void release();
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
-
+// While set on the calling thread, a DevBuf::reserve that would allocate
+// fails with GPRT_ERR_CAPACITY instead (the allocation-free entry points).
+void set_no_alloc(bool on);
 
 // ---------------------------------------------------------------------------
 // LBVH layout in HBM

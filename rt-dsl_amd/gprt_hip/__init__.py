@@ -103,6 +103,8 @@ SIGNATURES = [
     ("gprt_hip_debug_set_query_sort_bits", None, [_int]),
     ("gprt_hip_debug_alloc_stats", None, [ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     ("gprt_hip_debug_fail_alloc", None, [_i64]),
+    ("gprt_hip_knn_noalloc", _int, [_vp, _vp, _vp, _i64, _int, ctypes.POINTER(Config), _vp, _vp]),
+    ("gprt_hip_knn_noalloc_max_queries", _i64, []),
     ("gprt_hip_config_default", None, [ctypes.POINTER(Config)]),
     ("gprt_hip_ctx_create", _int, [_int, ctypes.POINTER(_vp)]),
     ("gprt_hip_ctx_destroy", _int, [_vp]),
