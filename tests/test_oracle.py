@@ -166,6 +166,76 @@ def test_sampler_lcg_tiny_inputs():
     assert oracle.sampler_lcg(pts, q, oracle.HEUR_FIXED, 0.5) == 10.0
 
 
+def _mmix_draws(count):
+    """Knuth's MMIX LCG from the sampler's seed (dispatch.rs:65-69), big-int
+    arithmetic; its first 32 draws are the golden file's."""
+    seed, out = 0x123456789ABCDEF, []
+    for _ in range(count):
+        seed = (seed * 6364136223846793005 + 1442695040888963407) % (1 << 64)
+        out.append(seed >> 33)
+    return out
+
+
+@pytest.mark.parametrize("n,nq", [(5, 500), (7_000, 137), (4, 100)])
+def test_sampler_lcg_closed_form(n, nq):
+    """Every data point at the origin and query j at (x_j, 0, 0) with x_j = j + 1:
+    whichever points a drawn query samples, its four smallest distances are all
+    exactly x_j, so the 4th is x_j.  The draw order (one query draw, then
+    min(5000, n) point draws, 100 times; query = draw mod nq) fixes which x_j
+    are pushed, and the radius follows in closed form: the p-th entry of the
+    sorted list (index min(len*p, len-1) in f32) times 1.1, the last one for
+    SampledMax, the parameter itself for Fixed (dispatch.rs:65-103)."""
+    pts = np.zeros((n, 3), np.float32)
+    q = np.zeros((nq, 3), np.float32)
+    q[:, 0] = np.arange(1, nq + 1, dtype=np.float32)
+    s = min(5000, n)
+    draws = _mmix_draws(100 * (s + 1))
+    kth = sorted(np.float32(draws[j * (s + 1)] % nq + 1) for j in range(min(100, nq)))
+    f32 = np.float32
+    for p in (0.95, 0.5, 0.0, 1.0, 0.123):
+        idx = int(min(f32(f32(len(kth)) * f32(p)), f32(len(kth) - 1)))
+        assert oracle.sampler_lcg(pts, q, oracle.HEUR_PERCENTILE, p) == float(f32(kth[idx] * f32(1.1))), p
+    assert oracle.sampler_lcg(pts, q, oracle.HEUR_MAX, 0.0) == float(f32(kth[-1] * f32(1.1)))
+    assert oracle.sampler_lcg(pts, q, oracle.HEUR_FIXED, 0.75) == float(f32(0.75))
+    # a planted mistake in the interleaving (query draws taken consecutively)
+    # gives another list, so the check above can tell the orders apart
+    wrong = sorted(np.float32(draws[j] % nq + 1) for j in range(min(100, nq)))
+    assert wrong != kth
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
+def test_sampler_strided_closed_form(k):
+    """1,000 points on 100 parallel lines, ten per line with spacing s_i =
+    1 + i/4 (lines 1,000 apart), and 100 queries, query i exactly on the
+    fifth point of line i.  Every point and every query is sampled (strides 1).
+    The query's own point has d^2 = 0 and is dropped by the d^2 > 1e-10 filter,
+    its neighbours lie at 1, 1, 2, 2, ... spacings, so its k-th distance is
+    ceil(k/2) * s_i exactly, and the radius is the p-th of those times 1.1
+    (gprt-macros/src/lib.rs:425-450)."""
+    pts = np.zeros((1000, 3), np.float32)
+    q = np.zeros((100, 3), np.float32)
+    f32 = np.float32
+    for i in range(100):
+        sp = f32(1 + i / 4)
+        pts[10 * i:10 * i + 10, 0] = np.arange(10, dtype=np.float32) * sp
+        pts[10 * i:10 * i + 10, 1] = f32(1000 * i)
+        q[i] = pts[10 * i + 4]
+    kth = sorted(f32((k + 1) // 2) * f32(1 + i / 4) for i in range(100))
+    for p in (0.10, 0.5, 0.95, 0.0, 1.0):
+        idx = min(int(f32(f32(100) * f32(p))), 99)
+        assert oracle.sampler_strided(pts, q, k, oracle.HEUR_PERCENTILE, p) == float(f32(kth[idx] * f32(1.1))), p
+    # a planted mistake, the self-filter left out: the query's own point counts
+    # as the nearest, the k-th becomes ceil((k-1)/2) spacings, which the closed
+    # form tells apart for odd k
+    planted = sorted(f32(k // 2) * f32(1 + i / 4) for i in range(100))
+    assert (planted != kth) == (k % 2 == 1)
+    # the filter keeps a distinct point at d^2 = 1e-8 > 1e-10
+    q2 = q.copy()
+    q2[:, 2] = f32(1e-4)
+    r2 = oracle.sampler_strided(pts, q2, 1, oracle.HEUR_PERCENTILE, 0.5)
+    assert abs(r2 - float(f32(1e-4) * f32(1.1))) < 1e-9
+
+
 def _py_sampler_strided(pts, q, k, heuristic, p):
     """Pure-Python restatement of gprt-macros/src/lib.rs:417-450."""
     f32 = np.float32
