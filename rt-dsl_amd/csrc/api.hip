@@ -165,7 +165,7 @@ __global__ void init_call_kernel(unsigned* counters, int n, int64_t* qoff, int64
 
 int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
               int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
-              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr);
+              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr, bool count = false);
 int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps, float G, float* d_acc,
            DevBuf& mass_buf, bool count, bool ref_law);
 int bh_prepare(Ctx* ctx, const float4* d_pm, int n, float* d_xyz, unsigned* flag);
@@ -1278,7 +1278,7 @@ extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* qu
     if (!ids_out) {
         GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, nullptr, 0, x->r_counts, x->r_d2, x->r_big, x->r_keys,
-                           x->r_perm, x->r_qs4, x->r_sums, &total));
+                           x->r_perm, x->r_qs4, x->r_sums, &total, nullptr, (flags & GPRT_COLLECT_COUNTERS) != 0));
         GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
         if (!dev) GPRT_HIP_TRY(hipMemcpyAsync(offsets_out, d_off, 8ull * (nq + 1), hipMemcpyDeviceToHost, s));
     } else {
@@ -1297,7 +1297,8 @@ extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* qu
         }
         GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, d_ids, total, x->r_counts, x->r_d2, x->r_big,
-                           x->r_keys, x->r_perm, x->r_qs4, x->r_sums, nullptr));
+                           x->r_keys, x->r_perm, x->r_qs4, x->r_sums, nullptr, nullptr,
+                           (flags & GPRT_COLLECT_COUNTERS) != 0));
         GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
         if (!dev && total) GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * total, hipMemcpyDeviceToHost, s));
     }

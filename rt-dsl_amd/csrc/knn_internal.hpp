@@ -12,7 +12,8 @@ constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
 constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)
 constexpr int CNT_BADID = 43;     // re-ranked candidates outside their query's cloud (must stay 0)
-constexpr int CNT_TOTAL = 48;
+constexpr int CNT_AUX = 44;       // [0..3]: scratch counters of the range pipeline (segment lists)
+constexpr int CNT_TOTAL = 56;   // the last slot (CNT_TOTAL - 1) is the input-validation flag
 constexpr int CNT_STATS64 = 6;   // u64 statistics after the counters: visits, tests, re-rank reads,
                                  // (spare), wave steps, lane steps
 constexpr int CNT_ALL = CNT_TOTAL + 2 * CNT_STATS64;  // words of the counter buffer

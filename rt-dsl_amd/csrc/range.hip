@@ -12,6 +12,7 @@
 // counts -> offsets (exclusive scan) -> the same traversal writing ids and
 // d64 -> per-query sort of (d64, id).
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 
 #include "gprt_internal.hpp"
@@ -20,83 +21,95 @@
 namespace gprt {
 
 // per-lane walk with a private stack of EXACT_STACK entries (never overflows:
-// the radix-tree depth bound).  mode 0 counts, mode 1 writes.
-template <bool WRITE>
+// the radix-tree depth bound).  The count pass stores each query's number of
+// hits; the fill pass writes each hit's sorted point position (4 bytes) into
+// the query's output range, in walk order: the sort pass turns positions into
+// ids in (d64, id) order.  COUNT adds node visits and point tests to stats64.
+template <bool WRITE, bool COUNT>
 __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nodes, const float4* __restrict__ pts,
                                                     int2 root, const float4* __restrict__ qs, int nq, float r_all,
                                                     const float* __restrict__ radii,
                                                     int64_t* __restrict__ counts, const int64_t* __restrict__ off,
-                                                    uint32_t* __restrict__ ids, double* __restrict__ d2,
-                                                    int64_t capacity, int max_steps, unsigned* err) {
+                                                    uint32_t* __restrict__ out, int64_t capacity, int max_steps,
+                                                    unsigned* err, unsigned long long* stats64) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const float4 q = qs[i];
-    const int g = __float_as_int(q.w);
-    const float r = radii ? radii[g] : r_all;
-    const double r2 = (double)r * (double)r;
-    const float tau = band(r * r);
-    int64_t cnt = 0;
-    int64_t pos = WRITE ? off[g] : 0, end = WRITE ? min(off[g + 1], capacity) : 0;
-    // a leaf range four points at a time: the loads of a group issue together
-    // (indices past the range clamped to its last point, loaded but not
-    // tested), and hits keep the range's order
-    auto visit = [&](int f, int m) {
-        for (int t0 = 0; t0 < m; t0 += 4) {
-            float4 p[4];
+    uint64_t visits = 0, tests = 0;
+    if (i < nq) {
+        const float4 q = qs[i];
+        const int g = __float_as_int(q.w);
+        const float r = radii ? radii[g] : r_all;
+        const double r2 = (double)r * (double)r;
+        const float tau = band(r * r);
+        int64_t cnt = 0;
+        int64_t pos = WRITE ? off[g] : 0, end = WRITE ? min(off[g + 1], capacity) : 0;
+        // a leaf range four points at a time: the loads of a group issue
+        // together (indices past the range clamped to its last point, loaded
+        // but not tested)
+        auto visit = [&](int f, int m) {
+            if (COUNT) tests += (uint64_t)m;
+            for (int t0 = 0; t0 < m; t0 += 4) {
+                float4 p[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) p[u] = pts[f + min(t0 + u, m - 1)];
+                for (int u = 0; u < 4; ++u) p[u] = pts[f + min(t0 + u, m - 1)];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (t0 + u < m && d2f(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= tau) {
-                    const double dd = d2d(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z);
-                    if (dd <= r2) {
+                for (int u = 0; u < 4; ++u)
+                    if (t0 + u < m && d2f(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= tau &&
+                        d2d(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= r2) {
                         if (WRITE) {
-                            if (pos < end) {
-                                ids[pos] = (uint32_t)__float_as_int(p[u].w);
-                                d2[pos] = dd;
-                            }
+                            if (pos < end) out[pos] = (uint32_t)(f + t0 + u);
                             ++pos;
                         } else {
                             ++cnt;
                         }
                     }
+            }
+        };
+        if (root.y > 0) {
+            if (root.x < 0) {
+                visit(~root.x, root.y);
+            } else {
+                int stk[EXACT_STACK];
+                int sp = 0, node = root.x, steps = 0;
+                while (true) {
+                    if (++steps > max_steps || sp >= EXACT_STACK) { atomicAdd(err, 1u); break; }
+                    const Node nd = nodes[node];
+                    if (COUNT) ++visits;
+                    const float dA = box2f(q.x, q.y, q.z, nd.alo, nd.ahi);
+                    const float dB = box2f(q.x, q.y, q.z, nd.blo, nd.bhi);
+                    const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                    const bool inA = dA <= tau, inB = dB <= tau;
+                    {
+                        // the two buckets of a node are adjacent ranges of the sorted
+                        // points (A's then B's): in-range ones are scanned as one range
+                        const bool sA = rA < 0 && inA, sB = rB < 0 && inB;
+                        const int mm = (sA ? __float_as_int(nd.ahi.w) : 0) + (sB ? __float_as_int(nd.bhi.w) : 0);
+                        if (mm > 0) visit(sA ? ~rA : ~rB, mm);
+                    }
+                    const bool gA = rA >= 0 && inA, gB = rB >= 0 && inB;
+                    if (gA && gB) {
+                        stk[sp++] = rB;
+                        node = rA;
+                        continue;
+                    }
+                    if (gA) { node = rA; continue; }
+                    if (gB) { node = rB; continue; }
+                    if (sp == 0) break;
+                    node = stk[--sp];
                 }
-        }
-    };
-    if (root.y > 0) {
-        if (root.x < 0) {
-            visit(~root.x, root.y);
-        } else {
-            int stk[EXACT_STACK];
-            int sp = 0, node = root.x, steps = 0;
-            while (true) {
-                if (++steps > max_steps || sp >= EXACT_STACK) { atomicAdd(err, 1u); break; }
-                const Node nd = nodes[node];
-                const float dA = box2f(q.x, q.y, q.z, nd.alo, nd.ahi);
-                const float dB = box2f(q.x, q.y, q.z, nd.blo, nd.bhi);
-                const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
-                const bool inA = dA <= tau, inB = dB <= tau;
-                {
-                    // the two buckets of a node are adjacent ranges of the sorted
-                    // points (A's then B's): in-range ones are scanned as one range
-                    const bool sA = rA < 0 && inA, sB = rB < 0 && inB;
-                    const int mm = (sA ? __float_as_int(nd.ahi.w) : 0) + (sB ? __float_as_int(nd.bhi.w) : 0);
-                    if (mm > 0) visit(sA ? ~rA : ~rB, mm);
-                }
-                const bool gA = rA >= 0 && inA, gB = rB >= 0 && inB;
-                if (gA && gB) {
-                    stk[sp++] = rB;
-                    node = rA;
-                    continue;
-                }
-                if (gA) { node = rA; continue; }
-                if (gB) { node = rB; continue; }
-                if (sp == 0) break;
-                node = stk[--sp];
             }
         }
+        if (!WRITE) counts[g] = cnt;
     }
-    if (!WRITE) counts[g] = cnt;
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1) {
+            visits += __shfl_xor(visits, o);
+            tests += __shfl_xor(tests, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&stats64[0], (unsigned long long)visits);
+            atomicAdd(&stats64[1], (unsigned long long)tests);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -163,136 +176,248 @@ __global__ __launch_bounds__(256) void scan_tile_apply(const int64_t* __restrict
 
 __global__ void set_last_kernel(int64_t* out, int n, const int64_t* total) { out[n] = *total; }
 
+
 // ---------------------------------------------------------------------------
-// per-query sort of (d64, id).  Segments of <= 64 are sorted by one wave with
-// a register bitonic network; longer ones are queued for a block each.
+// Per-query sort: the fill pass left each query's hits as sorted point
+// positions in walk order; these kernels turn them into ids in (d64, id)
+// order, in place.  d64 and the id are recomputed from the gathered point (a
+// query's hits are a few runs of neighbouring positions, so the gathers are
+// mostly cache hits) instead of being written and read back by the walk.
+// A classifying pass lists the segments by length (wave-aggregated appends):
+//   * up to 64 hits: one wavefront, a bitonic network in registers;
+//   * up to SEG_MID (2,048) / SEG_LARGE (8,192): one workgroup of 256 / 1,024
+//     lanes, a bucket sort in LDS.  With B = ceil(m/2) buckets, bucket b holds
+//     the elements with floor(d64 * B / r^2) = b, a monotone function of d64,
+//     so buckets come out in d64 order and only each bucket's few elements
+//     (two on average) need ordering by (d64, id), done by insertion; a bucket
+//     of more than SEG_RUN elements (many equal distances: lattices,
+//     duplicates) switches the segment to a bitonic network over the whole
+//     segment;
+//   * longer: SEG_LARGE-element chunks sorted the same way, then merged
+//     pairwise in global memory by rank.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool pair_gt(double da, uint32_t ia, double db, uint32_t ib) {
     return da > db || (da == db && ia > ib);
 }
 
-constexpr int SEG_LDS = 2048;  // longest segment sorted in LDS in one piece
+constexpr int SEG_MID = 2048;
+constexpr int SEG_LARGE = 8192;
+constexpr int SEG_RUN = 32;       // longest bucket ordered by insertion
+constexpr uint16_t SEG_NONE = 0xFFFF;
 
-// big_list: segments longer than 64; segments longer than SEG_LDS also get a
-// range of the merge scratch (big_tmp[slot], from the running total big_total)
-__global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __restrict__ off, int nq,
-                                                            uint32_t* ids, double* d2, int64_t capacity,
-                                                            int* big_list, int64_t* big_tmp, unsigned* big_count,
-                                                            unsigned long long* big_total) {
+struct SegQ {
+    float x, y, z;
+    double r2;
+};
+__device__ __forceinline__ SegQ seg_query(const float* __restrict__ qxyz, const float* __restrict__ radii,
+                                          float r_all, int g) {
+    const float r = radii ? radii[g] : r_all;
+    return SegQ{qxyz[3 * (size_t)g], qxyz[3 * (size_t)g + 1], qxyz[3 * (size_t)g + 2], (double)r * (double)r};
+}
+
+// list indices of the per-length segment lists
+enum { SEG_L_SMALL = 0, SEG_L_MID, SEG_L_LARGE, SEG_L_HUGE };
+
+// one lane per segment: lists segments of 2..64, ..SEG_MID, ..SEG_LARGE and
+// longer hits (one-hit segments are converted here); huge ones also get a range
+// of the merge scratch (from the running total huge_total)
+__global__ __launch_bounds__(256) void seg_classify_kernel(const int64_t* __restrict__ off, int nq, int64_t capacity,
+                                                           const float4* __restrict__ pts, uint32_t* ids,
+                                                           int* lists, unsigned* counts, int64_t* huge_tmp,
+                                                           unsigned long long* huge_total) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t m = 0, b = 0;
+    if (g < nq) {
+        b = off[g];
+        m = min(off[g + 1], capacity) - b;
+        if (m == 1) ids[b] = (uint32_t)__float_as_int(pts[ids[b]].w);
+    }
+    const int cls = m <= 1 ? -1 : (m <= 64 ? SEG_L_SMALL : (m <= SEG_MID ? SEG_L_MID : (m <= SEG_LARGE ? SEG_L_LARGE
+                                                                                                   : SEG_L_HUGE)));
+    for (int c = SEG_L_SMALL; c <= SEG_L_HUGE; ++c) {
+        const int slot = wave_append(&counts[c], cls == c);
+        if (slot >= 0) {
+            lists[(size_t)c * nq + slot] = g;
+            if (c == SEG_L_HUGE) huge_tmp[slot] = (int64_t)atomicAdd(huge_total, (unsigned long long)m);
+        }
+    }
+}
+
+// segments of 2..64 hits, one wavefront each (a persistent grid over the list)
+__global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __restrict__ off,
+                                                            const int* __restrict__ list,
+                                                            const unsigned* __restrict__ count,
+                                                            const float* __restrict__ qxyz,
+                                                            const float* __restrict__ radii, float r_all,
+                                                            const float4* __restrict__ pts, uint32_t* ids,
+                                                            int64_t capacity) {
     const int lane = threadIdx.x & 63;
-    const int seg = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (seg >= nq) return;  // whole wave leaves together
-    const int64_t b = off[seg], e = min(off[seg + 1], capacity);
-    const int64_t m = e - b;
-    if (m <= 1) return;
-    if (m > 64) {
-        if (lane == 0) {
-            const unsigned slot = atomicAdd(big_count, 1u);
-            big_list[slot] = seg;
-            big_tmp[slot] = m > SEG_LDS ? (int64_t)atomicAdd(big_total, (unsigned long long)m) : -1;
+    const unsigned n = *count, nw = (gridDim.x * blockDim.x) >> 6;
+    for (unsigned li = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; li < n; li += nw) {
+        const int seg = list[li];
+        const int64_t b = off[seg];
+        const int m = (int)(min(off[seg + 1], capacity) - b);
+        const SegQ q = seg_query(qxyz, radii, r_all, seg);
+        double d = INFINITY;
+        uint32_t id = 0xFFFFFFFFu;
+        if (lane < m) {
+            const float4 p = pts[ids[b + lane]];
+            d = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+            id = (uint32_t)__float_as_int(p.w);
         }
-        return;
-    }
-    double d = lane < m ? d2[b + lane] : INFINITY;
-    uint32_t id = lane < m ? ids[b + lane] : 0xFFFFFFFFu;
-    for (int size = 2; size <= 64; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const double od = __shfl_xor(d, stride);
-            const uint32_t oi = __shfl_xor(id, stride);
-            const bool up = (lane & size) == 0;
-            const bool lower = (lane & stride) == 0;
-            // keep the smaller element on the lower lane of an ascending pair
-            const bool take_other = lower == up ? pair_gt(d, id, od, oi) : pair_gt(od, oi, d, id);
-            if (take_other) { d = od; id = oi; }
-        }
-    if (lane < m) {
-        d2[b + lane] = d;
-        ids[b + lane] = id;
-    }
-}
-
-// Bitonic sort of x[0..m) (m <= SEG_LDS) by (d64, id) in LDS.  Stages of
-// stride >= 8 take compare-exchange pairs from LDS, one pair per lane at a
-// time.  The strides 4, 2 and 1 of every size run in registers: lane t loads
-// elements [8t, 8t + 8), which hold every pair of those stages, and stores them
-// back once.  That saves LDS traffic and barriers on three stages per size.
-constexpr int SEG_E = 8;
-__device__ __forceinline__ void seg_cx(double& da, uint32_t& ia, double& db, uint32_t& ib, bool up) {
-    if (pair_gt(da, ia, db, ib) == up) {
-        const double td = da; da = db; db = td;
-        const uint32_t ti = ia; ia = ib; ib = ti;
-    }
-}
-// Register stages on lane t's eight elements.  first: sizes 2, 4 and 8 in full
-// (each 8-element chunk sorted in its bitonic direction); otherwise the
-// strides 4, 2 and 1 of `size`.
-__device__ __forceinline__ void seg_small_strides(double* sd, uint32_t* si, int P, int size, bool first) {
-    const int t = threadIdx.x, base = SEG_E * t;
-    if (base >= P) return;
-    double d[SEG_E];
-    uint32_t id[SEG_E];
-#pragma unroll
-    for (int e = 0; e < SEG_E; ++e) {
-        d[e] = sd[base + e];
-        id[e] = si[base + e];
-    }
-    if (first) {
-#pragma unroll
-        for (int sz = 2; sz <= SEG_E; sz <<= 1)
-#pragma unroll
-            for (int stride = sz >> 1; stride > 0; stride >>= 1)
-#pragma unroll
-                for (int e = 0; e < SEG_E; ++e)
-                    if ((e & stride) == 0)
-                        seg_cx(d[e], id[e], d[e + stride], id[e + stride], ((base + e) & sz) == 0);
-    } else {
-        const bool up = (base & size) == 0;  // size >= 16: one direction for the eight
-#pragma unroll
-        for (int stride = SEG_E / 2; stride > 0; stride >>= 1)
-#pragma unroll
-            for (int e = 0; e < SEG_E; ++e)
-                if ((e & stride) == 0) seg_cx(d[e], id[e], d[e + stride], id[e + stride], up);
-    }
-#pragma unroll
-    for (int e = 0; e < SEG_E; ++e) {
-        sd[base + e] = d[e];
-        si[base + e] = id[e];
-    }
-}
-
-__device__ void lds_bitonic(double* __restrict__ xd, uint32_t* __restrict__ xi, int m, double* sd, uint32_t* si) {
-    int P = SEG_E;
-    while (P < m) P <<= 1;
-    for (int j = threadIdx.x; j < P; j += blockDim.x) {
-        sd[j] = j < m ? xd[j] : INFINITY;
-        si[j] = j < m ? xi[j] : 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    // sizes 2, 4 and 8 entirely in registers
-    seg_small_strides(sd, si, P, SEG_E, true);
-    __syncthreads();
-    for (int size = 2 * SEG_E; size <= P; size <<= 1) {
-        for (int stride = size >> 1; stride >= SEG_E; stride >>= 1) {
-            for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
-                const int j = 2 * t - (t & (stride - 1)), p = j + stride;
-                const bool up = (j & size) == 0;
-                const double dj = sd[j], dp = sd[p];
-                const uint32_t ij = si[j], ip = si[p];
-                if (pair_gt(dj, ij, dp, ip) == up) {
-                    sd[j] = dp; sd[p] = dj;
-                    si[j] = ip; si[p] = ij;
-                }
+        for (int size = 2; size <= 64; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const double od = __shfl_xor(d, stride);
+                const uint32_t oi = __shfl_xor(id, stride);
+                const bool up = (lane & size) == 0;
+                const bool lower = (lane & stride) == 0;
+                // keep the smaller element on the lower lane of an ascending pair
+                const bool take_other = lower == up ? pair_gt(d, id, od, oi) : pair_gt(od, oi, d, id);
+                if (take_other) { d = od; id = oi; }
             }
-            __syncthreads();
+        if (lane < m) ids[b + lane] = id;
+    }
+}
+
+// exclusive prefix sum of a[0..n) in place by the whole block of T lanes
+template <int T>
+__device__ __forceinline__ void block_exscan(uint32_t* a, int n, uint32_t* wsum) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (n + T - 1) / T;
+    const int b = min(t * per, n), e = min(b + per, n);
+    uint32_t s = 0;
+    for (int j = b; j < e; ++j) s += a[j];
+    uint32_t x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int v = 0; v < w; ++v) before += wsum[v];
+    uint32_t run = before + x - s;
+    for (int j = b; j < e; ++j) {
+        const uint32_t v = a[j];
+        a[j] = run;
+        run += v;
+    }
+}
+
+template <int CAP, int T>
+struct SegLds {
+    double d[CAP];
+    uint32_t id[CAP];
+    uint32_t cnt[CAP / 2 + 1];
+    uint16_t perm[CAP];
+    uint32_t wsum[T / 64];
+    int flag;
+};
+
+template <class L>
+__device__ __forceinline__ bool perm_gt(const L& S, uint16_t a, uint16_t b) {
+    if (a == SEG_NONE) return b != SEG_NONE;
+    if (b == SEG_NONE) return false;
+    return pair_gt(S.d[a], S.id[a], S.d[b], S.id[b]);
+}
+
+__device__ __forceinline__ int seg_bucket(double d, double scale, int nb) {
+    const double f = d * scale;
+    return f < (double)(nb - 1) ? (int)f : nb - 1;
+}
+
+// Sorts S.d/S.id[0..m) (m <= CAP, every d <= r2) by (d64, id): on return
+// S.perm[0..m) lists the element indices in order.  Called by the whole block,
+// which must be synchronised on entry.
+template <int CAP, int T>
+__device__ void block_bucket_sort(SegLds<CAP, T>& S, int m, double r2) {
+    const int t = threadIdx.x;
+    const int nb = (m + 1) / 2;
+    const double scale = r2 > 0.0 ? (double)nb / r2 : 0.0;
+    for (int j = t; j <= nb; j += T) S.cnt[j] = 0;
+    if (t == 0) S.flag = 0;
+    __syncthreads();
+    for (int j = t; j < m; j += T) atomicAdd(&S.cnt[seg_bucket(S.d[j], scale, nb)], 1u);
+    __syncthreads();
+    block_exscan<T>(S.cnt, nb, S.wsum);
+    __syncthreads();
+    for (int j = t; j < m; j += T) S.perm[atomicAdd(&S.cnt[seg_bucket(S.d[j], scale, nb)], 1u)] = (uint16_t)j;
+    __syncthreads();
+    // cnt[b] is now the end of bucket b (the start of bucket b + 1)
+    for (int b = t; b < nb; b += T) {
+        const int st = b ? (int)S.cnt[b - 1] : 0, en = (int)S.cnt[b];
+        if (en - st > SEG_RUN) {
+            S.flag = 1;
+        } else {
+            for (int x = st + 1; x < en; ++x) {
+                const uint16_t v = S.perm[x];
+                int y = x - 1;
+                while (y >= st && perm_gt(S, S.perm[y], v)) {
+                    S.perm[y + 1] = S.perm[y];
+                    --y;
+                }
+                S.perm[y + 1] = v;
+            }
         }
-        seg_small_strides(sd, si, P, size, false);
+    }
+    __syncthreads();
+    if (S.flag) {
+        // many equal distances: a bitonic network over the whole segment
+        int P = 1;
+        while (P < m) P <<= 1;
+        for (int j = t; j < P; j += T) S.perm[j] = j < m ? (uint16_t)j : SEG_NONE;
+        __syncthreads();
+        for (int size = 2; size <= P; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int u = t; u < P / 2; u += T) {
+                    const int j = 2 * u - (u & (stride - 1)), p = j + stride;
+                    const bool up = (j & size) == 0;
+                    const uint16_t a = S.perm[j], c = S.perm[p];
+                    if (perm_gt(S, a, c) == up) {
+                        S.perm[j] = c;
+                        S.perm[p] = a;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+}
+
+// gathers positions pos[0..m) of the segment's query into S.d / S.id
+template <int CAP, int T>
+__device__ __forceinline__ void seg_gather(SegLds<CAP, T>& S, const uint32_t* pos, int m, const SegQ& q,
+                                           const float4* __restrict__ pts) {
+    for (int j = threadIdx.x; j < m; j += T) {
+        const float4 p = pts[pos[j]];
+        S.d[j] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+        S.id[j] = (uint32_t)__float_as_int(p.w);
+    }
+}
+
+// one workgroup per listed segment of up to CAP hits (a persistent grid
+// striding over the list)
+template <int CAP, int T>
+__global__ __launch_bounds__(T) void segsort_block_kernel(const int64_t* __restrict__ off,
+                                                          const int* __restrict__ list,
+                                                          const unsigned* __restrict__ count,
+                                                          const float* __restrict__ qxyz,
+                                                          const float* __restrict__ radii, float r_all,
+                                                          const float4* __restrict__ pts, uint32_t* ids,
+                                                          int64_t capacity) {
+    __shared__ SegLds<CAP, T> S;
+    const unsigned n = *count;
+    for (unsigned li = blockIdx.x; li < n; li += gridDim.x) {
+        const int g = list[li];
+        const int64_t b = off[g];
+        const int m = (int)(min(off[g + 1], capacity) - b);
+        const SegQ q = seg_query(qxyz, radii, r_all, g);
+        seg_gather(S, ids + b, m, q, pts);
+        __syncthreads();
+        block_bucket_sort(S, m, q.r2);
+        __syncthreads();
+        for (int j = threadIdx.x; j < m; j += T) ids[b + j] = S.id[S.perm[j]];
         __syncthreads();
     }
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
-        xd[j] = sd[j];
-        xi[j] = si[j];
-    }
-    __syncthreads();
 }
 
 // number of elements of the sorted run (rd, ri)[0..n) that order before (d, i)
@@ -307,33 +432,45 @@ __device__ __forceinline__ int rank_in(const double* __restrict__ rd, const uint
     return lo;
 }
 
-// One block per segment longer than 64.  Up to SEG_LDS elements: one bitonic
-// network in LDS.  Longer: SEG_LDS-element chunks are sorted in LDS, then
-// runs are merged pairwise, ping-ponging with the segment's range of the
-// merge scratch; each element finds its output slot as its index in its own
-// run plus its rank in the partner run (binary search; (d64, id) keys are
-// distinct within a segment, so ranks never collide).
-__global__ __launch_bounds__(512) void segsort_big_kernel(const int64_t* __restrict__ off, const int* big_list,
-                                                          const int64_t* big_tmp, const unsigned* big_count,
-                                                          uint32_t* ids, double* d2, int64_t capacity,
-                                                          uint32_t* tmp_ids, double* tmp_d2) {
-    __shared__ double sd[SEG_LDS];
-    __shared__ uint32_t si[SEG_LDS];
-    if (blockIdx.x >= *big_count) return;
-    const int seg = big_list[blockIdx.x];
-    const int64_t b = off[seg], e = min(off[seg + 1], capacity);
-    const int m = (int)(e - b);
-    if (m <= SEG_LDS) {
-        lds_bitonic(d2 + b, ids + b, m, sd, si);
-        return;
+// One workgroup per segment of more than SEG_LARGE hits: SEG_LARGE-element
+// chunks bucket-sorted in LDS into the merge scratch A, then runs merged
+// pairwise (A -> B -> A ...); each element finds its output slot as its index
+// in its own run plus its rank in the partner run ((d64, id) keys are distinct
+// within a segment, so ranks never collide).  The last pass writes the ids.
+constexpr int SEG_HUGE_T = 1024;
+__global__ __launch_bounds__(SEG_HUGE_T) void segsort_huge_kernel(const int64_t* __restrict__ off,
+                                                                  const int* __restrict__ list,
+                                                                  const int64_t* __restrict__ huge_tmp,
+                                                                  const unsigned* count,
+                                                                  const float* __restrict__ qxyz,
+                                                                  const float* __restrict__ radii, float r_all,
+                                                                  const float4* __restrict__ pts, uint32_t* ids,
+                                                                  int64_t capacity, double* tmp_d, uint32_t* tmp_i,
+                                                                  int64_t tmp_half) {
+    __shared__ SegLds<SEG_LARGE, SEG_HUGE_T> S;
+    if (blockIdx.x >= *count) return;
+    const int g = list[blockIdx.x];
+    const int64_t b = off[g];
+    const int m = (int)(min(off[g + 1], capacity) - b);
+    const SegQ q = seg_query(qxyz, radii, r_all, g);
+    double* ad = tmp_d + huge_tmp[blockIdx.x];
+    uint32_t* ai = tmp_i + huge_tmp[blockIdx.x];
+    double* bd = ad + tmp_half;
+    uint32_t* bi = ai + tmp_half;
+    for (int c0 = 0; c0 < m; c0 += SEG_LARGE) {
+        const int cm = min(SEG_LARGE, m - c0);
+        seg_gather(S, ids + b + c0, cm, q, pts);
+        __syncthreads();
+        block_bucket_sort(S, cm, q.r2);
+        __syncthreads();
+        for (int j = threadIdx.x; j < cm; j += SEG_HUGE_T) {
+            ad[c0 + j] = S.d[S.perm[j]];
+            ai[c0 + j] = S.id[S.perm[j]];
+        }
+        __syncthreads();
     }
-    for (int c0 = 0; c0 < m; c0 += SEG_LDS) lds_bitonic(d2 + b + c0, ids + b + c0, min(SEG_LDS, m - c0), sd, si);
-    double* ad = d2 + b;
-    uint32_t* ai = ids + b;
-    double* bd = tmp_d2 + big_tmp[blockIdx.x];
-    uint32_t* bi = tmp_ids + big_tmp[blockIdx.x];
-    for (int w = SEG_LDS; w < m; w <<= 1) {
-        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    for (int w = SEG_LARGE; w < m; w <<= 1) {
+        for (int j = threadIdx.x; j < m; j += SEG_HUGE_T) {
             const int run = j / w, pos = j - run * w;
             const int pbase = (run ^ 1) * w;
             const int base = min(run, run ^ 1) * w;
@@ -346,12 +483,21 @@ __global__ __launch_bounds__(512) void segsort_big_kernel(const int64_t* __restr
         double* td = ad; ad = bd; bd = td;
         uint32_t* ti = ai; ai = bi; bi = ti;
     }
-    if (ad != d2 + b) {
-        for (int j = threadIdx.x; j < m; j += blockDim.x) {
-            d2[b + j] = ad[j];
-            ids[b + j] = ai[j];
-        }
-    }
+    for (int j = threadIdx.x; j < m; j += SEG_HUGE_T) ids[b + j] = ai[j];
+}
+
+template <class K>
+static int persistent_blocks(K kernel, int threads, int num_cus, std::atomic<int>* cache) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int g = cache[dev & 63].load(std::memory_order_relaxed);
+    if (g > 0) return g;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    g = per_cu * num_cus;
+    cache[dev & 63].store(g, std::memory_order_relaxed);
+    return g;
 }
 
 // ---------------------------------------------------------------------------
@@ -359,9 +505,12 @@ __global__ __launch_bounds__(512) void segsort_big_kernel(const int64_t* __restr
 // ---------------------------------------------------------------------------
 int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
               int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
-              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii) {
+              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii,
+              bool count) {
+    (void)d2_buf;
     hipStream_t s = ctx->stream;
     unsigned* counters = ctx->counters.as<unsigned>();
+    unsigned long long* stats64 = (unsigned long long*)(counters + CNT_TOTAL);
     GPRT_TRY(keys_buf.reserve(8ull * nq));
     GPRT_TRY(perm_buf.reserve(std::max<size_t>(4ull * nq, query_prep_scratch_bytes(1, 1))));
     GPRT_TRY(qs4_buf.reserve(16ull * nq));
@@ -381,10 +530,18 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     GPRT_HIP_TRY(hipStreamSynchronize(s));
     const int grid = (nq + 255) / 256;
     const int max_steps = T.n + 64;
+    const Node* nodes = T.nodes.as<Node>();
+    const float4* pts = T.pts4.as<float4>();
+    const float4* qs4 = qs4_buf.as<float4>();
     if (!d_ids) {
-        hipLaunchKernelGGL(range_kernel<false>, dim3(grid), dim3(256), 0, s, T.nodes.as<Node>(),
-                           T.pts4.as<float4>(), root, qs4_buf.as<float4>(), nq, r, d_radii, counts_buf.as<int64_t>(),
-                           nullptr, nullptr, nullptr, (int64_t)0, max_steps, counters + CNT_ERROR);
+        if (count)
+            hipLaunchKernelGGL((range_kernel<false, true>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
+                               d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, max_steps,
+                               counters + CNT_ERROR, stats64);
+        else
+            hipLaunchKernelGGL((range_kernel<false, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq,
+                               r, d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, max_steps,
+                               counters + CNT_ERROR, stats64);
         int64_t* sums = sums_buf.as<int64_t>();
         int64_t* total = sums + tiles;
         hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(256), 0, s, counts_buf.as<int64_t>(), nq, sums);
@@ -396,32 +553,47 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
         GPRT_HIP_TRY(hipStreamSynchronize(s));
         return GPRT_OK;
     }
-    GPRT_TRY(d2_buf.reserve(8ull * std::max<int64_t>(capacity, 1)));
-    GPRT_TRY(big_buf.reserve(4ull * (nq + 2) + 8ull * nq + 16));  // segment list, then merge-scratch offsets
-    hipLaunchKernelGGL(range_kernel<true>, dim3(grid), dim3(256), 0, s, T.nodes.as<Node>(), T.pts4.as<float4>(),
-                       root, qs4_buf.as<float4>(), nq, r, d_radii, nullptr, d_off, d_ids, d2_buf.as<double>(), capacity,
-                       max_steps, counters + CNT_ERROR);
-    // counters: [CNT_FALLBACK] long-segment count, stats64[3] their merge-scratch total
-    unsigned* big_count = counters + CNT_FALLBACK;
-    unsigned long long* big_total = (unsigned long long*)(counters + CNT_TOTAL) + 3;
-    int64_t* big_tmp = (int64_t*)(big_buf.as<char>() + sizeof(int) * (((size_t)nq + 1) & ~(size_t)1));
-    hipLaunchKernelGGL(segsort_small_kernel, dim3((nq + 3) / 4), dim3(256), 0, s, d_off, nq, d_ids,
-                       d2_buf.as<double>(), capacity, big_buf.as<int>(), big_tmp, big_count, big_total);
+    // segment lists (4 x nq ints) and the huge segments' merge-scratch offsets
+    GPRT_TRY(big_buf.reserve(16ull * nq + 8ull * nq + 16));
+    int* lists = big_buf.as<int>();
+    int64_t* huge_tmp = (int64_t*)(lists + 4 * (size_t)nq);
+    if (count)
+        hipLaunchKernelGGL((range_kernel<true, true>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
+                           d_radii, nullptr, d_off, d_ids, capacity, max_steps, counters + CNT_ERROR, stats64);
+    else
+        hipLaunchKernelGGL((range_kernel<true, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
+                           d_radii, nullptr, d_off, d_ids, capacity, max_steps, counters + CNT_ERROR, stats64);
+    // list lengths in counters the query sort does not touch (cleared above);
+    // stats64[3]: the huge segments' total
+    unsigned* cnt = counters + CNT_AUX;
+    unsigned long long* huge_total = stats64 + 3;
+    hipLaunchKernelGGL(seg_classify_kernel, dim3(grid), dim3(256), 0, s, d_off, nq, capacity, pts, d_ids, lists, cnt,
+                       huge_tmp, huge_total);
+    static std::atomic<int> g_small[64], g_mid[64], g_large[64];
+    hipLaunchKernelGGL(segsort_small_kernel, dim3(persistent_blocks(segsort_small_kernel, 256, ctx->num_cus, g_small)),
+                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_SMALL * nq, cnt + SEG_L_SMALL, d_q, d_radii, r,
+                       pts, d_ids, capacity);
+    hipLaunchKernelGGL((segsort_block_kernel<SEG_MID, 256>),
+                       dim3(persistent_blocks(segsort_block_kernel<SEG_MID, 256>, 256, ctx->num_cus, g_mid)),
+                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_MID * nq, cnt + SEG_L_MID, d_q, d_radii, r, pts,
+                       d_ids, capacity);
+    hipLaunchKernelGGL((segsort_block_kernel<SEG_LARGE, 1024>),
+                       dim3(persistent_blocks(segsort_block_kernel<SEG_LARGE, 1024>, 1024, ctx->num_cus, g_large)),
+                       dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_LARGE * nq, cnt + SEG_L_LARGE, d_q, d_radii, r,
+                       pts, d_ids, capacity);
     struct {
-        unsigned nbig;
+        unsigned nhuge;
         unsigned long long total;
     } h{0, 0};
-    GPRT_HIP_TRY(hipMemcpyAsync(&h.nbig, big_count, 4, hipMemcpyDeviceToHost, s));
-    GPRT_HIP_TRY(hipMemcpyAsync(&h.total, big_total, 8, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(&h.nhuge, cnt + SEG_L_HUGE, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(&h.total, huge_total, 8, hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipStreamSynchronize(s));
-    if (h.nbig) {
-        if (h.total) {
-            GPRT_TRY(ctx->sort_tmp[0].reserve(8ull * h.total));
-            GPRT_TRY(ctx->sort_tmp[1].reserve(4ull * h.total));
-        }
-        hipLaunchKernelGGL(segsort_big_kernel, dim3(h.nbig), dim3(512), 0, s, d_off, big_buf.as<int>(), big_tmp,
-                           big_count, d_ids, d2_buf.as<double>(), capacity, ctx->sort_tmp[1].as<uint32_t>(),
-                           ctx->sort_tmp[0].as<double>());
+    if (h.nhuge) {
+        GPRT_TRY(ctx->sort_tmp[0].reserve(16ull * h.total));
+        GPRT_TRY(ctx->sort_tmp[1].reserve(8ull * h.total));
+        hipLaunchKernelGGL(segsort_huge_kernel, dim3(h.nhuge), dim3(SEG_HUGE_T), 0, s, d_off,
+                           lists + (size_t)SEG_L_HUGE * nq, huge_tmp, cnt + SEG_L_HUGE, d_q, d_radii, r, pts, d_ids,
+                           capacity, ctx->sort_tmp[0].as<double>(), ctx->sort_tmp[1].as<uint32_t>(), (int64_t)h.total);
     }
     GPRT_HIP_TRY(hipGetLastError());
     return GPRT_OK;

@@ -211,6 +211,35 @@ def test_range_long_segments(ctx):
         _check_range(off, ids, *oracle.range_brute(pts, q, r))
 
 
+def test_range_segment_tiers(ctx):
+    """Every sort tier at its edges: points at the integers of a line, query i at
+    x = 20000.5 with r = m/2 hits exactly m points, in pairs at equal distance
+    (ties broken by id), for m on both sides of 1, 64, 2,048 and 8,192 and
+    beyond (chunks plus merges); plus a segment of 9,000 coincident points
+    (one bucket: the bitonic network, and merges of all-equal distances)."""
+    x = np.arange(40_001, dtype=np.float32)
+    line = np.c_[x, np.zeros((len(x), 2), np.float32)].astype(np.float32)
+    dup = np.tile(np.float32([[5e4, 7.0, 7.0]]), (9_000, 1))
+    pts = np.concatenate([line, dup])
+    t = ctx.build(pts, leaf_size=8)
+    try:
+        for m in (2, 64, 66, 2048, 2050, 8192, 8194, 17_000):
+            q = np.float32([[20000.5, 0.0, 0.0]])
+            r = float(np.float32(m / 2))
+            off, ids = t.range_query(q, r)
+            assert off[-1] == m, (m, off)
+            _check_range(off, ids, *oracle.range_brute(pts, q, r))
+        q = np.float32([[20000.0, 0.0, 0.0]])  # one hit
+        off, ids = t.range_query(q, 0.25)
+        assert off[-1] == 1
+        _check_range(off, ids, *oracle.range_brute(pts, q, 0.25))
+        q = np.float32([[5e4, 7.0, 7.0], [5e4, 7.0, 7.5]])
+        for r in (0.0, 0.5):
+            _check_range(*t.range_query(q, r), *oracle.range_brute(pts, q, r))
+    finally:
+        t.close()
+
+
 def test_range_lidar_subset(ctx):
     pts, q = D.lidar_scan(frames=4, nq=20_000)
     t = ctx.build(pts)
