@@ -1,6 +1,7 @@
 """Headline benchmark: exact k-NN, k = 5, on the LiDAR-like workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--leaf L] [--no-cpu] [--no-extra] [--all]
+                    [--traffic auto|off] [--only range|barnes-hut|build]
 
 Workload: 1,310,720 float32 points (20 frames x 64 rings x 1,024 azimuths of a
 ground-plane + boxes + 120 m cylinder scene, 2 cm range noise) and 400,000
@@ -10,24 +11,33 @@ k-NN pass of all queries over a prebuilt LBVH, with the queries already
 resident in HBM and the ids left resident in HBM.
 
 Prints one JSON line.  `value` is queries/s of the batched FP32 + FP64-recheck
-form with the default schedule (device-resident inputs, K steps enqueued back
-to back between two synchronisations); the plain single-launch FP64 form is
-timed the same way on the same tree.  The roofline entry is for the dominant
-kernel (the first traversal round): algorithmic bytes counted by an
-instrumented run, divided by the kernel's duration timed with HIP events on the
-library stream.  The CPU baseline is the oracle's multi-threaded KD-tree on this
-host's cores.  `details` adds, on the same data: the autotuner's pick, the
-end-to-end time with the build and host transfers, k = 32 and the r = 0.5 m
-range query (skipped with --no-extra); --all adds the road-network, batched
-multi-cloud and Barnes-Hut workloads.
+form (device-resident inputs, K steps enqueued back to back between two
+synchronisations), with the default schedule or the autotuner's pick when that
+is faster at identical ids; the plain single-launch FP64 form is timed the same
+way on the same tree.  The roofline entry is for the dominant kernel (the first
+traversal round): algorithmic bytes counted by an instrumented run, divided by
+the kernel's duration timed with HIP events on the library stream; `traffic` is
+the kernel's HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE,
+WRITE_SIZE) run as child processes of this run, scaled by what the same counters
+read for a 64 MiB streaming copy kernel in those passes (null when rocprofv3 is absent or
+--traffic off).  The CPU baseline is the oracle's multi-threaded KD-tree on this
+host's cores.  `details` adds, on the same data: the end-to-end time with the
+build and host transfers, k = 32, the r = 0.5 m range query and the LBVH
+rebuild, each with its own roofline (skipped with --no-extra); --all adds the
+road-network, batched multi-cloud and Barnes-Hut workloads.  --only runs one
+secondary workload alone and prints its own line (for profiling).
 """
 import argparse
+import csv
+import glob
 import hashlib
 import json
 import os
 import platform
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -37,7 +47,10 @@ import numpy as np  # noqa: E402
 
 METRIC = "exact kNN queries/s, k=5, 1.3M pts: batched FP32+FP64-recheck vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 K = 5
+ROUND0 = "knn_lane_kernel<8, 3, 8, false, true>"
+COPY_BYTES = 64 << 20
 
 
 def _sha(*arrays):
@@ -69,6 +82,62 @@ def _cpu_model():
     return platform.processor() or "unknown"
 
 
+def _roof(alg_bytes, ms, **extra):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return dict({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "alg_bytes": int(alg_bytes), "kernel_ms": ms}, **extra)
+
+
+# ---------------------------------------------------------------- HBM traffic
+def _pmc_pass(counter, tmp):
+    """One rocprofv3 --pmc pass over `bench.py --only knn-round0` in a child
+    process; returns {kernel name: mean counter value (KiB) per dispatch}."""
+    d = os.path.join(tmp, counter.lower())
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__), "--only", "knn-round0", "--traffic", "off"]
+    env = dict(os.environ, TMPDIR=tmp)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=env, cwd=tmp)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"{counter}: no counter file (rc {r.returncode}): {r.stderr[-400:]}")
+    acc = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            acc.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    shown = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", "<tmp>", "-o", "run", "--", "python3",
+             "bench.py", "--only", "knn-round0", "--traffic", "off"]
+    return {k: sum(v) / len(v) for k, v in acc.items()}, " ".join(shown)
+
+
+def _traffic():
+    """HBM bytes per launch of the first traversal round from two counter
+    passes, each scaled by the counter's reading for a plain 64 MiB copy."""
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(prefix="gprt_pmc_")
+    try:
+        fetch, c1 = _pmc_pass("FETCH_SIZE", tmp)
+        write, c2 = _pmc_pass("WRITE_SIZE", tmp)
+        kern = [k for k in fetch if ROUND0 in k]
+        copy = [k for k in fetch if "stream_copy_kernel" in k]
+        if not kern or not copy:
+            return None, f"kernels not found in the counter files: {sorted(fetch)[:6]}"
+        kib = COPY_BYTES / 1024.0
+        fs = kib / fetch[copy[0]]     # the copy reads COPY_BYTES
+        ws = kib / write[copy[0]]     # and writes COPY_BYTES
+        t = {"hbm_bytes_per_launch": 1024.0 * (fetch[kern[0]] * fs + write[kern[0]] * ws),
+             "fetch_kib": fetch[kern[0]], "write_kib": write[kern[0]], "fetch_scale": fs, "write_scale": ws,
+             "write_bytes_per_launch": 1024.0 * write[kern[0]] * ws, "commands": [c1, c2]}
+        return t, None
+    except Exception as e:  # the line still prints; the reason is recorded
+        return None, f"{type(e).__name__}: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+# ---------------------------------------------------------------- k-NN helpers
 def _knn_pair(ctx, tree, dq, k, cfg, steps, warmup):
     """(batched ms/step, plain ms/step, batched ids, plain ids) on device arrays."""
     import gprt_hip
@@ -85,36 +154,101 @@ def _knn_pair(ctx, tree, dq, k, cfg, steps, warmup):
     return ms_b, ms_p, out[0], out[1], st
 
 
-def _extra_lidar(ctx, tree, pts, q, dq, cfg, args, oracle):
-    """k = 32 and the r = 0.5 m range query on the headline data."""
-    ex = {}
-    steps = max(5, args.steps // 5)
-    ms_b, ms_p, ib, ip, st = _knn_pair(ctx, tree, dq, 32, cfg, steps, 2)
-    ref, _, _, _ = oracle.kdtree_knn(pts, q, 32)
-    ex["knn_k32"] = {"batched_qps": len(q) / (ms_b * 1e-3), "plain_qps": len(q) / (ms_p * 1e-3),
-                     "speedup_vs_plain": ms_p / ms_b, "ms_batched": ms_b, "ms_plain": ms_p,
-                     "fallback": st["fallback_queries"], "rounds": st["rounds_run"],
-                     "batched_eq_oracle": bool(np.array_equal(ib, ref)),
-                     "plain_eq_oracle": bool(np.array_equal(ip, ref))}
-    # range query r = 0.5 m: device time of the fill call (Morton order, FP32
-    # traversal with FP64 confirmation, CSR write, per-query (d64, id) sort)
-    t0 = time.perf_counter()
-    off, ids = tree.range_query(q, 0.5)
-    wall = time.perf_counter() - t0
-    ms = ctx.stats()["ms_query"]
-    lens = np.diff(off)
-    ok_sorted = True
-    # spot-check 2,000 queries against the oracle's brute force (every hit within r, (d64, id) order, complete)
+def _range_line(ctx, tree, pts, q, oracle, reps=3):
+    """r = 0.5 m over the LiDAR scan: both calls timed (count, then fill and
+    per-query sort), the fill walk alone for the roofline, counts from an
+    instrumented run, 2,000 sampled queries against FP64 brute force."""
+    import gprt_hip
+    nq = len(q)
+    tot, cnt_ms, fill_ms, walk_ms, cwalk_ms = [], [], [], [], []
+    off = ids = None
+    for _ in range(reps + 1):
+        st = []
+        off, ids = tree.range_query(q, 0.5, flags=gprt_hip.TIME_KERNELS, stats=st)
+        cnt_ms.append(st[0]["ms_query"])
+        fill_ms.append(st[1]["ms_query"])
+        cwalk_ms.append(st[0]["ms_kernel"])
+        walk_ms.append(st[1]["ms_kernel"])
+        tot.append(st[0]["ms_query"] + st[1]["ms_query"])
+    med = lambda v: float(np.median(v[1:]))  # noqa: E731
+    st = []
+    tree.range_query(q, 0.5, flags=gprt_hip.COLLECT_COUNTERS, stats=st)
+    V, P = st[1]["node_visits"], st[1]["point_tests"]
+    hits = int(off[-1])
+    # fill walk: 16 B query + 64 B per node visit + 16 B per point test + 4 B
+    # position written per hit + 16 B of offsets per query
+    alg = 16 * nq + 64 * V + 16 * P + 4 * hits + 16 * nq
     rng = np.random.default_rng(11)
-    sel = np.sort(rng.choice(len(q), 2000, replace=False))
+    sel = np.sort(rng.choice(nq, 2000, replace=False))
     roff, rids = oracle.range_brute(pts, q[sel], 0.5)
     got = np.concatenate([ids[off[i]:off[i + 1]] for i in sel])
-    ok_sorted = bool(np.array_equal(np.diff(roff), lens[sel]) and np.array_equal(got, rids))
-    ex["range_r0.5"] = {"qps": len(q) / (ms * 1e-3), "ms_device": ms, "wall_s_incl_transfers": wall,
-                        "results": int(off[-1]), "mean_per_query": float(lens.mean()), "max_per_query": int(lens.max()),
-                        "results_per_s": float(off[-1]) / (ms * 1e-3),
-                        "sample_eq_oracle": ok_sorted, "sample": "2,000 random queries vs FP64 brute force"}
-    return ex
+    ok = bool(np.array_equal(np.diff(off)[sel], np.diff(roff)) and np.array_equal(got, rids))
+    return {"workload": "lidar_scan range r=0.5 m, 1,310,720 pts x 400,000 queries",
+            "ms_count_plus_fill": med(tot), "ms_count_call": med(cnt_ms), "ms_fill_call": med(fill_ms),
+            "qps": nq / (med(tot) * 1e-3), "results": hits, "results_per_s": hits / (med(tot) * 1e-3),
+            "mean_per_query": hits / nq, "max_per_query": int(np.diff(off).max()),
+            "roofline": _roof(alg, med(walk_ms), kernel="range fill walk (range_kernel<true>)", node_visits=V,
+                              point_tests=P, count_walk_ms=med(cwalk_ms)),
+            "sample_eq_oracle": ok, "sample": "2,000 random queries vs FP64 brute force"}
+
+
+def _build_line(ctx, dp, n, leaf, reps=7):
+    """LBVH rebuild from device-resident points (the context keeps the last
+    destroyed tree's buffers).  Bytes per phase: bounds 12N read, keys 12N
+    read + 12N written, 8 radix passes over (8 B key, 4 B value) 24N each,
+    gather 16N read + 16N + 16N written (sorted and input-order float4),
+    Karras ~8 log2(N) N read + 40N written, refit 16N + 48N, 4-wide view 64N
+    read + 128N written."""
+    ms = []
+    for _ in range(reps + 1):
+        t = ctx.build(dp, leaf_size=leaf)
+        ms.append(ctx.stats()["ms_build"])
+        t.close()
+    med = float(np.median(ms[1:]))
+    alg = n * (12 + 24 + 8 * 24 + 48 + 8 * np.log2(n) + 40 + 64 + 192)
+    return {"workload": f"LBVH rebuild, {n} device-resident points, leaf {leaf}", "ms": med,
+            "roofline": _roof(alg, med, kernel="LBVH build (all kernels)")}
+
+
+def _bh_line(ctx, pm, oracle, thetas=(0.3, 0.5, 0.7), check=True):
+    """1M Plummer bodies: force kernel time per theta (HIP events), counts from
+    an instrumented run: bytes = per body 16 + 96 per node opened (64 B node +
+    32 B moments) + 20 per body interaction (float4 + mass) + 12 written;
+    FP32 work = 30 flops per interaction (body or far node: 3 sub, 3 fma,
+    sqrt, div, 4 mul, 3 mul, 3 compensated adds of 4) + 32 per node opened
+    (two opening tests)."""
+    import gprt_hip
+    n = len(pm)
+    dpm = gprt_hip.DeviceArray.from_host(ctx, pm)
+    out = {"workload": "barnes-hut plummer 1,000,000 bodies, eps=1e-3, G=1, leaf 8"}
+    ref = sample = None
+    if check:
+        sample = np.sort(np.random.default_rng(5).choice(n, 10_000, replace=False))
+        ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3)
+    for theta in thetas:
+        ms = []
+        for _ in range(4):
+            ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0).free()
+            ms.append(ctx.stats()["ms_query"])
+        acc = ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0, flags=gprt_hip.COLLECT_COUNTERS)
+        st = ctx.stats()
+        V, L, F = st["node_visits"], st["point_tests"], st["rerank_reads"]
+        t = float(np.median(ms[1:]))
+        alg = 16 * n + 96 * V + 20 * L + 12 * n
+        flops = 30 * (L + F) + 32 * V
+        line = {"ms_force": t, "bodies_per_s": n / (t * 1e-3),
+                "roofline": _roof(alg, t, kernel="bh_force_kernel", nodes_opened=V, body_interactions=L,
+                                  far_interactions=F, fp32_tflops=flops / (t * 1e-3) / 1e12,
+                                  fp32_frac=flops / (t * 1e-3) / 1e12 / FP32_PEAK_TFLOPS)}
+        if check:
+            a = acc.to_host()[sample].astype(np.float64)
+            err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            line.update(rel_err_median=float(np.median(err)), rel_err_p99=float(np.percentile(err, 99)),
+                        rel_err_max=float(err.max()))
+        acc.free()
+        out[str(theta)] = line
+    dpm.free()
+    return out
 
 
 def _extra_all(ctx, args, oracle):
@@ -123,7 +257,6 @@ def _extra_all(ctx, args, oracle):
     from gprt_hip import datasets as D
     ex = {}
     steps = max(5, args.steps // 5)
-    # road network: 434K points x 400K queries
     pts, q = D.road_network()
     dp, dq = gprt_hip.DeviceArray.from_host(ctx, pts), gprt_hip.DeviceArray.from_host(ctx, q)
     tree = ctx.build(dp, leaf_size=8)
@@ -136,7 +269,6 @@ def _extra_all(ctx, args, oracle):
     tree.close()
     dp.free()
     dq.free()
-    # batched multi-cloud: 256 clouds, k in {1, 5, 16, 64}
     pts, poff, q, qoff, kc = D.clustered_batch()
     tree = ctx.build_batched(pts, poff, leaf_size=8)
     cfg = gprt_hip.default_config()
@@ -146,64 +278,51 @@ def _extra_all(ctx, args, oracle):
         tree.k_nn_batched(q, qoff, kc, cfg)
         times.append(ctx.stats()["ms_query"])
     ms = float(np.median(times))
-    # parity on 16 clouds spread over the size range (per-cloud KD-tree oracle)
     ok = True
-    for c in np.linspace(0, len(kc) - 1, 16).astype(int):
+    for c in range(len(kc)):
         k = int(kc[c])
         r, _, _, _ = oracle.kdtree_knn(pts[poff[c]:poff[c + 1]], q[qoff[c]:qoff[c + 1]], k)
-        got = ids[out_off[c]:out_off[c + 1]].reshape(-1, k)
-        ok &= bool(np.array_equal(got, r))
+        ok &= bool(np.array_equal(ids[out_off[c]:out_off[c + 1]].reshape(-1, k), r))
     ex["batched_clouds"] = {"clouds": len(kc), "points": len(pts), "queries": len(q), "ms_device": ms,
-                            "qps": len(q) / (ms * 1e-3), "sample_eq_oracle": ok,
-                            "sample": "16 clouds across the size range vs the KD-tree oracle"}
+                            "qps": len(q) / (ms * 1e-3), "all_clouds_eq_oracle": ok}
     tree.close()
-    # Barnes-Hut: 1M-body Plummer sphere, theta in {0.3, 0.5, 0.7}, error on 10,000 sampled bodies
-    pm = D.plummer()
-    dpm = gprt_hip.DeviceArray.from_host(ctx, pm)
-    sample = np.sort(np.random.default_rng(5).choice(len(pm), 10_000, replace=False))
-    ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3)
-    bh = {}
-    for theta in (0.3, 0.5, 0.7):
-        acc = ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0)
-        st = ctx.stats()
-        a = acc.to_host()[sample].astype(np.float64)
-        err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        bh[str(theta)] = {"ms_force": st["ms_query"], "ms_total": st["ms_total"], "bodies_per_s": len(pm) / (st["ms_query"] * 1e-3),
-                          "rel_err_median": float(np.median(err)), "rel_err_p99": float(np.percentile(err, 99)),
-                          "rel_err_max": float(err.max())}
-        acc.free()
-    ex["barnes_hut_plummer_1m"] = bh
-    dpm.free()
+    ex["barnes_hut_plummer_1m"] = _bh_line(ctx, D.plummer(), oracle)
     return ex
 
 
 def _only(args):
-    """One secondary workload alone, for profiling: the r = 0.5 m range query on
-    the LiDAR data, or Barnes-Hut on the 1M-body Plummer sphere.  Prints one
-    JSON line for that workload (not the headline metric)."""
+    """One workload alone (for profiling): the r = 0.5 m range query on the
+    LiDAR data, Barnes-Hut on the 1M-body Plummer sphere, the LBVH rebuild, or
+    (knn-round0, used by the counter passes) the first traversal round of the
+    headline alone plus a 64 MiB streaming copy (gprt_hip_debug_stream_copy)."""
     import gprt_hip
     from gprt_hip import datasets as D
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
     ctx = gprt_hip.Context(0)
-    steps = max(2, args.steps)
     if args.only == "range":
         pts, q = D.lidar_scan()
         tree = ctx.build(pts, leaf_size=args.leaf)
-        ms = []
-        for _ in range(steps + 1):
-            off, ids = tree.range_query(q, 0.5)
-            ms.append(ctx.stats()["ms_query"])
-        line = {"workload": "lidar_scan range r=0.5 m, 1,310,720 pts x 400,000 queries", "steps": steps,
-                "ms_device_fill_call": float(np.median(ms[1:])), "results": int(off[-1])}
+        line = _range_line(ctx, tree, pts, q, oracle, reps=max(2, args.steps))
+    elif args.only == "build":
+        pts, _ = D.lidar_scan()
+        dp = gprt_hip.DeviceArray.from_host(ctx, pts)
+        line = _build_line(ctx, dp, len(pts), args.leaf, reps=max(3, args.steps))
+    elif args.only == "knn-round0":
+        pts, q = D.lidar_scan()
+        dp, dq = gprt_hip.DeviceArray.from_host(ctx, pts), gprt_hip.DeviceArray.from_host(ctx, q)
+        ids = gprt_hip.DeviceArray(ctx, (len(q), K), np.uint32)
+        tree = ctx.build(dp, leaf_size=args.leaf)
+        cfg = gprt_hip.default_config(use_graph=0)
+        for _ in range(max(2, args.steps)):
+            tree.k_nn(dq, K, cfg, ids_out=ids)
+        a = gprt_hip.DeviceArray(ctx, (COPY_BYTES // 4,), np.uint32)
+        b = gprt_hip.DeviceArray(ctx, (COPY_BYTES // 4,), np.uint32)
+        for _ in range(3):
+            gprt_hip.lib().gprt_hip_debug_stream_copy(ctx.h, b.ptr, a.ptr, COPY_BYTES)
+        line = {"workload": "knn round 0 (lidar_scan, k=5) and a 64 MiB device copy, for counter passes"}
     else:
-        pm = D.plummer()
-        dpm = gprt_hip.DeviceArray.from_host(ctx, pm)
-        line = {"workload": "barnes-hut plummer 1,000,000 bodies, eps=1e-3, G=1", "steps": steps}
-        for theta in (0.3, 0.5, 0.7):
-            ms = []
-            for _ in range(steps + 1):
-                ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0).free()
-                ms.append(ctx.stats()["ms_query"])
-            line[f"ms_force_theta_{theta}"] = float(np.median(ms[1:]))
+        line = _bh_line(ctx, D.plummer(), oracle, check=not args.no_cpu)
     print(json.dumps(line))
     return 0
 
@@ -215,10 +334,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--leaf", type=int, default=8, help="leaf size of the LBVH")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip autotune, end-to-end, k=32 and range")
+    ap.add_argument("--no-extra", action="store_true", help="skip autotune, end-to-end, k=32, range and build")
     ap.add_argument("--all", action="store_true", help="also the road, batched-cloud and Barnes-Hut workloads")
-    ap.add_argument("--only", choices=["range", "barnes-hut"],
-                    help="run one secondary workload alone (for profiling) and print its JSON line")
+    ap.add_argument("--traffic", choices=["auto", "off"], default="auto",
+                    help="measure the round kernel's HBM bytes with two rocprofv3 --pmc child passes")
+    ap.add_argument("--only", choices=["range", "barnes-hut", "build", "knn-round0"],
+                    help="run one workload alone (for profiling) and print its JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -226,6 +347,10 @@ def main():
         return 2
     if args.only:
         return _only(args)
+
+    # the counter passes run first, as child processes, before this process
+    # touches the GPU
+    traffic, traffic_note = (None, "--traffic off") if args.traffic == "off" else _traffic()
 
     import gprt_hip
     from gprt_hip import datasets as D
@@ -244,59 +369,60 @@ def main():
 
     leaf = args.leaf
     cfg = gprt_hip.default_config()
-    cfg.use_graph = 1
     tree = ctx.build(dp, leaf_size=leaf)
     build_ms = ctx.stats()["ms_build"]
 
-    def batched(sync):
-        tree.k_nn(dq, K, cfg, ids_out=ids_b, sync=sync)
+    def batched_on(t, c):
+        return lambda sync: t.k_nn(dq, K, c, ids_out=ids_b, sync=sync)
 
-    def plain(sync):
-        tree.k_nn_plain(dq, K, ids_out=ids_p, sync=sync)
-
-    ms_b = _time_steps(ctx, batched, args.steps, args.warmup)
-    batched(True)
+    ms_b = _time_steps(ctx, batched_on(tree, cfg), args.steps, args.warmup)
+    tree.k_nn(dq, K, cfg, ids_out=ids_b)
     st_b = ctx.stats()
-    ms_p = _time_steps(ctx, plain, args.steps, args.warmup)
-    plain(True)
+    got_default = ids_b.to_host()
+    ms_p = _time_steps(ctx, lambda sync: tree.k_nn_plain(dq, K, ids_out=ids_p, sync=sync), args.steps,
+                       args.warmup)
+    tree.k_nn_plain(dq, K, ids_out=ids_p)
     ms_p_dev = ctx.stats()["ms_query"]
+
+    # the autotuner's pick, timed the same way; adopted when faster at equal ids
+    auto = None
+    head_tree, head_cfg, head_leaf, head_ms = tree, cfg, leaf, ms_b
+    if not args.no_extra:
+        t0 = time.perf_counter()
+        acfg, aleaf = ctx.autotune(dp, dq, K, budget_trials=14)
+        tune_s = time.perf_counter() - t0
+        atree = tree if aleaf == leaf else ctx.build(dp, leaf_size=aleaf)
+        ms_a = _time_steps(ctx, batched_on(atree, acfg), args.steps, args.warmup)
+        atree.k_nn(dq, K, acfg, ids_out=ids_b)
+        same = bool(np.array_equal(ids_b.to_host(), got_default))
+        adopt = same and ms_a < ms_b * 0.99
+        auto = {"seconds": tune_s, "leaf_size": aleaf, "schedule": acfg.as_dict(), "ms_per_step": ms_a,
+                "default_ms_per_step": ms_b, "ids_equal_default": same, "adopted": adopt}
+        if adopt:
+            head_tree, head_cfg, head_leaf, head_ms = atree, acfg, aleaf, ms_a
+        elif atree is not tree:
+            atree.close()
 
     # roofline of the dominant kernel (first traversal round): counts from an
     # instrumented run, duration from HIP events around the uninstrumented kernel
-    ccfg = gprt_hip.default_config(**cfg.as_dict())
+    ccfg = gprt_hip.default_config(**head_cfg.as_dict())
     ccfg.use_graph, ccfg.collect_counters = 0, 1
-    tree.k_nn(dq, K, ccfg, ids_out=ids_b)
+    head_tree.k_nn(dq, K, ccfg, ids_out=ids_b)
     cnt = ctx.stats()
-    tcfg = gprt_hip.default_config(**cfg.as_dict())
+    tcfg = gprt_hip.default_config(**head_cfg.as_dict())
     tcfg.use_graph, tcfg.time_kernels = 0, 1
     kms = []
     for _ in range(max(5, args.steps // 5)):
-        tree.k_nn(dq, K, tcfg, ids_out=ids_b)
+        head_tree.k_nn(dq, K, tcfg, ids_out=ids_b)
         kms.append(ctx.stats()["ms_kernel"])
     k_ms = float(np.mean(kms))
     r0 = cnt["active_per_round"][0]
-    # per query: 16 B query + 32 B per AABB test (two per node visit) + 16 B
-    # per point test; + 4k B of ids written + 16 B per FP64 re-rank read
-    alg_bytes = 16 * r0 + 32 * 2 * cnt["node_visits"] + 16 * cnt["point_tests"] + 4 * K * nq + 16 * cnt["rerank_reads"]
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-
-    # a device-to-device copy of known size: the reference point for scaling
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes
-    copy_bytes = 64 << 20
-    cbuf_a = gprt_hip.DeviceArray(ctx, (copy_bytes // 4,), np.uint32)
-    cbuf_b = gprt_hip.DeviceArray(ctx, (copy_bytes // 4,), np.uint32)
-    gprt_hip.lib().gprt_hip_copy(ctx.h, cbuf_b.ptr, cbuf_a.ptr, copy_bytes, 2)
-    ctx.sync()
-    cbuf_a.free()
-    cbuf_b.free()
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "knn_round0_lidar_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # per query: 16 B query + 64 B per node visit (two child boxes with refs) +
+    # 16 B per point test; + 4k B of ids written + 16 B per FP64 re-rank read
+    alg_bytes = 16 * r0 + 64 * cnt["node_visits"] + 16 * cnt["point_tests"] + 4 * K * nq + 16 * cnt["rerank_reads"]
 
     # parity of both forms against the oracle (CPU KD-tree, pinned to brute force)
-    batched(True)
+    head_tree.k_nn(dq, K, head_cfg, ids_out=ids_b)
     got_b = ids_b.to_host()
     got_p = ids_p.to_host()
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -321,14 +447,10 @@ def main():
     details = {"ms_device_batched": st_b["ms_query"], "rounds": st_b["rounds_run"],
                "active_per_round": st_b["active_per_round"][:8], "fallback": st_b["fallback_queries"],
                "build_ms": build_ms, "datagen_s": gen_s, "data_sha256_16": _sha(pts, q),
-               "seeds": {"lidar_scan": 3}}
+               "seeds": {"lidar_scan": 3}, "traffic_note": traffic_note}
+    if auto is not None:
+        details["autotune"] = auto
     if not args.no_extra:
-        # the autotuner's pick on this data (every candidate must reproduce the
-        # default schedule's ids); the headline keeps the default schedule
-        t0 = time.perf_counter()
-        acfg, aleaf = ctx.autotune(dp, dq, K, budget_trials=14)
-        details["autotune"] = {"seconds": time.perf_counter() - t0, "leaf_size": aleaf, "schedule": acfg.as_dict()}
-        # end to end: host points -> build -> host queries -> ids on the host
         e2e = []
         for _ in range(3):
             t0 = time.perf_counter()
@@ -339,34 +461,46 @@ def main():
         details["end_to_end_ms"] = {"median": 1e3 * float(np.median(e2e)),
                                     "what": "H2D points + LBVH build + H2D queries + batched k-NN + D2H ids, "
                                             "wall clock"}
-        details.update(_extra_lidar(ctx, tree, pts, q, dq, cfg, args, oracle))
+        steps = max(5, args.steps // 5)
+        ms32, msp32, ib, ip, st = _knn_pair(ctx, tree, dq, 32, cfg, steps, 2)
+        ref32, _, _, _ = oracle.kdtree_knn(pts, q, 32)
+        details["knn_k32"] = {"batched_qps": nq / (ms32 * 1e-3), "plain_qps": nq / (msp32 * 1e-3),
+                              "speedup_vs_plain": msp32 / ms32, "ms_batched": ms32, "ms_plain": msp32,
+                              "fallback": st["fallback_queries"], "rounds": st["rounds_run"],
+                              "batched_eq_oracle": bool(np.array_equal(ib, ref32)),
+                              "plain_eq_oracle": bool(np.array_equal(ip, ref32))}
+        details["range_r0.5"] = _range_line(ctx, tree, pts, q, oracle)
+        details["lbvh_rebuild"] = _build_line(ctx, dp, len(pts), leaf)
     if args.all:
         details.update(_extra_all(ctx, args, oracle))
 
+    roof = _roof(alg_bytes, k_ms, kernel="knn round 0 traversal (" + ROUND0 + ")",
+                 node_visits=cnt["node_visits"], point_tests=cnt["point_tests"], rerank_reads=cnt["rerank_reads"],
+                 node_visits_per_s=cnt["node_visits"] / (k_ms * 1e-3))
+    roof["traffic"] = traffic["hbm_bytes_per_launch"] if traffic else None
+    if traffic:
+        roof["traffic_detail"] = traffic
     out = {
         "metric": METRIC,
-        "value": nq / (ms_b * 1e-3),
+        "value": nq / (head_ms * 1e-3),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_b,
+        "ms_per_step": head_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "lidar_scan: 1,310,720 pts (20x64x1024) x 400,000 queries, k=5",
-                   "leaf_size": leaf, "schedule": cfg.as_dict(), "form": "batched FP32 + FP64 re-rank"},
+                   "leaf_size": head_leaf, "schedule": head_cfg.as_dict(),
+                   "schedule_source": "autotuner" if head_cfg is not cfg else "default",
+                   "form": "batched FP32 + FP64 re-rank"},
         "plain": {"value": nq / (ms_p * 1e-3), "ms_per_step": ms_p, "ms_device": ms_p_dev,
                   "form": "one launch, one query per lane in input order, FP64, r=inf"},
-        "speedup_vs_plain": ms_p / ms_b,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "knn round 0 traversal", "kernel_ms": k_ms, "alg_bytes": int(alg_bytes),
-                     "node_visits": cnt["node_visits"], "point_tests": cnt["point_tests"],
-                     "rerank_reads": cnt["rerank_reads"],
-                     "node_visits_per_s": cnt["node_visits"] / (k_ms * 1e-3)},
+        "speedup_vs_plain": ms_p / head_ms,
+        "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
         "details": details,

@@ -67,7 +67,9 @@ extern "C" {
 /* with GPRT_PTR_DEVICE: only enqueue on the context stream and return; the
  * caller synchronises with gprt_hip_ctx_sync() (statistics are then partial) */
 #define GPRT_NO_SYNC 2
-/* gprt_hip_knn_plain: count node visits and point tests into the statistics */
+/* gprt_hip_knn_plain, gprt_hip_range, gprt_hip_bh_forces: count node visits
+ * and point tests (Barnes-Hut: also far-field interactions) into the
+ * statistics (slower: for the roofline figures) */
 #define GPRT_COLLECT_COUNTERS 4
 /* tree build: order the points by the reference's 30-bit Morton codes (10 bits
  * per axis, gprt-core/src/morton.rs:4-48) instead of the default 21 bits per
@@ -77,6 +79,9 @@ extern "C" {
  * G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over pairs with |d|^2 > 1e-10
  * (gprt-macros/src/lib.rs:785; there eps = 1e-3) instead of Plummer softening */
 #define GPRT_BH_REFERENCE_LAW 16
+/* gprt_hip_range: time the traversal kernel of the call (count walk in the
+ * first phase, fill walk in the second) alone, into ms_kernel */
+#define GPRT_TIME_KERNELS 32
 
 #define GPRT_ID_NONE 0xFFFFFFFFu
 
@@ -138,10 +143,11 @@ typedef struct {
     double ms_total;           /* device time of the last call (hipEvents) */
     double ms_query;           /* query phase only (resident -> resident) */
     double ms_build;           /* last tree build */
-    int64_t node_visits;       /* only with counters enabled */
-    int64_t point_tests;
-    int64_t rerank_reads;      /* FP64 re-rank point reads (counters enabled) */
-    double ms_kernel;          /* first traversal round alone (time_kernels) */
+    int64_t node_visits;       /* only with counters enabled (Barnes-Hut: nodes opened) */
+    int64_t point_tests;       /* (Barnes-Hut: body-body interactions) */
+    int64_t rerank_reads;      /* FP64 re-rank point reads (counters enabled; Barnes-Hut: far-field
+                                  node interactions) */
+    double ms_kernel;          /* first traversal round alone (time_kernels; range: GPRT_TIME_KERNELS) */
     int64_t wave_steps;        /* per-lane rounds: sum over wavefront tasks of their longest walk in
                                   loop iterations (counters enabled); lane steps / (64 * this) is the
                                   SIMD efficiency of the walk loop */
@@ -314,6 +320,11 @@ GPRT_HIP_API void gprt_hip_debug_alloc_stats(int64_t* calls, int64_t* live_buffe
  * again; after < 0 turns it off.  The environment variable GPRT_HIP_FAULT=alloc
  * (or alloc:N) sets it at load.  Process-wide. */
 GPRT_HIP_API void gprt_hip_debug_fail_alloc(int64_t after);
+/* A plain streaming copy kernel (stream_copy_kernel) of `bytes` (a multiple of
+ * 16) between device buffers: reads and writes exactly that many bytes, the
+ * reference point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings.  No
+ * counterpart in the reference.  Synchronous. */
+GPRT_HIP_API int gprt_hip_debug_stream_copy(gprt_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---------------- sandbox inputs ---------------- */
 #define GPRT_CSV_POINTS 0 /* "x,y,z[,...]" per line (gprt-sandbox/src/io.rs:8-30) */

@@ -165,7 +165,7 @@ __global__ void init_call_kernel(unsigned* counters, int n, int64_t* qoff, int64
 
 int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
               int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
-              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr, bool count = false);
+              DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr, bool count = false, bool time_walk = false);
 int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps, float G, float* d_acc,
            DevBuf& mass_buf, bool count, bool ref_law);
 int bh_prepare(Ctx* ctx, const float4* d_pm, int n, float* d_xyz, unsigned* flag);
@@ -186,6 +186,27 @@ extern "C" void gprt_hip_debug_alloc_stats(int64_t* calls, int64_t* live_buffers
 }
 
 extern "C" void gprt_hip_debug_fail_alloc(int64_t after) { g_alloc_budget.store(after < 0 ? -1 : after); }
+
+// a plain streaming copy, 16 bytes per lane and step: reads and writes exactly
+// `bytes`, the reference point for scaling hardware byte counters
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                          size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int gprt_hip_debug_stream_copy(gprt_hip_ctx* h, void* dst, const void* src, size_t bytes) {
+    if (!h || !dst || !src || bytes % 16) return fail(GPRT_ERR_ARG, "null pointer or bytes not a multiple of 16");
+    GPRT_CTX_LOCK(h);
+    GPRT_HIP_TRY(hipSetDevice(h->c.device));
+    const size_t n = bytes / 16;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)h->c.num_cus * 8);
+    if (n) hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, h->c.stream, (const float4*)src,
+                              (float4*)dst, n);
+    GPRT_HIP_TRY(hipGetLastError());
+    GPRT_HIP_TRY(hipStreamSynchronize(h->c.stream));
+    return GPRT_OK;
+}
 
 extern "C" const char* gprt_hip_last_error(void) { return g_err_msg.c_str(); }
 extern "C" void gprt_hip_clear_error(void) { g_err_msg.clear(); }
@@ -1278,7 +1299,8 @@ extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* qu
     if (!ids_out) {
         GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, nullptr, 0, x->r_counts, x->r_d2, x->r_big, x->r_keys,
-                           x->r_perm, x->r_qs4, x->r_sums, &total, nullptr, (flags & GPRT_COLLECT_COUNTERS) != 0));
+                           x->r_perm, x->r_qs4, x->r_sums, &total, nullptr, (flags & GPRT_COLLECT_COUNTERS) != 0,
+                           (flags & GPRT_TIME_KERNELS) != 0));
         GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
         if (!dev) GPRT_HIP_TRY(hipMemcpyAsync(offsets_out, d_off, 8ull * (nq + 1), hipMemcpyDeviceToHost, s));
     } else {
@@ -1298,7 +1320,7 @@ extern "C" int gprt_hip_range(gprt_hip_ctx* h, gprt_hip_tree* t, const float* qu
         GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_TRY(range_run(&c, T, d_q, (int)nq, radius, d_off, d_ids, total, x->r_counts, x->r_d2, x->r_big,
                            x->r_keys, x->r_perm, x->r_qs4, x->r_sums, nullptr, nullptr,
-                           (flags & GPRT_COLLECT_COUNTERS) != 0));
+                           (flags & GPRT_COLLECT_COUNTERS) != 0, (flags & GPRT_TIME_KERNELS) != 0));
         GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
         if (!dev && total) GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * total, hipMemcpyDeviceToHost, s));
     }

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
     A.x = me.x; A.y = me.y; A.z = me.z;
     A.eps2 = eps * eps;
     A.G = G;
-    uint64_t visits = 0, bodies = 0;
+    uint64_t visits = 0, bodies = 0, accepted = 0;
     auto leaf = [&](int f, int m) {
         for (int t = 0; t < m; ++t) {
             const float4 p = pts[f + t];
@@ -175,12 +175,22 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
                 const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
                 int go[2];
                 int ngo = 0;
-                if (far(nd.alo, nd.ahi, b.a)) A.add_mass(b.a.x, b.a.y, b.a.z, b.a.w);
-                else if (rA < 0) leaf(~rA, __float_as_int(nd.ahi.w));
-                else go[ngo++] = rA;
-                if (far(nd.blo, nd.bhi, b.b)) A.add_mass(b.b.x, b.b.y, b.b.z, b.b.w);
-                else if (rB < 0) leaf(~rB, __float_as_int(nd.bhi.w));
-                else go[ngo++] = rB;
+                if (far(nd.alo, nd.ahi, b.a)) {
+                    A.add_mass(b.a.x, b.a.y, b.a.z, b.a.w);
+                    if (COUNT) ++accepted;
+                } else if (rA < 0) {
+                    leaf(~rA, __float_as_int(nd.ahi.w));
+                } else {
+                    go[ngo++] = rA;
+                }
+                if (far(nd.blo, nd.bhi, b.b)) {
+                    A.add_mass(b.b.x, b.b.y, b.b.z, b.b.w);
+                    if (COUNT) ++accepted;
+                } else if (rB < 0) {
+                    leaf(~rB, __float_as_int(nd.bhi.w));
+                } else {
+                    go[ngo++] = rB;
+                }
                 if (ngo == 2) {
                     stk[sp++] = go[1];
                     node = go[0];
@@ -199,6 +209,7 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
     if (COUNT) {
         atomicAdd(&stats64[0], (unsigned long long)visits);
         atomicAdd(&stats64[1], (unsigned long long)bodies);
+        atomicAdd(&stats64[2], (unsigned long long)accepted);
     }
 }
 

@@ -506,8 +506,9 @@ static int persistent_blocks(K kernel, int threads, int num_cus, std::atomic<int
 int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_t* d_off, uint32_t* d_ids,
               int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
               DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii,
-              bool count) {
+              bool count, bool time_walk) {
     (void)d2_buf;
+    ctx->kernel_timed = false;
     hipStream_t s = ctx->stream;
     unsigned* counters = ctx->counters.as<unsigned>();
     unsigned long long* stats64 = (unsigned long long*)(counters + CNT_TOTAL);
@@ -533,6 +534,7 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     const Node* nodes = T.nodes.as<Node>();
     const float4* pts = T.pts4.as<float4>();
     const float4* qs4 = qs4_buf.as<float4>();
+    if (time_walk) GPRT_HIP_TRY(hipEventRecord(ctx->ev2, s));
     if (!d_ids) {
         if (count)
             hipLaunchKernelGGL((range_kernel<false, true>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
@@ -542,6 +544,10 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
             hipLaunchKernelGGL((range_kernel<false, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq,
                                r, d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, max_steps,
                                counters + CNT_ERROR, stats64);
+        if (time_walk) {
+            GPRT_HIP_TRY(hipEventRecord(ctx->ev3, s));
+            ctx->kernel_timed = true;
+        }
         int64_t* sums = sums_buf.as<int64_t>();
         int64_t* total = sums + tiles;
         hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(256), 0, s, counts_buf.as<int64_t>(), nq, sums);
@@ -563,6 +569,10 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     else
         hipLaunchKernelGGL((range_kernel<true, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
                            d_radii, nullptr, d_off, d_ids, capacity, max_steps, counters + CNT_ERROR, stats64);
+    if (time_walk) {
+        GPRT_HIP_TRY(hipEventRecord(ctx->ev3, s));
+        ctx->kernel_timed = true;
+    }
     // list lengths in counters the query sort does not touch (cleared above);
     // stats64[3]: the huge segments' total
     unsigned* cnt = counters + CNT_AUX;

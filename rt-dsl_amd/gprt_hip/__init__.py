@@ -22,6 +22,7 @@ LIB_PATH = os.environ.get("GPRT_HIP_LIBRARY", LIB_PATH)
 OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
 PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30, BH_REFERENCE_LAW = 0, 1, 2, 4, 8, 16
+TIME_KERNELS = 32
 ID_NONE = 0xFFFFFFFF
 K_MAX = 4096  # GPRT_HIP_K_MAX: largest k of the k-NN entry points
 RADIUS_MAX, RADIUS_PERCENTILE, RADIUS_FIXED, RADIUS_INFINITE = 0, 1, 2, 3
@@ -103,6 +104,7 @@ SIGNATURES = [
     ("gprt_hip_debug_set_query_sort_bits", None, [_int]),
     ("gprt_hip_debug_alloc_stats", None, [ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     ("gprt_hip_debug_fail_alloc", None, [_i64]),
+    ("gprt_hip_debug_stream_copy", _int, [_vp, _vp, _vp, ctypes.c_size_t]),
     ("gprt_hip_knn_noalloc", _int, [_vp, _vp, _vp, _i64, _int, ctypes.POINTER(Config), _vp, _vp]),
     ("gprt_hip_knn_noalloc_max_queries", _i64, []),
     ("gprt_hip_config_default", None, [ctypes.POINTER(Config)]),
@@ -332,7 +334,7 @@ class Context:
         return keys, vals
 
     # ---- Barnes-Hut ----
-    def barnes_hut(self, pos_mass, theta=0.5, eps=1e-3, G=1.0, leaf_size=8, out=None, law="plummer"):
+    def barnes_hut(self, pos_mass, theta=0.5, eps=1e-3, G=1.0, leaf_size=8, out=None, law="plummer", flags=0):
         """Accelerations (n, 3) of float4 bodies (x, y, z, m).  law "plummer":
         G m_j d / (|d|^2 + eps^2)^1.5; law "reference": the reference shader's
         G m_j d / (|d|^2 sqrt(|d|^2 + eps^2)) over pairs with |d|^2 > 1e-10."""
@@ -344,7 +346,7 @@ class Context:
         if out is None:
             out = DeviceArray(self, (n, 3), np.float32) if dev else np.empty((n, 3), np.float32)
         o, _ = _ptr(out)
-        flags = (PTR_DEVICE if dev else PTR_HOST) | (BH_REFERENCE_LAW if law == "reference" else 0)
+        flags |= (PTR_DEVICE if dev else PTR_HOST) | (BH_REFERENCE_LAW if law == "reference" else 0)
         _check(lib().gprt_hip_bh_forces(self.h, p, n, theta, eps, G, leaf_size, o, flags))
         return out
 
@@ -461,18 +463,25 @@ class Tree:
                                           d2.ctypes.data if with_d2 else None, PTR_HOST))
         return (ids, out_off, d2) if with_d2 else (ids, out_off)
 
-    def range_query(self, queries, radius):
-        """All ids with d64 <= r*r per query, CSR (offsets, ids), ordered by (d64, id)."""
+    def range_query(self, queries, radius, flags=0, stats=None):
+        """All ids with d64 <= r*r per query, CSR (offsets, ids), ordered by (d64, id).
+        flags (COLLECT_COUNTERS, TIME_KERNELS) apply to both calls; with a list
+        `stats`, the statistics of the counting call and of the filling call
+        are appended to it."""
         queries = _points(queries)
         q, dev = _ptr(queries)
         if dev:
             raise ValueError("range_query takes host arrays")
         nq = len(queries)
         off = np.zeros(nq + 1, np.int64)
-        _check(lib().gprt_hip_range(self.ctx.h, self.h, q, nq, radius, off.ctypes.data, None, 0, PTR_HOST))
+        _check(lib().gprt_hip_range(self.ctx.h, self.h, q, nq, radius, off.ctypes.data, None, 0, PTR_HOST | flags))
+        if stats is not None:
+            stats.append(self.ctx.stats())
         ids = np.empty(max(int(off[-1]), 1), np.uint32)
         _check(lib().gprt_hip_range(self.ctx.h, self.h, q, nq, radius, off.ctypes.data, ids.ctypes.data,
-                                    len(ids), PTR_HOST))
+                                    len(ids), PTR_HOST | flags))
+        if stats is not None:
+            stats.append(self.ctx.stats())
         return off, ids[: int(off[-1])]
 
 
