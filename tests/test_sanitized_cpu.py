@@ -13,9 +13,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SAN = os.path.join(ROOT, "rt-dsl_amd", "_build_asan")
 
 
-@pytest.mark.skipif(not (os.path.exists(os.path.join(SAN, "libgprt_hip.so")) and
-                         os.path.exists(os.path.join(SAN, "libgprt_host.so"))),
-                    reason="no sanitizer build (python rt-dsl_amd/build.py --sanitize)")
+def _sanitized_build_state():
+    libs = [os.path.join(SAN, "libgprt_hip.so"), os.path.join(SAN, "libgprt_host.so")]
+    if not all(os.path.exists(p) for p in libs):
+        return "no sanitizer build (python rt-dsl_amd/build.py --sanitize)"
+    srcs = [os.path.join(d, f) for d in (os.path.join(ROOT, "rt-dsl_amd", "csrc"), os.path.join(ROOT, "include"))
+            for f in os.listdir(d)]
+    if max(os.path.getmtime(p) for p in srcs) > min(os.path.getmtime(p) for p in libs):
+        return "the sanitizer build is older than the sources (python rt-dsl_amd/build.py --sanitize)"
+    return None
+
+
+@pytest.mark.skipif(_sanitized_build_state() is not None, reason=str(_sanitized_build_state()))
 def test_cpu_suite_under_asan_ubsan():
     sys.path.insert(0, os.path.join(ROOT, "rt-dsl_amd"))
     import build
