@@ -808,6 +808,7 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_r
         c.stats.point_tests = (int64_t)v64[1];
         if (h[CNT_TOTAL - 1]) return fail(GPRT_ERR_INPUT, "queries contain NaN/Inf or |coordinate| > 1e18");
         if (h[CNT_ERROR]) return fail(GPRT_ERR_STATE, "traversal step guard tripped: corrupt tree");
+        if (h[CNT_SORTERR]) return fail(GPRT_ERR_STATE, "query sort: radix sort look-back did not complete");
         if (h[CNT_BADID])
             return fail(GPRT_ERR_STATE, "k-NN id guard tripped: " + std::to_string(h[CNT_BADID]) +
                                             " queries re-ranked a candidate outside their cloud");
@@ -1223,8 +1224,8 @@ extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* va
                                    int end_bit, int flags) {
     if (!h || (n > 0 && (!keys || !vals))) return fail(GPRT_ERR_ARG, "null argument");
     GPRT_CTX_LOCK(h);
-    if (n < 0 || n > 0x7FFFFFF0 || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
-        return fail(GPRT_ERR_ARG, "bad size/bit range");
+    if (n < 0 || n >= (1 << 30) || begin_bit < 0 || end_bit > 64 || begin_bit > end_bit)
+        return fail(GPRT_ERR_ARG, "bad size/bit range (n < 2^30)");
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));
     hipStream_t s = c.stream;
@@ -1240,13 +1241,19 @@ extern "C" int gprt_hip_sort_pairs(gprt_hip_ctx* h, uint64_t* keys, uint32_t* va
         dk = kb.as<uint64_t>();
         dv = vb.as<uint32_t>();
     }
+    unsigned* sort_err = c.counters.as<unsigned>() + CNT_SORTERR;
+    GPRT_TRY(launch_fill_u32(s, sort_err, 1, 0u));
     int rc = radix_sort_pairs_u64(&c, dk, dv, (int)n, begin_bit, end_bit);
     if (rc == GPRT_OK && !dev && n > 0) {
         if (hipMemcpyAsync(keys, dk, 8ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipMemcpyAsync(vals, dv, 4ull * n, hipMemcpyDeviceToHost, s) != hipSuccess)
             rc = fail(GPRT_ERR_HIP, "copy back failed");
     }
+    unsigned serr = 0;
+    if (rc == GPRT_OK && hipMemcpyAsync(&serr, sort_err, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = fail(GPRT_ERR_HIP, "copy back failed");
     if (hipStreamSynchronize(s) != hipSuccess && rc == GPRT_OK) rc = fail(GPRT_ERR_HIP, "sync failed");
+    if (rc == GPRT_OK && serr) rc = fail(GPRT_ERR_STATE, "radix sort look-back did not complete");
     kb.release();
     vb.release();
     return rc;

@@ -523,6 +523,7 @@ __global__ void query_gather_kernel(const float* __restrict__ qs, const uint32_t
 // the same key as query_key32_kernel for one cloud, generated inside the
 // first radix pass's histogram kernel
 struct QueryKeyGen {
+    static constexpr bool kGenerates = true;
     const float* qs;
     const float* bbox;
     int use_morton;
@@ -593,20 +594,19 @@ int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int ncloud
     // one cloud (so one k class, whose bits are not sorted): the keys come out
     // of the first pass's histogram kernel, without the class bits
     const bool fused_keys = nclouds == 1 && !sort_classes && nq > 1 && begin < end;
+    int passes = 0;
+    unsigned* err = counters + CNT_SORTERR;
     if (fused_keys) {
-        GPRT_TRY(ctx->sort_tmp[2].reserve(radix_sort_hist_bytes(nq)));
         const QueryKeyGen gen{qs, bbox, use_morton, 0u, 30 - mbits, flag};
-        hipLaunchKernelGGL(rs_keygen_hist_kernel<QueryKeyGen>, dim3((nq + RS_TILE - 1) / RS_TILE), dim3(RS_THREADS),
-                           0, s, gen, nq, begin, (1u << std::min(8, end - begin)) - 1u, k32, perm,
-                           ctx->sort_tmp[2].as<uint32_t>());
+        GPRT_TRY((radix_sort_run<uint32_t, QueryKeyGen, QueryGatherSink>(ctx, k32, perm, nq, begin, end, gen, &fin,
+                                                                         nullptr, nullptr, &passes, err)));
     } else {
         const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
         hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb,
                            kc, k_uniform, k32, perm, flag);
+        GPRT_TRY((radix_sort_run<uint32_t, KeyLoad<uint32_t>, QueryGatherSink>(
+            ctx, k32, perm, nq, begin, end, KeyLoad<uint32_t>{k32}, &fin, nullptr, nullptr, &passes, err)));
     }
-    int passes = 0;
-    GPRT_TRY((radix_sort_run<uint32_t, QueryGatherSink>(ctx, k32, perm, nq, begin, end, fused_keys, &fin, nullptr,
-                                                        nullptr, &passes)));
     if (passes == 0)  // nothing to sort: input order
         hipLaunchKernelGGL(query_gather_kernel, dim3(std::max(1, (nq + 255) / 256)), dim3(256), 0, s, qs, perm, k32,
                            cb, nq, qs4, qcloud, counters);

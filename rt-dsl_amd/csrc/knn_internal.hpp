@@ -8,6 +8,7 @@ namespace gprt {
 constexpr int MAX_ROUNDS = 16;
 constexpr int CNT_ACTIVE = 0;     // [r]: queries active in round r (r = 0 .. MAX_ROUNDS)
 constexpr int CNT_WORK = 20;      // [r]: work-queue head of round r
+constexpr int CNT_SORTERR = 39;   // radix-sort look-back walks that gave up (must stay 0)
 constexpr int CNT_FALLBACK = 40;  // queries sent to the exact kernel
 constexpr int CNT_PARTIAL = 41;   // queries of clouds with fewer than k points
 constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corrupt tree)

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "gprt_internal.hpp"
+#include "knn_internal.hpp"
 #include "radix.hpp"
 #include "refit.hpp"
 
@@ -170,16 +171,19 @@ __global__ void morton_codes_kernel(const float* __restrict__ pts, int n, const 
     }
 }
 
-size_t radix_sort_hist_bytes(int n) { return sizeof(uint32_t) * (size_t)RS_BINS * ((n + RS_TILE - 1) / RS_TILE) + 1024; }
+// scratch of a sort of n 32-bit keys (at most four passes)
+size_t radix_sort_hist_bytes(int n) { return sizeof(uint32_t) * os_scratch_words(n, 4) + 1024; }
 
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit) {
-    return radix_sort_run<uint64_t, KVSink<uint64_t>>(ctx, keys, vals, n, begin_bit, end_bit, false, nullptr,
-                                                      nullptr, nullptr, nullptr);
+    return radix_sort_run<uint64_t, KeyLoad<uint64_t>, KVSink<uint64_t>>(
+        ctx, keys, vals, n, begin_bit, end_bit, KeyLoad<uint64_t>{keys}, nullptr, nullptr, nullptr, nullptr,
+        ctx->counters.as<unsigned>() + CNT_SORTERR);
 }
 int radix_sort_pairs_u32(Ctx* ctx, uint32_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit,
                          uint32_t** res_keys, uint32_t** res_vals) {
-    return radix_sort_run<uint32_t, KVSink<uint32_t>>(ctx, keys, vals, n, begin_bit, end_bit, false, nullptr,
-                                                      res_keys, res_vals, nullptr);
+    return radix_sort_run<uint32_t, KeyLoad<uint32_t>, KVSink<uint32_t>>(
+        ctx, keys, vals, n, begin_bit, end_bit, KeyLoad<uint32_t>{keys}, nullptr, res_keys, res_vals, nullptr,
+        ctx->counters.as<unsigned>() + CNT_SORTERR);
 }
 
 // ---------------------------------------------------------------------------
@@ -377,6 +381,7 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
         hipLaunchKernelGGL(morton_keys_kernel, dim3(std::max(1, std::min(cdiv(maxc, 256), 4096)), nclouds),
                            dim3(256), 0, s, d_pts, t->cloud_off.as<int64_t>(), t->key_bbox.as<float>(), b,
                            t->codes.as<uint64_t>(), t->perm.as<uint32_t>());
+        GPRT_TRY(launch_fill_u32(s, ctx->counters.as<unsigned>() + CNT_SORTERR, 1, 0u));
         GPRT_TRY(radix_sort_pairs_u64(ctx, t->codes.as<uint64_t>(), t->perm.as<uint32_t>(), n, 0, t->key_bits));
         hipLaunchKernelGGL(gather_points_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, d_pts,
                            t->perm.as<uint32_t>(), t->codes.as<uint64_t>(), 3 * b, t->cloud_off.as<int64_t>(),
@@ -400,7 +405,11 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
     }
     GPRT_HIP_TRY(hipGetLastError());
     GPRT_HIP_TRY(hipMemcpyAsync(t->h_bbox, t->cloud_bbox.ptr, sizeof(float) * 6, hipMemcpyDeviceToHost, s));
+    unsigned sort_err = 0;
+    if (n > 0) GPRT_HIP_TRY(hipMemcpyAsync(&sort_err, ctx->counters.as<unsigned>() + CNT_SORTERR, 4,
+                                           hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipStreamSynchronize(s));
+    if (sort_err) return fail(GPRT_ERR_STATE, "tree build: radix sort look-back did not complete");
     return GPRT_OK;
 }
 

@@ -2,12 +2,12 @@
 // 2048-key tiles (256 lanes x 8 keys), shared by the tree build (64-bit Morton
 // keys) and the query ordering (32-bit keys).
 //
-// Per pass: a tile histogram per workgroup, one workgroup turning all tile
-// histograms into output offsets, and a stable scatter.  Two hooks let a
-// caller fuse its own work into the first and the last pass: the first
-// pass's histogram can come from a kernel that also generates the keys
-// (rs_keygen_hist_kernel), and the last pass's scatter can write through a
-// caller's sink instead of the key / value arrays.
+// One histogram kernel for the digit totals of every pass, then one scatter
+// kernel per pass (decoupled look-back for the offsets of earlier tiles).  Two
+// hooks let a caller fuse its own work into the first and the last kernel: the
+// histogram kernel can take a key generator that also stores the keys, and the
+// last pass's scatter can write through a caller's sink instead of the key /
+// value arrays.
 #pragma once
 #include "gprt_internal.hpp"
 
@@ -18,154 +18,6 @@ constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
-
-// hist layout: tile-major [nblocks][RS_BINS] (each tile writes 1 KB
-// contiguously); rs_offsets_kernel turns it into the output offset of every
-// (tile, digit) pair.
-template <typename K>
-__global__ __launch_bounds__(RS_THREADS) void rs_hist_kernel(const K* __restrict__ keys, int n, int shift,
-                                                             uint32_t mask, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[RS_BINS];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = blockIdx.x * RS_TILE;
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-        const int idx = base + i * RS_THREADS + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & mask], 1u);
-    }
-    __syncthreads();
-    hist[blockIdx.x * RS_BINS + threadIdx.x] = h[threadIdx.x];
-}
-
-// The first pass's histogram from a key generator: keys[i] = gen(i, bad),
-// vals[i] = i, with the same tiling as rs_hist_kernel (workgroup `tile` of
-// RS_THREADS lanes, all of which call this).  gen.report(bad) is called once
-// per lane with whether any of its keys came from a bad input.
-template <class KeyGen>
-__device__ __forceinline__ void rs_keygen_hist_tile(const KeyGen& gen, int n, int shift, uint32_t mask,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                    uint32_t* __restrict__ hist, int tile) {
-    __shared__ uint32_t h[RS_BINS];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = tile * RS_TILE;
-    bool bad = false;
-#pragma unroll
-    for (int i = 0; i < RS_ITEMS; ++i) {
-        const int idx = base + i * RS_THREADS + threadIdx.x;
-        if (idx < n) {
-            const uint32_t k = gen(idx, bad);
-            keys[idx] = k;
-            vals[idx] = (uint32_t)idx;
-            atomicAdd(&h[(k >> shift) & mask], 1u);
-        }
-    }
-    gen.report(bad);
-    __syncthreads();
-    hist[tile * RS_BINS + threadIdx.x] = h[threadIdx.x];
-}
-
-template <class KeyGen>
-__global__ __launch_bounds__(RS_THREADS) void rs_keygen_hist_kernel(KeyGen gen, int n, int shift, uint32_t mask,
-                                                                    uint32_t* __restrict__ keys,
-                                                                    uint32_t* __restrict__ vals,
-                                                                    uint32_t* __restrict__ hist) {
-    rs_keygen_hist_tile(gen, n, shift, mask, keys, vals, hist, blockIdx.x);
-}
-
-// One block of 1024 threads = 4 groups x 256 digits.  Group g scans tiles
-// [g*nb/4, (g+1)*nb/4) per digit; the groups and the digit totals are then
-// combined so hist[b][d] = (keys with digit < d) + (keys with digit d in tiles < b).
-// Up to RS_REG tiles per group are held in registers, so a group waits for one
-// memory latency for its reads; beyond that the tiles are read twice, RS_UNROLL
-// independent loads per round trip.
-template <int RS_REG = 64>
-__global__ __launch_bounds__(1024) void rs_offsets_kernel(uint32_t* hist, int nblocks) {
-    __shared__ uint32_t gsum[4][RS_BINS];
-    __shared__ uint32_t dtot[RS_BINS];
-    const int d = threadIdx.x & 255, g = threadIdx.x >> 8;
-    const int per = (nblocks + 3) / 4;
-    const int b0 = min(g * per, nblocks), b1 = min(b0 + per, nblocks);
-    constexpr int RS_UNROLL = 8;
-    const bool in_regs = per <= RS_REG;
-    uint32_t r[RS_REG];
-    uint32_t run = 0;
-    int b = b0;
-    if (in_regs) {
-#pragma unroll
-        for (int u = 0; u < RS_REG; ++u) r[u] = b0 + u < b1 ? hist[(b0 + u) * RS_BINS + d] : 0u;
-#pragma unroll
-        for (int u = 0; u < RS_REG; ++u) run += r[u];
-    } else {
-        for (; b + RS_UNROLL <= b1; b += RS_UNROLL) {
-            uint32_t v[RS_UNROLL];
-#pragma unroll
-            for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
-#pragma unroll
-            for (int u = 0; u < RS_UNROLL; ++u) run += v[u];
-        }
-        for (; b < b1; ++b) run += hist[b * RS_BINS + d];
-    }
-    gsum[g][d] = run;
-    __syncthreads();
-    if (g == 0) {
-        uint32_t t = 0;
-        for (int gg = 0; gg < 4; ++gg) {
-            const uint32_t v = gsum[gg][d];
-            gsum[gg][d] = t;
-            t += v;
-        }
-        dtot[d] = t;
-    }
-    __syncthreads();
-    // exclusive scan of the 256 digit totals by the first four wavefronts:
-    // a shuffle scan within each, then the wavefront totals
-    __shared__ uint32_t wtot[4];
-    if (g == 0) {
-        const uint32_t tot = dtot[d];
-        uint32_t inc = tot;
-        const int lane = d & 63;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t x = __shfl_up(inc, o);
-            if (lane >= o) inc += x;
-        }
-        if (lane == 63) wtot[d >> 6] = inc;
-        dtot[d] = inc - tot;
-    }
-    __syncthreads();
-    if (g == 0) {
-        uint32_t before = 0;
-        for (int w = 0; w < (d >> 6); ++w) before += wtot[w];
-        dtot[d] += before;
-    }
-    __syncthreads();
-    run = dtot[d] + gsum[g][d];
-    if (in_regs) {
-#pragma unroll
-        for (int u = 0; u < RS_REG; ++u)
-            if (b0 + u < b1) {
-                hist[(b0 + u) * RS_BINS + d] = run;
-                run += r[u];
-            }
-        return;
-    }
-    for (b = b0; b + RS_UNROLL <= b1; b += RS_UNROLL) {
-        uint32_t v[RS_UNROLL];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) v[u] = hist[(b + u) * RS_BINS + d];
-#pragma unroll
-        for (int u = 0; u < RS_UNROLL; ++u) {
-            hist[(b + u) * RS_BINS + d] = run;
-            run += v[u];
-        }
-    }
-    for (; b < b1; ++b) {
-        const uint32_t v = hist[b * RS_BINS + d];
-        hist[b * RS_BINS + d] = run;
-        run += v;
-    }
-}
 
 // the default destination of a pass: the other key / value arrays
 template <typename K>
@@ -178,21 +30,126 @@ struct KVSink {
     }
 };
 
-// Stable scatter of one tile: ranks within a wavefront by matching the digit
-// with 8 ballots, per-wave running counts in LDS, then the tile's offsets.
+// ---------------------------------------------------------------------------
+// One-sweep passes (Merrill and Garland's decoupled look-back, as in
+// Adinets and Merrill's Onesweep radix sort): the digit totals of every pass
+// come from one histogram kernel over the keys, and each pass is one scatter
+// kernel whose workgroups take tiles in launch order from a counter, publish
+// their tile's digit counts at once and find the counts of all earlier tiles
+// by walking back over the published words, instead of a separate offsets
+// kernel per pass.  A status word is a 2-bit flag (aggregate / inclusive
+// prefix) and a 30-bit count, so n < 2^30.
+// ---------------------------------------------------------------------------
+constexpr int OS_MAXP = 8;
+constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_VAL = (1u << 30) - 1;
+constexpr uint32_t OS_SPIN_CAP = 1u << 22;  // look-back reads of one word before giving up (never reached)
+
+constexpr int OS_COPIES = 8;    // digit totals kept in 8 copies (tile % 8) to spread the atomics
+constexpr int OS_WINDOW = 8;    // earlier tiles' words read together per look-back step
+
+struct OsLayout {
+    uint32_t* totals;   // [OS_COPIES][passes][RS_BINS] digit totals
+    uint32_t* tilectr;  // [passes] tile counters
+    uint32_t* status;   // [passes][ntiles][RS_BINS]
+    int ntiles;
+    int passes;
+};
+// words cleared before a sort (totals and tile counters)
+inline int os_clear_words(int passes) { return OS_COPIES * passes * RS_BINS + 64; }
+// words of the one-sweep scratch for n keys and `passes` passes
+inline size_t os_scratch_words(int n, int passes) {
+    const size_t nt = (size_t)(n + RS_TILE - 1) / RS_TILE;
+    return (size_t)os_clear_words(passes) + (size_t)passes * nt * RS_BINS;
+}
+inline OsLayout os_layout(uint32_t* base, int n, int passes) {
+    OsLayout L;
+    L.totals = base;
+    L.tilectr = base + (size_t)OS_COPIES * passes * RS_BINS;
+    L.status = L.tilectr + 64;
+    L.ntiles = (n + RS_TILE - 1) / RS_TILE;
+    L.passes = passes;
+    return L;
+}
+
+// keys already in memory: the histogram kernel's key source
+template <typename K>
+struct KeyLoad {
+    const K* keys;
+    static constexpr bool kGenerates = false;
+    __device__ __forceinline__ K operator()(int i, bool&) const { return keys[i]; }
+    __device__ __forceinline__ void report(bool) const {}
+};
+
+// Digit totals of all passes (one LDS histogram per pass, then one atomic per
+// non-empty bin), and the look-back words of this tile cleared for the
+// scatters.  A generating key source also stores keys[i] and vals[i] = i.
+template <typename K, class Gen>
+__global__ __launch_bounds__(RS_THREADS) void os_hist_kernel(Gen gen, int n, int begin_bit, int end_bit,
+                                                             K* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                             OsLayout L) {
+    __shared__ uint32_t h[OS_MAXP][RS_BINS];
+    const int t = threadIdx.x;
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        h[p][t] = 0;
+        L.status[((size_t)p * L.ntiles + blockIdx.x) * RS_BINS + t] = 0;
+    }
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; ++i) {
+        const int idx = base + i * RS_THREADS + t;
+        if (idx < n) {
+            const K k = gen(idx, bad);
+            if (Gen::kGenerates) {
+                keys[idx] = k;
+                vals[idx] = (uint32_t)idx;
+            }
+            for (int p = 0; p < passes; ++p) {
+                const int shift = begin_bit + 8 * p;
+                const uint32_t mask = (1u << min(8, end_bit - shift)) - 1u;
+                atomicAdd(&h[p][(uint32_t)(k >> shift) & mask], 1u);
+            }
+        }
+    }
+    gen.report(bad);
+    __syncthreads();
+    uint32_t* tot = L.totals + (size_t)(blockIdx.x % OS_COPIES) * passes * RS_BINS;
+    for (int p = 0; p < passes; ++p)
+        if (h[p][t]) atomicAdd(&tot[p * RS_BINS + t], h[p][t]);
+}
+
+// One stable pass: the tile's ranks as in rs_scatter_kernel, the offsets of
+// earlier tiles by look-back, the digit base by a scan of the totals.
 template <typename K, class Sink>
-__global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin, int n,
-                                                                int shift, uint32_t mask,
-                                                                const uint32_t* __restrict__ hist, Sink sink) {
+__global__ __launch_bounds__(RS_THREADS) void os_scatter_kernel(const K* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin, int n, int shift,
+                                                                uint32_t mask, int pass, OsLayout L, Sink sink,
+                                                                unsigned* err) {
     __shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
     __shared__ uint32_t boff[RS_BINS];
+    __shared__ uint32_t wtot[RS_WAVES];
+    __shared__ int s_tile;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    if (t == 0) s_tile = (int)atomicAdd(&L.tilectr[pass], 1u);
     for (int i = t; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&wcnt[0][0])[i] = 0;
-    boff[t] = hist[blockIdx.x * RS_BINS + t];
+    // digit base: exclusive scan of this pass's digit totals
+    uint32_t tot = 0;
+#pragma unroll
+    for (int c = 0; c < OS_COPIES; ++c) tot += L.totals[((size_t)c * L.passes + pass) * RS_BINS + t];
+    uint32_t inc = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = __shfl_up(inc, o);
+        if (lane >= o) inc += x;
+    }
+    if (lane == 63) wtot[w] = inc;
     __syncthreads();
+    uint32_t dbase = inc - tot;
+    for (int ww = 0; ww < w; ++ww) dbase += wtot[ww];
+    const int tile = s_tile;
 
-    const int base = blockIdx.x * RS_TILE + w * (64 * RS_ITEMS);
+    const int base = tile * RS_TILE + w * (64 * RS_ITEMS);
     K k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
@@ -218,15 +175,54 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    {
-        uint32_t s = 0;
+    uint32_t local = 0;
 #pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ++ww) {
-            const uint32_t c = wcnt[ww][t];
-            wcnt[ww][t] = s;
-            s += c;
-        }
+    for (int ww = 0; ww < RS_WAVES; ++ww) {
+        const uint32_t c = wcnt[ww][t];
+        wcnt[ww][t] = local;
+        local += c;
     }
+    // publish this tile's count of digit t, then look back for the earlier tiles'
+    uint32_t* const row = L.status + (size_t)pass * L.ntiles * RS_BINS;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(&row[t], OS_INC | local, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(&row[(size_t)tile * RS_BINS + t], OS_AGG | local, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // OS_WINDOW earlier words per step, loaded together, taken in order up
+        // to the first inclusive prefix (done) or the first unpublished word
+        // (read again from there)
+        uint32_t spins = 0;
+        for (int j = tile - 1; j >= 0;) {
+            uint32_t w[OS_WINDOW];
+#pragma unroll
+            for (int u = 0; u < OS_WINDOW; ++u)
+                w[u] = u <= j ? __hip_atomic_load(&row[(size_t)(j - u) * RS_BINS + t], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : OS_INC;
+            bool done = false;
+            int used = 0;
+#pragma unroll
+            for (int u = 0; u < OS_WINDOW; ++u) {
+                if (done || used < u) continue;
+                const uint32_t f = w[u] & ~OS_VAL;
+                if (f == 0) continue;  // unpublished: stop taking words
+                excl += w[u] & OS_VAL;
+                used = u + 1;
+                done = f == OS_INC;
+            }
+            if (done) break;
+            j -= used;
+            if (used < OS_WINDOW && ++spins > OS_SPIN_CAP) {
+                atomicAdd(err, 1u);
+                break;
+            }
+        }
+        __hip_atomic_store(&row[(size_t)tile * RS_BINS + t], OS_INC | (excl + local), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    boff[t] = dbase + excl;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
@@ -238,44 +234,49 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter_kernel(const K* __restr
     }
 }
 
-// Sort (keys, vals)[0, n) by bits [begin_bit, end_bit).  first_hist_ready: the
-// first pass's tile histograms are already in the context's histogram buffer
-// (rs_keygen_hist_kernel).  fin: when given, the last pass writes through it
-// and the sorted pairs are not stored.  Otherwise, with res_keys / res_vals
+static __global__ void os_clear_kernel(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// Sort (keys, vals)[0, n) by bits [begin_bit, end_bit) with one-sweep passes.
+// gen: the keys' source for the histogram (KeyLoad, or a generator that also
+// stores keys and vals = index).  fin: when given, the last pass writes through
+// it and the sorted pairs are not stored.  Otherwise, with res_keys / res_vals
 // the result is left in whichever of (keys, vals) and the context's
 // temporaries the last pass wrote, and these receive its location; without
 // them it is copied back into keys / vals.  *passes receives the pass count.
-template <typename K, class Fin>
-int radix_sort_run(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit, bool first_hist_ready,
-                   const Fin* fin, K** res_keys, uint32_t** res_vals, int* passes_out) {
+// err counts look-back walks that gave up (a result to be discarded).
+template <typename K, class Gen, class Fin>
+int radix_sort_run(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit, const Gen& gen,
+                   const Fin* fin, K** res_keys, uint32_t** res_vals, int* passes_out, unsigned* err) {
     if (res_keys) *res_keys = keys;
     if (res_vals) *res_vals = vals;
     if (passes_out) *passes_out = 0;
     if (n <= 1 || begin_bit >= end_bit) return GPRT_OK;
+    const int passes = (end_bit - begin_bit + 7) / 8;
+    if (passes > OS_MAXP || n >= (1 << 30)) return fail(GPRT_ERR_ARG, "radix sort: too many keys or bits");
     const int nblocks = (n + RS_TILE - 1) / RS_TILE;
     GPRT_TRY(ctx->sort_tmp[0].reserve(sizeof(K) * (size_t)n));
     GPRT_TRY(ctx->sort_tmp[1].reserve(sizeof(uint32_t) * (size_t)n));
-    GPRT_TRY(ctx->sort_tmp[2].reserve(sizeof(uint32_t) * (size_t)RS_BINS * nblocks));
+    GPRT_TRY(ctx->sort_tmp[2].reserve(sizeof(uint32_t) * os_scratch_words(n, passes)));
     K* k0 = keys;
     uint32_t* v0 = vals;
     K* k1 = ctx->sort_tmp[0].as<K>();
     uint32_t* v1 = ctx->sort_tmp[1].as<uint32_t>();
-    uint32_t* hist = ctx->sort_tmp[2].as<uint32_t>();
+    const OsLayout L = os_layout(ctx->sort_tmp[2].as<uint32_t>(), n, passes);
     hipStream_t s = ctx->stream;
-    int passes = 0;
-    for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
-        const int nb = std::min(8, end_bit - shift);
-        const uint32_t mask = (1u << nb) - 1u;
-        const bool last = shift + 8 >= end_bit;
-        if (!(passes == 0 && first_hist_ready))
-            hipLaunchKernelGGL(rs_hist_kernel<K>, dim3(nblocks), dim3(RS_THREADS), 0, s, k0, n, shift, mask, hist);
-        hipLaunchKernelGGL(rs_offsets_kernel<>, dim3(1), dim3(1024), 0, s, hist, nblocks);
-        if (last && fin) {
-            hipLaunchKernelGGL((rs_scatter_kernel<K, Fin>), dim3(nblocks), dim3(RS_THREADS), 0, s, k0, v0, n, shift,
-                               mask, hist, *fin);
+    hipLaunchKernelGGL(os_clear_kernel, dim3(1), dim3(256), 0, s, L.totals, os_clear_words(passes));
+    hipLaunchKernelGGL((os_hist_kernel<K, Gen>), dim3(nblocks), dim3(RS_THREADS), 0, s, gen, n, begin_bit, end_bit,
+                       keys, vals, L);
+    for (int p = 0; p < passes; ++p) {
+        const int shift = begin_bit + 8 * p;
+        const uint32_t mask = (1u << std::min(8, end_bit - shift)) - 1u;
+        if (p == passes - 1 && fin) {
+            hipLaunchKernelGGL((os_scatter_kernel<K, Fin>), dim3(nblocks), dim3(RS_THREADS), 0, s, k0, v0, n, shift,
+                               mask, p, L, *fin, err);
         } else {
-            hipLaunchKernelGGL((rs_scatter_kernel<K, KVSink<K>>), dim3(nblocks), dim3(RS_THREADS), 0, s, k0, v0, n,
-                               shift, mask, hist, KVSink<K>{k1, v1});
+            hipLaunchKernelGGL((os_scatter_kernel<K, KVSink<K>>), dim3(nblocks), dim3(RS_THREADS), 0, s, k0, v0, n,
+                               shift, mask, p, L, KVSink<K>{k1, v1}, err);
             std::swap(k0, k1);
             std::swap(v0, v1);
         }
