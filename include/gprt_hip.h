@@ -274,7 +274,9 @@ GPRT_HIP_API int gprt_hip_range(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const fl
  * leaf_size: bodies per leaf bucket (1..32).  A bucket that is opened is
  * summed body by body, so larger buckets trade time for accuracy: on a
  * 1M-body Plummer sphere at theta 0.5, leaf 8 gives a median relative error
- * of 1.1e-3 in 26 ms, leaf 1 gives 1.3e-3 in 20 ms.
+ * of 1.1e-3 in 26 ms, leaf 1 gives 1.3e-3 in 20 ms.  The Plummer law needs
+ * eps > 0 (GPRT_ERR_ARG otherwise: every body's own term would be 0/0); the
+ * reference law takes eps = 0.
  * Replaces barnes_hut! (gprt-macros/src/lib.rs:752-834). */
 GPRT_HIP_API int gprt_hip_bh_forces(gprt_hip_ctx* ctx, const float* pos_mass, int64_t n, float theta, float eps,
                                     float G, int leaf_size, float* acc_out, int ptr_flags);

@@ -1345,6 +1345,9 @@ extern "C" int gprt_hip_bh_forces(gprt_hip_ctx* h, const float* pos_mass, int64_
     if (n > 0 && (!pos_mass || !acc_out)) return fail(GPRT_ERR_ARG, "null pos_mass/acc_out");
     if (!(theta >= 0.0f && theta <= 1e6f) || !(eps >= 0.0f && eps <= 1e18f) || !(fabsf(G) <= 1e30f))
         return fail(GPRT_ERR_ARG, "theta, eps and G must be finite (theta, eps >= 0)");
+    // Plummer softening with eps = 0 divides 0 by 0 on every body's own term
+    if (eps == 0.0f && !(flags & GPRT_BH_REFERENCE_LAW))
+        return fail(GPRT_ERR_ARG, "eps must be > 0 for the Plummer law (GPRT_BH_REFERENCE_LAW has no softening floor)");
     if (leaf_size < 1 || leaf_size > 32) return fail(GPRT_ERR_ARG, "leaf_size must be in [1, 32]");
     Ctx& c = h->c;
     GPRT_HIP_TRY(hipSetDevice(c.device));

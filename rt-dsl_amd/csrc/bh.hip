@@ -13,22 +13,16 @@
 // (gprt-macros/src/lib.rs:785, 790, 799; there eps^2 = 1e-6):
 //   a_i = sum_{j : d2 > 1e-10} G m_j (x_j - x_i) / (d2 sqrt(d2 + eps^2)),
 //   d2 = |x_j - x_i|^2, applied alike to bodies and to centres of mass.
-// Opening rule for a child of size s (largest box edge) whose centre of mass is
-// at distance d from x_i: take it whole when s < theta * d and x_i lies outside
-// the child's box; otherwise open it (leaf buckets are summed body by body).
-// Accumulation is FP32 with Neumaier compensation per component.
+// The walk itself (opening rule, force laws, compensated sums) is in
+// bh_walk.hpp, shared with the host build.
 #include <algorithm>
 
 #include "gprt_internal.hpp"
 #include "knn_internal.hpp"
+#include "bh_walk.hpp"
 #include "refit.hpp"
 
 namespace gprt {
-
-// per child: (com.x, com.y, com.z, mass)
-struct __align__(16) BHNode {
-    float4 a, b;
-};
 
 struct MassSlots {
     BHNode* bh;
@@ -48,18 +42,9 @@ struct MassP {
         return r;
     }
     __device__ static MassP combine(const MassP& a, const MassP& b) {
+        const float4 v = mass_merge(make_float4(a.cx, a.cy, a.cz, a.m), make_float4(b.cx, b.cy, b.cz, b.m));
         MassP r;
-        r.m = a.m + b.m;
-        if (r.m > 0.0f) {
-            const float wa = a.m / r.m, wb = b.m / r.m;
-            r.cx = __builtin_fmaf(wa, a.cx, wb * b.cx);
-            r.cy = __builtin_fmaf(wa, a.cy, wb * b.cy);
-            r.cz = __builtin_fmaf(wa, a.cz, wb * b.cz);
-        } else {
-            r.cx = 0.5f * (a.cx + b.cx);
-            r.cy = 0.5f * (a.cy + b.cy);
-            r.cz = 0.5f * (a.cz + b.cz);
-        }
+        r.cx = v.x; r.cy = v.y; r.cz = v.z; r.m = v.w;
         return r;
     }
     __device__ static void store(const MassSlots& s, int node, int side, const MassP& v) {
@@ -95,38 +80,15 @@ __global__ void gather_mass_kernel(const float4* __restrict__ pm, const uint32_t
     if (i < n) mass[i] = pm[perm[i]].w;
 }
 
-// Neumaier compensated sum
-struct KSum {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float x) {
-        const float t = s + x;
-        if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
-        else c += (x - t) + s;
-        s = t;
-    }
-    __device__ __forceinline__ float value() const { return s + c; }
-};
-
-template <bool REFLAW>
-struct BHAcc {
-    KSum ax, ay, az;
-    float x, y, z, eps2, G;
-    __device__ __forceinline__ void add_mass(float mx, float my, float mz, float m) {
-        const float dx = mx - x, dy = my - y, dz = mz - z;
-        float w;
-        if (REFLAW) {
-            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            if (!(d2 > 1e-10f)) return;
-            w = G * m / (d2 * sqrtf(d2 + eps2));
-        } else {
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-            const float inv = 1.0f / sqrtf(r2);
-            w = G * m * inv * inv * inv;
-        }
-        ax.add(w * dx);
-        ay.add(w * dy);
-        az.add(w * dz);
-    }
+struct BHMemDev {
+    const Node* __restrict__ nodes_;
+    const BHNode* __restrict__ bh_;
+    const float4* __restrict__ pts_;
+    const float* __restrict__ mass_;
+    __device__ __forceinline__ Node node(int i) const { return nodes_[i]; }
+    __device__ __forceinline__ BHNode bh(int i) const { return bh_[i]; }
+    __device__ __forceinline__ float4 pt(int i) const { return pts_[i]; }
+    __device__ __forceinline__ float mass(int i) const { return mass_[i]; }
 };
 
 template <bool COUNT, bool REFLAW>
@@ -143,73 +105,19 @@ __global__ __launch_bounds__(256) void bh_force_kernel(const Node* __restrict__ 
     A.x = me.x; A.y = me.y; A.z = me.z;
     A.eps2 = eps * eps;
     A.G = G;
-    uint64_t visits = 0, bodies = 0, accepted = 0;
-    auto leaf = [&](int f, int m) {
-        for (int t = 0; t < m; ++t) {
-            const float4 p = pts[f + t];
-            A.add_mass(p.x, p.y, p.z, mass[f + t]);
-        }
-        bodies += m;
-    };
-    // a child is taken whole when s < theta d, i.e. s^2 < theta^2 d^2, and the
-    // body lies outside its box
-    auto far = [&](const float4& lo, const float4& hi, const float4& c) -> bool {
-        const float sx = hi.x - lo.x, sy = hi.y - lo.y, sz = hi.z - lo.z;
-        const float s = fmaxf(sx, fmaxf(sy, sz));
-        const float dx = c.x - A.x, dy = c.y - A.y, dz = c.z - A.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        const bool outside = A.x < lo.x || A.x > hi.x || A.y < lo.y || A.y > hi.y || A.z < lo.z || A.z > hi.z;
-        return outside && s * s < theta2 * d2;
-    };
-    if (root.y > 0) {
-        if (root.x < 0) {
-            leaf(~root.x, root.y);
-        } else {
-            int stk[EXACT_STACK];
-            int sp = 0, node = root.x, steps = 0;
-            while (true) {
-                if (++steps > max_steps || sp >= EXACT_STACK) { atomicAdd(err, 1u); break; }
-                const Node nd = nodes[node];
-                const BHNode b = bh[node];
-                ++visits;
-                const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
-                int go[2];
-                int ngo = 0;
-                if (far(nd.alo, nd.ahi, b.a)) {
-                    A.add_mass(b.a.x, b.a.y, b.a.z, b.a.w);
-                    if (COUNT) ++accepted;
-                } else if (rA < 0) {
-                    leaf(~rA, __float_as_int(nd.ahi.w));
-                } else {
-                    go[ngo++] = rA;
-                }
-                if (far(nd.blo, nd.bhi, b.b)) {
-                    A.add_mass(b.b.x, b.b.y, b.b.z, b.b.w);
-                    if (COUNT) ++accepted;
-                } else if (rB < 0) {
-                    leaf(~rB, __float_as_int(nd.bhi.w));
-                } else {
-                    go[ngo++] = rB;
-                }
-                if (ngo == 2) {
-                    stk[sp++] = go[1];
-                    node = go[0];
-                    continue;
-                }
-                if (ngo == 1) { node = go[0]; continue; }
-                if (sp == 0) break;
-                node = stk[--sp];
-            }
-        }
-    }
+    const BHMemDev M{nodes, bh, pts, mass};
+    BHCounts cnt;
+    if (!bh_walk<EXACT_STACK, COUNT>(A, root, theta2, max_steps, M, cnt, [](int, int, const float4&) {},
+                                     [](int, int) {}))
+        atomicAdd(err, 1u);
     const uint32_t o = perm[i];
     acc[3 * (size_t)o] = A.ax.value();
     acc[3 * (size_t)o + 1] = A.ay.value();
     acc[3 * (size_t)o + 2] = A.az.value();
     if (COUNT) {
-        atomicAdd(&stats64[0], (unsigned long long)visits);
-        atomicAdd(&stats64[1], (unsigned long long)bodies);
-        atomicAdd(&stats64[2], (unsigned long long)accepted);
+        atomicAdd(&stats64[0], (unsigned long long)cnt.visits);
+        atomicAdd(&stats64[1], (unsigned long long)cnt.bodies);
+        atomicAdd(&stats64[2], (unsigned long long)cnt.accepted);
     }
 }
 

@@ -1,5 +1,6 @@
-// host_walk.cpp -- a CPU build of the per-lane FP32 k-NN walk (lane_walk.hpp),
-// for tests: libgprt_host.so, compiled host-only (no GPU code, no GPU needed).
+// host_walk.cpp -- a CPU build of the per-lane FP32 k-NN walk (lane_walk.hpp)
+// and of the Barnes-Hut walk (bh_walk.hpp), for tests: libgprt_host.so,
+// compiled host-only (no GPU code, no GPU needed).
 //
 // It rebuilds the LBVH exactly as lbvh.hip lays it out (same keys, stable key
 // order, Karras hierarchy with the index tie-break, leaf collapse, min/max
@@ -8,12 +9,15 @@
 // The list capacity C is a parameter, so capacities the GPU build no longer
 // uses can be exercised too.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "bh_walk.hpp"
+#include "knn_internal.hpp"
 #include "lane_walk.hpp"
 
 namespace {
@@ -30,6 +34,7 @@ struct HostTree {
     std::vector<uint32_t> perm;
     std::vector<float4> pts4;
     std::vector<Node> nodes;
+    std::vector<int2> ranges;  // sorted point range [x, y] of each internal node
     int2 root{0, 0};
 };
 
@@ -121,6 +126,7 @@ void build_tree(const float* p, int n, int leaf, HostTree& T) {
         parent[(lo == gamma ? (n - 1) : 0) + gamma] = (i << 1) | 0;
         parent[(hi == gamma + 1 ? (n - 1) : 0) + gamma + 1] = (i << 1) | 1;
     }
+    T.ranges = ranges;
     // boxes: every child box is the min/max of the points of its range
     for (int i = 0; i < n - 1; ++i) {
         Node& nd = T.nodes[i];
@@ -244,6 +250,167 @@ int run_query(const HostTree& T, const float* q, int k, float R0, float mult, in
     return bad ? 2 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Barnes-Hut: the host build of bh.hip (mass refit + bh_walk), and per body an
+// FP64 evaluation of the same accepted set with a bound on its distance to the
+// direct sum.
+//
+// The bound (Plummer law, phi(r) = 1/sqrt(|r|^2 + eps^2), body force
+// G m f(r) with f = -grad phi).  For an accepted child with FP32 centre c and
+// mass M_f, R = c - x_i, d = |R|, bodies j at c + delta_j:
+//   sum_j m_j f(R + delta_j) - M_f f(R)
+//     = (M - M_f) f(R) + Df(R) [sum_j m_j delta_j] + sum_j m_j rem_j,
+//   |rem_j| <= (1/2) |delta_j|^2 sup_t ||D^3 phi(R + t delta_j)||.
+// Along a line, 1/|(r, eps) + t (u, 0)| (a point of R^4) expands in Legendre
+// polynomials, so the n-th directional derivative of phi at r is
+// n! P_n(cos) / (|r|^2 + eps^2)^{(n+1)/2} with |P_n| <= 1; the norm of a
+// symmetric multilinear form is its largest value on equal unit vectors, so
+//   ||D^2 phi(r)|| <= 2 / (|r|^2+eps^2)^{3/2},  ||D^3 phi(r)|| <= 6 / (|r|^2+eps^2)^2,
+// and |f(R)| <= 1/(d^2+eps^2).  With Q = sum m_j |delta_j|^2, b = max |delta_j|
+// (bodies of positive mass) and |R + t delta_j| >= max(0, d - b):
+//   error <= |M - M_f| / (d^2+eps^2) + 2 |sum m_j delta_j| / (d^2+eps^2)^{3/2}
+//            + 3 Q / (max(0, d - b)^2 + eps^2)^2,
+// summed over the body's accepted children (times G).  Bodies summed one by
+// one contribute no approximation error.
+// ---------------------------------------------------------------------------
+struct BHMemHost {
+    const HostTree* T;
+    const std::vector<BHNode>* bhn;
+    const std::vector<float>* mass_;
+    Node node(int i) const {
+        if (i < 0 || i >= (int)T->nodes.size()) oob("node", i, (long long)T->nodes.size());
+        return T->nodes[i];
+    }
+    BHNode bh(int i) const {
+        if (i < 0 || i >= (int)bhn->size()) oob("bh node", i, (long long)bhn->size());
+        return (*bhn)[i];
+    }
+    float4 pt(int i) const {
+        if (i < 0 || i >= T->n) oob("body", i, T->n);
+        return T->pts4[i];
+    }
+    float mass(int i) const {
+        if (i < 0 || i >= T->n) oob("mass", i, T->n);
+        return (*mass_)[i];
+    }
+};
+
+// the GPU refit's values: each child slot holds mass_merge over the child's
+// binary Karras subtree, leaves (sorted position, mass) first
+void mass_refit(const HostTree& T, const std::vector<float>& mass, std::vector<BHNode>& bh) {
+    const int n = T.n;
+    bh.assign(std::max(n - 1, 1), BHNode{});
+    if (n < 2) return;
+    std::vector<int> order(n - 1);
+    for (int i = 0; i < n - 1; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return T.ranges[a].y - T.ranges[a].x < T.ranges[b].y - T.ranges[b].x;
+    });
+    std::vector<float4> val(n - 1);
+    auto leafv = [&](int i) {
+        const float4 p = T.pts4[i];
+        return make_float4(p.x, p.y, p.z, mass[i]);
+    };
+    for (int i : order) {
+        const int lo = T.ranges[i].x, hi = T.ranges[i].y;
+        const int gamma = lo + as_int(T.nodes[i].ahi.w) - 1;
+        const float4 a = lo == gamma ? leafv(lo) : val[gamma];
+        const float4 b = hi == gamma + 1 ? leafv(gamma + 1) : val[gamma + 1];
+        bh[i].a = a;
+        bh[i].b = b;
+        val[i] = mass_merge(a, b);
+    }
+}
+
+// FP64 moments of one child's bodies about its FP32 centre
+struct ChildMoments {
+    double m = 0, s1[3] = {0, 0, 0}, q = 0, b = 0;
+};
+
+void child_moments(const HostTree& T, const std::vector<float>& mass, const std::vector<BHNode>& bh,
+                   std::vector<ChildMoments>& out) {
+    const int n = T.n;
+    out.assign(2 * (size_t)std::max(n - 1, 0), ChildMoments{});
+    for (int i = 0; i < n - 1; ++i) {
+        const int lo = T.ranges[i].x, hi = T.ranges[i].y, cA = as_int(T.nodes[i].ahi.w);
+        for (int side = 0; side < 2; ++side) {
+            const float4 c = side ? bh[i].b : bh[i].a;
+            const int f = side ? lo + cA : lo, l = side ? hi : lo + cA - 1;
+            ChildMoments& M = out[2 * (size_t)i + side];
+            for (int t = f; t <= l; ++t) {
+                const double m = mass[t];
+                const double dx = (double)T.pts4[t].x - c.x, dy = (double)T.pts4[t].y - c.y,
+                             dz = (double)T.pts4[t].z - c.z;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                M.m += m;
+                M.s1[0] += m * dx;
+                M.s1[1] += m * dy;
+                M.s1[2] += m * dz;
+                M.q += m * r2;
+                if (m > 0) M.b = std::max(M.b, std::sqrt(r2));
+            }
+        }
+    }
+}
+
+// one FP64 force term G m d / (|d|^2 + eps^2)^{3/2} (or the reference law) at
+// offset d; adds it to acc and its magnitude to absum
+void term64(bool ref_law, double G, double e2, double m, double dx, double dy, double dz, double* acc,
+            double& absum) {
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    double w;
+    if (ref_law) {
+        if (!(d2 > 1e-10)) return;
+        w = G * m / (d2 * std::sqrt(d2 + e2));
+    } else {
+        const double r2 = d2 + e2;
+        w = G * m / (r2 * std::sqrt(r2));
+    }
+    acc[0] += w * dx;
+    acc[1] += w * dy;
+    acc[2] += w * dz;
+    absum += std::fabs(w) * std::sqrt(d2);
+}
+
+template <bool REFLAW>
+void bh_body(const HostTree& T, const std::vector<BHNode>& bh, const std::vector<float>& mass,
+             const std::vector<ChildMoments>& mom, int i, float theta2, float eps, float G, float* acc32,
+             double* acc64, double& bound, double& absum, BHCounts& cnt, bool& ok) {
+    const float4 me = T.pts4[i];
+    BHAcc<REFLAW> A;
+    A.x = me.x;
+    A.y = me.y;
+    A.z = me.z;
+    A.eps2 = eps * eps;
+    A.G = G;
+    const double e2 = (double)eps * (double)eps, G64 = G;
+    acc64[0] = acc64[1] = acc64[2] = 0.0;
+    bound = 0.0;
+    absum = 0.0;
+    const BHMemHost M{&T, &bh, &mass};
+    auto on_far = [&](int node, int side, const float4& c) {
+        const double Rx = (double)c.x - me.x, Ry = (double)c.y - me.y, Rz = (double)c.z - me.z;
+        term64(REFLAW, G64, e2, c.w, Rx, Ry, Rz, acc64, absum);
+        const ChildMoments& cm = mom[2 * (size_t)node + side];
+        const double d = std::sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
+        const double den = d * d + e2;
+        const double s1 = std::sqrt(cm.s1[0] * cm.s1[0] + cm.s1[1] * cm.s1[1] + cm.s1[2] * cm.s1[2]);
+        const double g = std::max(0.0, d - cm.b);
+        bound += std::fabs(G64) * (std::fabs(cm.m - (double)c.w) / den + 2.0 * s1 / (den * std::sqrt(den)) +
+                                   3.0 * cm.q / ((g * g + e2) * (g * g + e2)));
+    };
+    auto on_leaf = [&](int f, int m) {
+        for (int t = f; t < f + m; ++t)
+            term64(REFLAW, G64, e2, mass[t], (double)T.pts4[t].x - me.x, (double)T.pts4[t].y - me.y,
+                   (double)T.pts4[t].z - me.z, acc64, absum);
+    };
+    ok = bh_walk<EXACT_STACK, true>(A, T.root, theta2, T.n + 64, M, cnt, on_far, on_leaf);
+    acc32[0] = A.ax.value();
+    acc32[1] = A.ay.value();
+    acc32[2] = A.az.value();
+    if (REFLAW) bound = NAN;
+}
+
 thread_local std::string g_msg;
 
 }  // namespace
@@ -305,6 +472,69 @@ __attribute__((visibility("default"))) int gprt_host_lane_knn(const float* pts, 
             status_out[i] = r;
         }
         if (stats_out) std::memcpy(stats_out, st, sizeof(st));
+    } catch (const std::exception& e) {
+        g_msg = e.what();
+        return 1;
+    }
+    return 0;
+}
+
+// Barnes-Hut for the bodies `sel` (original indices; NULL = all n) as
+// gprt_hip_bh_forces computes them with this leaf size: acc32 (nsel*3, the
+// FP32 results the kernel's operations give), acc64 (nsel*3, the same
+// accepted children and bodies summed in FP64, eps^2 taken in FP64), bound
+// (nsel: the bound on |acc64 - direct sum| derived above; NaN for the
+// reference law), absum (nsel: sum of the magnitudes of the summed terms),
+// stats[3] (node visits, bodies summed, children accepted over the selected
+// bodies).  Any output but acc32 may be NULL.  Returns 0, or 1 with
+// gprt_host_last_error() naming an out-of-bounds access, a walk over its step
+// or stack limit, or a bad argument.
+__attribute__((visibility("default"))) int gprt_host_bh(const float* pm, int64_t n, int leaf_size, float theta,
+                                                        float eps, float G, int ref_law, const int64_t* sel,
+                                                        int64_t nsel, float* acc32, double* acc64, double* bound,
+                                                        double* absum, int64_t* stats) {
+    g_msg.clear();
+    try {
+        if (n < 1 || n > 0x7FFFFFF0 || leaf_size < 1 || leaf_size > 32 || nsel < 0 || !acc32 || !(theta >= 0.0f))
+            throw std::invalid_argument("bad argument");
+        if (!(eps > 0.0f) && !ref_law) throw std::invalid_argument("eps must be > 0 for the Plummer law");
+        std::vector<float> xyz(3 * (size_t)n);
+        for (int64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) xyz[3 * i + a] = pm[4 * i + a];
+        HostTree T;
+        build_tree(xyz.data(), (int)n, leaf_size, T);
+        std::vector<float> mass(n);
+        std::vector<int> inv(n);
+        for (int64_t i = 0; i < n; ++i) {
+            mass[i] = pm[4 * (size_t)T.perm[i] + 3];
+            inv[T.perm[i]] = (int)i;
+        }
+        std::vector<BHNode> bh;
+        mass_refit(T, mass, bh);
+        std::vector<ChildMoments> mom;
+        child_moments(T, mass, bh, mom);
+        const float theta2 = theta * theta;
+        BHCounts cnt;
+        const int64_t ns = sel ? nsel : n;
+        for (int64_t s = 0; s < ns; ++s) {
+            const int64_t o = sel ? sel[s] : s;
+            if (o < 0 || o >= n) throw std::out_of_range("selected body outside [0, n)");
+            double a64[3], bd, ab;
+            bool ok;
+            if (ref_law)
+                bh_body<true>(T, bh, mass, mom, inv[o], theta2, eps, G, acc32 + 3 * s, a64, bd, ab, cnt, ok);
+            else
+                bh_body<false>(T, bh, mass, mom, inv[o], theta2, eps, G, acc32 + 3 * s, a64, bd, ab, cnt, ok);
+            if (!ok) throw std::runtime_error("walk over its step or stack limit");
+            if (acc64) std::memcpy(acc64 + 3 * s, a64, sizeof(a64));
+            if (bound) bound[s] = bd;
+            if (absum) absum[s] = ab;
+        }
+        if (stats) {
+            stats[0] = (int64_t)cnt.visits;
+            stats[1] = (int64_t)cnt.bodies;
+            stats[2] = (int64_t)cnt.accepted;
+        }
     } catch (const std::exception& e) {
         g_msg = e.what();
         return 1;

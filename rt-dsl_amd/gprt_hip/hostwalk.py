@@ -1,6 +1,7 @@
 """ctypes front end of libgprt_host.so: the per-lane FP32 k-NN walk, its
-candidate list and FP64 re-rank (lane_walk.hpp) and the LBVH layout of
-lbvh.hip, compiled for the host with every memory access bounds-checked.
+candidate list and FP64 re-rank (lane_walk.hpp), the Barnes-Hut walk and mass
+refit (bh_walk.hpp, bh.hip) and the LBVH layout of lbvh.hip, compiled for the
+host with every memory access bounds-checked.
 
 For tests of the device code's logic on the CPU; nothing in the GPU path uses
 it.  GPRT_HOST_LIBRARY selects another build (e.g. the ASan + UBSan one).
@@ -18,6 +19,7 @@ _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 
 CAPACITIES = (8, 16, 24, 32, 40, 48, 64, 67)
 
@@ -33,6 +35,9 @@ def lib():
         L.gprt_host_lane_knn.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, _f32p, ctypes.c_int64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                          _u32p, _i32p, _i64p]
+        L.gprt_host_bh.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, _f32p, _f64p,
+                                   _f64p, _f64p, _i64p]
         _lib = L
     return _lib
 
@@ -65,3 +70,25 @@ def lane_knn(pts, queries, k, cap, leaf_size=8, radius0=1.0, mult=3.0, rounds=3,
     if rc:
         raise RuntimeError(lib().gprt_host_last_error().decode())
     return ids, st, stats
+
+
+def barnes_hut(pos_mass, theta, eps=1e-3, G=1.0, leaf_size=8, law="plummer", sample=None):
+    """Barnes-Hut of the bodies `sample` (original indices, default all) as the
+    GPU kernel computes it.  Returns a dict: acc32 (ns, 3) float32, the kernel's
+    FP32 operations; acc64 (ns, 3), the same accepted children and bodies
+    summed in FP64; bound (ns,), a derived bound on |acc64 - direct sum| per
+    body (NaN for the reference law); absum (ns,), the sum of the magnitudes
+    of the summed terms; stats (visits, bodies, accepted)."""
+    pm = np.ascontiguousarray(pos_mass, np.float32)
+    sel = None if sample is None else np.ascontiguousarray(sample, np.int64)
+    ns = len(pm) if sel is None else len(sel)
+    acc32 = np.zeros((ns, 3), np.float32)
+    acc64 = np.zeros((ns, 3), np.float64)
+    bound = np.zeros(ns, np.float64)
+    absum = np.zeros(ns, np.float64)
+    stats = np.zeros(3, np.int64)
+    rc = lib().gprt_host_bh(pm, len(pm), leaf_size, theta, eps, G, 1 if law == "reference" else 0,
+                            None if sel is None else sel.ctypes.data, ns, acc32, acc64, bound, absum, stats)
+    if rc:
+        raise RuntimeError(lib().gprt_host_last_error().decode())
+    return {"acc32": acc32, "acc64": acc64, "bound": bound, "absum": absum, "stats": stats}

@@ -11,6 +11,8 @@ import pytest
 import gprt_hip
 import oracle
 from gprt_hip import datasets as D
+from gprt_hip import hostwalk as H
+from test_host_bh import EPS, F64_SLACK, ROUND_U, U, bound_violations, rounding_violations
 
 pytestmark = pytest.mark.gpu
 
@@ -286,43 +288,58 @@ def _rel_err(a, b):
     return np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-300)
 
 
+def _bh_same_as_host(acc, h):
+    return not (acc.view(np.uint32) != h["acc32"].view(np.uint32)).any()
+
+
 def test_bh_theta_zero_is_direct_sum(ctx):
+    """theta = 0 opens every node: the FP64 direct sum to the rounding
+    allowance of FP32 terms and compensated sums (test_host_bh.py)."""
     pm = D.plummer(20_000, seed=7)
-    acc = ctx.barnes_hut(pm, theta=0.0, eps=1e-3, G=1.0)
+    acc = ctx.barnes_hut(pm, theta=0.0, eps=EPS, G=1.0)
     sample = np.arange(0, 20_000, 97)
-    ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3)
-    err = _rel_err(acc[sample].astype(np.float64), ref)
-    # FP32 inputs and compensated FP32 sums: relative error of a few ulps
-    assert np.max(err) < 2e-5, np.max(err)
+    h = H.barnes_hut(pm, 0.0, eps=EPS, sample=sample)
+    assert _bh_same_as_host(acc[sample], h)
+    assert len(bound_violations(acc[sample], h, oracle.bh_direct(pm, sample, G=1.0, eps=EPS))) == 0
 
 
-@pytest.mark.parametrize("theta,tol_med,tol_p99", [(0.3, 2e-3, 2e-2), (0.5, 5e-3, 5e-2), (0.7, 1e-2, 1e-1)])
-def test_bh_error_vs_direct(ctx, theta, tol_med, tol_p99):
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.7])
+def test_bh_error_vs_direct(ctx, theta):
+    """100,000 Plummer bodies: the host build bit for bit and the derived
+    monopole bound against the FP64 direct sum; errors printed."""
     pm = D.plummer(100_000, seed=8)
-    acc = ctx.barnes_hut(pm, theta=theta, eps=1e-3, G=1.0)
-    sample = np.random.default_rng(1).choice(100_000, 1000, replace=False)
-    err = _rel_err(acc[sample].astype(np.float64), oracle.bh_direct(pm, sample, G=1.0, eps=1e-3))
+    acc = ctx.barnes_hut(pm, theta=theta, eps=EPS, G=1.0)
+    sample = np.sort(np.random.default_rng(1).choice(100_000, 1000, replace=False))
+    ref = oracle.bh_direct(pm, sample, G=1.0, eps=EPS)
+    h = H.barnes_hut(pm, theta, eps=EPS, sample=sample)
+    assert _bh_same_as_host(acc[sample], h)
+    assert len(bound_violations(acc[sample], h, ref)) == 0
+    err = _rel_err(acc[sample].astype(np.float64), ref)
     print(f"theta {theta}: median {np.median(err):.2e} p99 {np.quantile(err, 0.99):.2e} max {err.max():.2e}")
-    assert np.median(err) < tol_med and np.quantile(err, 0.99) < tol_p99
 
 
 def test_bh_reference_law(ctx):
     """GPRT_BH_REFERENCE_LAW: the reference shader's law.  At theta = 0 it is
     the FP64 direct sum of that law (close pairs and a coincident pair
-    included), and it differs from the Plummer result where pairs are close."""
+    included), and it differs from the Plummer result where pairs are close;
+    at theta = 0.5 it equals the host build bit for bit."""
     pm = D.plummer(20_000, seed=9)
     pm[1, :3] = pm[0, :3] + np.float32([1e-3, 0, 0])
     pm[2, :3] = pm[0, :3] + np.float32([0, 2e-4, 0])
     pm[4, :3] = pm[3, :3]
     sample = np.concatenate([np.arange(5), np.arange(5, 20_000, 97)])
-    acc = ctx.barnes_hut(pm, theta=0.0, eps=1e-3, G=1.0, law="reference")
-    ref = oracle.bh_direct(pm, sample, G=1.0, eps=1e-3, law="reference")
-    assert np.max(_rel_err(acc[sample].astype(np.float64), ref)) < 2e-5
-    pl = oracle.bh_direct(pm, np.arange(3), G=1.0, eps=1e-3)
+    acc = ctx.barnes_hut(pm, theta=0.0, eps=EPS, G=1.0, law="reference")
+    ref = oracle.bh_direct(pm, sample, G=1.0, eps=EPS, law="reference")
+    h = H.barnes_hut(pm, 0.0, eps=EPS, law="reference", sample=sample)
+    assert _bh_same_as_host(acc[sample], h)
+    err = np.abs(acc[sample].astype(np.float64) - ref).max(axis=1)
+    assert np.all(err <= (ROUND_U * U + F64_SLACK) * h["absum"])
+    pl = oracle.bh_direct(pm, np.arange(3), G=1.0, eps=EPS)
     assert np.min(_rel_err(acc[:3].astype(np.float64), pl)) > 0.05
-    acc5 = ctx.barnes_hut(pm, theta=0.5, eps=1e-3, G=1.0, law="reference")
-    err = _rel_err(acc5[sample].astype(np.float64), ref)
-    assert np.median(err) < 5e-3 and np.quantile(err, 0.99) < 5e-2
+    acc5 = ctx.barnes_hut(pm, theta=0.5, eps=EPS, G=1.0, law="reference")
+    h5 = H.barnes_hut(pm, 0.5, eps=EPS, law="reference", sample=sample)
+    assert _bh_same_as_host(acc5[sample], h5)
+    assert len(rounding_violations(acc5[sample], h5)) == 0
     with pytest.raises(ValueError):
         ctx.barnes_hut(pm, law="newton")
 
