@@ -155,7 +155,11 @@ __global__ void class_init_kernel(unsigned* counters, unsigned size) {
 }
 
 // clears the call's counters and, for one cloud, writes its query offsets
-__global__ void init_call_kernel(unsigned* counters, int n, int64_t* qoff, int64_t nq) {
+// zeroes the call's counters (and the query sort's scratch prefix, z2/n2) and
+// writes the one-cloud query offsets
+__global__ void init_call_kernel(unsigned* counters, int n, int64_t* qoff, int64_t nq, unsigned* z2, int n2) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) z2[i] = 0u;
+    if (blockIdx.x != 0) return;
     for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0u;
     if (qoff && threadIdx.x == 0) {
         qoff[0] = 0;
@@ -602,14 +606,18 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     float* radius = kth + 100 * p.nclouds;
     int nq = (int)p.nq;
     CtxExtra* x = extra(&c);
-    hipLaunchKernelGGL(init_call_kernel, dim3(1), dim3(64), 0, s, counters, CNT_ALL,
-                       p.nclouds == 1 ? (int64_t*)p.d_qoff : nullptr, p.nq);
+    int nclasses = 0;
+    for (int cl = 0; cl < N_KCLASS; ++cl) nclasses += p.class_size[cl] > 0;
+    unsigned* sort_prefix = nullptr;
+    int sort_prefix_words = 0;
+    GPRT_TRY(query_prep_clear_range(&c, p.nclouds, cfg.sort_queries, nq, nclasses > 1, &sort_prefix,
+                                    &sort_prefix_words));
+    hipLaunchKernelGGL(init_call_kernel, dim3(1 + sort_prefix_words / 4096), dim3(256), 0, s, counters, CNT_ALL,
+                       p.nclouds == 1 ? (int64_t*)p.d_qoff : nullptr, p.nq, sort_prefix, sort_prefix_words);
     uint64_t* keys = c.ws[WS_QKEYS].as<uint64_t>();
     uint32_t* perm = c.ws[WS_QPERM].as<uint32_t>();
     float4* qs4 = c.ws[WS_QS4].as<float4>();
     int* qcloud = p.nclouds > 1 ? x->qcloud.as<int>() : nullptr;
-    int nclasses = 0;
-    for (int cl = 0; cl < N_KCLASS; ++cl) nclasses += p.class_size[cl] > 0;
     // the radius sampler runs on the side stream, overlapping the query sort;
     // the rounds wait for both.  The longer branch is enqueued first (a
     // captured graph launches its branches in that order): the sort for one
@@ -618,7 +626,7 @@ static int knn_enqueue(Ctx& c, const KnnPlan& p) {
     auto sort = [&]() {
         return launch_query_prep(&c, p.d_q, p.d_qoff, p.nclouds, T.key_bbox.as<float>(), cfg.sort_queries, nq, keys,
                                  perm, qs4, counters, p.max_q_per_cloud, qcloud, perm, counters + CNT_TOTAL - 1,
-                                 p.d_kc, p.k, nclasses > 1);
+                                 p.d_kc, p.k, nclasses > 1, sort_prefix_words > 0);
     };
     if (p.nclouds == 1) GPRT_TRY(sort());
     GPRT_HIP_TRY(hipStreamWaitEvent(c.side, c.ev_fork, 0));
@@ -721,7 +729,10 @@ static uint64_t fnv(uint64_t h, const void* p, size_t n) {
     return h;
 }
 
-static int knn_run(Ctx& c, const KnnPlan& p) {
+// timed: bracket the call with the ev0/ev1 timing events (read back by
+// finish_stats); calls that only enqueue (GPRT_NO_SYNC) leave them out, as an
+// event record between two calls keeps the GPU idle while it is processed
+static int knn_run(Ctx& c, const KnnPlan& p, bool timed) {
     GPRT_TRY(knn_reserve(c, p));
     hipStream_t s = c.stream;
     if (p.cfg.use_graph) {
@@ -755,13 +766,13 @@ static int knn_run(Ctx& c, const KnnPlan& p) {
             GPRT_HIP_TRY(hipGraphInstantiate(&c.graph_exec, g, nullptr, nullptr, 0));
             c.graph_key = key;
         }
-        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_HIP_TRY(hipGraphLaunch(c.graph_exec, s));
-        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+        if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
     } else {
-        GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+        if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
         GPRT_TRY(knn_enqueue(c, p));
-        GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+        if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
     }
     return GPRT_OK;
 }
@@ -868,7 +879,7 @@ extern "C" int gprt_hip_knn(gprt_hip_ctx* h, gprt_hip_tree* t, const float* quer
         int64_t qo[2] = {0, nq};
         plan_classes(p, nullptr, qo, 1, k);
     }
-    GPRT_TRY(knn_run(c, p));
+    GPRT_TRY(knn_run(c, p, !(flags & GPRT_NO_SYNC)));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
         if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nq * k, hipMemcpyDeviceToHost, s));
@@ -969,7 +980,7 @@ extern "C" int gprt_hip_knn_noalloc(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
         int64_t qo[2] = {0, nq};
         plan_classes(p, nullptr, qo, 1, k);
     }
-    int rc = knn_run(c, p);
+    int rc = knn_run(c, p, true);
     if (rc == GPRT_OK) rc = finish_stats(c, nq, true, c.ws[WS_MISC].as<float>() + 100);
     set_no_alloc(false);
     for (int i = 0; i < 11; ++i) *slots[i] = saved[i];
@@ -1021,7 +1032,8 @@ extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float
     bool count = (flags & GPRT_COLLECT_COUNTERS) != 0;
     GPRT_TRY(launch_fill_u32(s, counters, CNT_TOTAL + 2 * CNT_STATS64, 0u));
     launch_validate(s, d_q, 3 * nq, counters + CNT_TOTAL - 1);
-    GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
+    const bool timed = !(flags & GPRT_NO_SYNC);
+    if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev0, s));
     if (k <= 64) {
         GPRT_TRY(launch_exact(s, ea, k, (int)nq, count));
     } else {
@@ -1043,7 +1055,7 @@ extern "C" int gprt_hip_knn_plain(gprt_hip_ctx* h, gprt_hip_tree* t, const float
         ra.nq_plain = nq;
         GPRT_TRY(launch_qwave(s, ra, k, 0, count));
     }
-    GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
+    if (timed) GPRT_HIP_TRY(hipEventRecord(c.ev1, s));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nq * k, hipMemcpyDeviceToHost, s));
         if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nq * k, hipMemcpyDeviceToHost, s));
@@ -1117,7 +1129,7 @@ extern "C" int gprt_hip_knn_batched(gprt_hip_ctx* h, gprt_hip_tree* t, const flo
     p.d_ids = d_ids;
     p.d_d2 = d_d2;
     plan_classes(p, k_per_cloud, query_offsets, nc, 0);
-    GPRT_TRY(knn_run(c, p));
+    GPRT_TRY(knn_run(c, p, !(flags & GPRT_NO_SYNC)));
     if (!dev) {
         GPRT_HIP_TRY(hipMemcpyAsync(ids_out, d_ids, 4ull * nout, hipMemcpyDeviceToHost, s));
         if (d2_out) GPRT_HIP_TRY(hipMemcpyAsync(d2_out, d_d2, 8ull * nout, hipMemcpyDeviceToHost, s));

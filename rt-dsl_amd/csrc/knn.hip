@@ -576,36 +576,60 @@ size_t query_prep_scratch_bytes(int nclouds, int use_morton) {
     return 0;
 }
 
+// the key bits the query order sorts, [begin, end), and whether the first
+// pass's histogram kernel generates the keys
+struct QueryOrderBits {
+    int cb, mbits, qbits, begin, end;
+    bool fused_keys;
+};
+
+static QueryOrderBits query_order_bits(int nclouds, int use_morton, int nq, int sort_classes) {
+    QueryOrderBits b;
+    b.cb = cloud_bits_of(nclouds);
+    b.mbits = 29 - b.cb;
+    b.qbits = use_morton ? std::min(query_sort_bits(), b.mbits) : 0;
+    b.begin = b.mbits - b.qbits;
+    // the class bits (29..31) only need sorting when several k classes are present
+    b.end = sort_classes ? 32 : 29;
+    // one cloud (so one k class, whose bits are not sorted): the keys come out
+    // of the first pass's histogram kernel, without the class bits
+    b.fused_keys = nclouds == 1 && !sort_classes && nq > 1 && b.begin < b.end;
+    return b;
+}
+
+int query_prep_clear_range(Ctx* ctx, int nclouds, int use_morton, int nq, int sort_classes, unsigned** first,
+                           int* words) {
+    *first = nullptr;
+    *words = 0;
+    const QueryOrderBits b = query_order_bits(nclouds, use_morton, nq, sort_classes);
+    if (b.cb > 20 || nq <= 1 || b.begin >= b.end) return GPRT_OK;
+    return radix_sort_prefix(ctx, nq, radix_passes(b.begin, b.end), first, words);
+}
+
 int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
                       int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc,
-                      int k_uniform, int sort_classes) {
+                      int k_uniform, int sort_classes, bool prefix_cleared) {
     (void)scratch;
     hipStream_t s = ctx->stream;
-    const int cb = cloud_bits_of(nclouds);
-    if (cb > 20) return fail(GPRT_ERR_ARG, "too many clouds (max 2^20)");
-    const int mbits = 29 - cb;
-    const int qbits = use_morton ? std::min(query_sort_bits(), mbits) : 0;
-    const int begin = mbits - qbits;
+    const QueryOrderBits b = query_order_bits(nclouds, use_morton, nq, sort_classes);
+    if (b.cb > 20) return fail(GPRT_ERR_ARG, "too many clouds (max 2^20)");
+    const int cb = b.cb, mbits = b.mbits;
     uint32_t* k32 = (uint32_t*)keys;
-    // the class bits (29..31) only need sorting when several k classes are present
-    const int end = sort_classes ? 32 : 29;
     const QueryGatherSink fin{qs, qs4, qcloud, mbits, (1u << cb) - 1u, counters, nq};
-    // one cloud (so one k class, whose bits are not sorted): the keys come out
-    // of the first pass's histogram kernel, without the class bits
-    const bool fused_keys = nclouds == 1 && !sort_classes && nq > 1 && begin < end;
     int passes = 0;
     unsigned* err = counters + CNT_SORTERR;
-    if (fused_keys) {
+    if (b.fused_keys) {
         const QueryKeyGen gen{qs, bbox, use_morton, 0u, 30 - mbits, flag};
-        GPRT_TRY((radix_sort_run<uint32_t, QueryKeyGen, QueryGatherSink>(ctx, k32, perm, nq, begin, end, gen, &fin,
-                                                                         nullptr, nullptr, &passes, err)));
+        GPRT_TRY((radix_sort_run<uint32_t, QueryKeyGen, QueryGatherSink>(
+            ctx, k32, perm, nq, b.begin, b.end, gen, &fin, nullptr, nullptr, &passes, err, prefix_cleared)));
     } else {
         const int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((max_per_cloud + 255) / 256, 4096));
         hipLaunchKernelGGL(query_key32_kernel, dim3(chunks, nclouds), dim3(256), 0, s, qs, qoff, bbox, use_morton, cb,
                            kc, k_uniform, k32, perm, flag);
         GPRT_TRY((radix_sort_run<uint32_t, KeyLoad<uint32_t>, QueryGatherSink>(
-            ctx, k32, perm, nq, begin, end, KeyLoad<uint32_t>{k32}, &fin, nullptr, nullptr, &passes, err)));
+            ctx, k32, perm, nq, b.begin, b.end, KeyLoad<uint32_t>{k32}, &fin, nullptr, nullptr, &passes, err,
+            prefix_cleared)));
     }
     if (passes == 0)  // nothing to sort: input order
         hipLaunchKernelGGL(query_gather_kernel, dim3(std::max(1, (nq + 255) / 256)), dim3(256), 0, s, qs, perm, k32,

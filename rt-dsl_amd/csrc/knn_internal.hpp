@@ -85,11 +85,16 @@ int launch_sampler(hipStream_t s, int sampler, const float4* orig4, const int64_
                    const int64_t* qoff, int nclouds, int k, int heuristic, float param, float* kth, float* radius);
 // queries -> Morton-bucket order (float4 with the input index in .w); keys is
 // 8 B of scratch per query, scratch holds query_prep_scratch_bytes(); flag
-// receives the input-validation result
+// receives the input-validation result.  prefix_cleared: the words named by
+// query_prep_clear_range were zeroed by the caller, ahead on the same stream.
 int launch_query_prep(Ctx* ctx, const float* qs, const int64_t* qoff, int nclouds, const float* bbox,
                       int use_morton, int nq, uint64_t* keys, uint32_t* perm, float4* qs4, unsigned* counters,
                       int64_t max_per_cloud, int* qcloud, unsigned* scratch, unsigned* flag, const int* kc = nullptr,
-                      int k_uniform = 1, int sort_classes = 0);
+                      int k_uniform = 1, int sort_classes = 0, bool prefix_cleared = false);
+// the sort scratch words launch_query_prep would zero first (none when the
+// order needs no sort); reserves that scratch
+int query_prep_clear_range(Ctx* ctx, int nclouds, int use_morton, int nq, int sort_classes, unsigned** first,
+                           int* words);
 // k classes: queries of one class share a per-lane candidate list of
 // capacity C with G = 3 guard slots, the smallest C of {8, 16, 24, 32, 40, 48}
 // with k + G <= C (registers hold the list, so a tighter class runs at a

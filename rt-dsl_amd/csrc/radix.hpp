@@ -242,18 +242,37 @@ static __global__ void os_clear_kernel(uint32_t* p, int n) {
 // gen: the keys' source for the histogram (KeyLoad, or a generator that also
 // stores keys and vals = index).  fin: when given, the last pass writes through
 // it and the sorted pairs are not stored.  Otherwise, with res_keys / res_vals
+// Pass count of a sort of key bits [begin_bit, end_bit).
+inline int radix_passes(int begin_bit, int end_bit) {
+    return begin_bit >= end_bit ? 0 : (end_bit - begin_bit + 7) / 8;
+}
+
+// The words a sort of n keys over `passes` passes zeroes before its histogram
+// kernel (digit totals and tile counters), for a caller that clears them in a
+// kernel it launches anyway: reserves the sort's scratch, returns the first
+// word and the count.
+inline int radix_sort_prefix(Ctx* ctx, int n, int passes, uint32_t** first, int* words) {
+    GPRT_TRY(ctx->sort_tmp[2].reserve(sizeof(uint32_t) * os_scratch_words(n, passes)));
+    *first = ctx->sort_tmp[2].as<uint32_t>();
+    *words = os_clear_words(passes);
+    return GPRT_OK;
+}
+
 // the result is left in whichever of (keys, vals) and the context's
 // temporaries the last pass wrote, and these receive its location; without
 // them it is copied back into keys / vals.  *passes receives the pass count.
 // err counts look-back walks that gave up (a result to be discarded).
+// prefix_cleared: the caller has zeroed the radix_sort_prefix words in a
+// kernel of its own, ahead of this sort on the same stream.
 template <typename K, class Gen, class Fin>
 int radix_sort_run(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int end_bit, const Gen& gen,
-                   const Fin* fin, K** res_keys, uint32_t** res_vals, int* passes_out, unsigned* err) {
+                   const Fin* fin, K** res_keys, uint32_t** res_vals, int* passes_out, unsigned* err,
+                   bool prefix_cleared = false) {
     if (res_keys) *res_keys = keys;
     if (res_vals) *res_vals = vals;
     if (passes_out) *passes_out = 0;
     if (n <= 1 || begin_bit >= end_bit) return GPRT_OK;
-    const int passes = (end_bit - begin_bit + 7) / 8;
+    const int passes = radix_passes(begin_bit, end_bit);
     if (passes > OS_MAXP || n >= (1 << 30)) return fail(GPRT_ERR_ARG, "radix sort: too many keys or bits");
     const int nblocks = (n + RS_TILE - 1) / RS_TILE;
     GPRT_TRY(ctx->sort_tmp[0].reserve(sizeof(K) * (size_t)n));
@@ -265,7 +284,8 @@ int radix_sort_run(Ctx* ctx, K* keys, uint32_t* vals, int n, int begin_bit, int 
     uint32_t* v1 = ctx->sort_tmp[1].as<uint32_t>();
     const OsLayout L = os_layout(ctx->sort_tmp[2].as<uint32_t>(), n, passes);
     hipStream_t s = ctx->stream;
-    hipLaunchKernelGGL(os_clear_kernel, dim3(1), dim3(256), 0, s, L.totals, os_clear_words(passes));
+    if (!prefix_cleared)
+        hipLaunchKernelGGL(os_clear_kernel, dim3(1), dim3(256), 0, s, L.totals, os_clear_words(passes));
     hipLaunchKernelGGL((os_hist_kernel<K, Gen>), dim3(nblocks), dim3(RS_THREADS), 0, s, gen, n, begin_bit, end_bit,
                        keys, vals, L);
     for (int p = 0; p < passes; ++p) {
