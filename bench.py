@@ -173,11 +173,12 @@ def _range_line(ctx, tree, pts, q, oracle, reps=3):
     med = lambda v: float(np.median(v[1:]))  # noqa: E731
     st = []
     tree.range_query(q, 0.5, flags=gprt_hip.COLLECT_COUNTERS, stats=st)
-    V, P = st[1]["node_visits"], st[1]["point_tests"]
+    V, P, S = st[1]["node_visits"], st[1]["point_tests"], st[1]["rerank_reads"]
     hits = int(off[-1])
     # fill walk: 16 B query + 64 B per node visit + 16 B per point test + 4 B
-    # position written per hit + 16 B of offsets per query
-    alg = 16 * nq + 64 * V + 16 * P + 4 * hits + 16 * nq
+    # per output slot written (a hit's position, or two per subtree taken
+    # whole) + 16 B of offsets per query
+    alg = 16 * nq + 64 * V + 16 * P + 4 * S + 16 * nq
     rng = np.random.default_rng(11)
     sel = np.sort(rng.choice(nq, 2000, replace=False))
     roff, rids = oracle.range_brute(pts, q[sel], 0.5)
@@ -188,7 +189,7 @@ def _range_line(ctx, tree, pts, q, oracle, reps=3):
             "qps": nq / (med(tot) * 1e-3), "results": hits, "results_per_s": hits / (med(tot) * 1e-3),
             "mean_per_query": hits / nq, "max_per_query": int(np.diff(off).max()),
             "roofline": _roof(alg, med(walk_ms), kernel="range fill walk (range_kernel<true>)", node_visits=V,
-                              point_tests=P, count_walk_ms=med(cwalk_ms)),
+                              point_tests=P, slots=S, count_walk_ms=med(cwalk_ms)),
             "sample_eq_oracle": ok, "sample": "2,000 random queries vs FP64 brute force"}
 
 

@@ -146,7 +146,8 @@ typedef struct {
     int64_t node_visits;       /* only with counters enabled (Barnes-Hut: nodes opened) */
     int64_t point_tests;       /* (Barnes-Hut: body-body interactions) */
     int64_t rerank_reads;      /* FP64 re-rank point reads (counters enabled; Barnes-Hut: far-field
-                                  node interactions) */
+                                  node interactions; range query: output slots its walk writes, one
+                                  per hit found by a point test and two per subtree taken whole) */
     double ms_kernel;          /* first traversal round alone (time_kernels; range: GPRT_TIME_KERNELS) */
     int64_t wave_steps;        /* per-lane rounds: sum over wavefront tasks of their longest walk in
                                   loop iterations (counters enabled); lane steps / (64 * this) is the
@@ -258,7 +259,10 @@ GPRT_HIP_API int64_t gprt_hip_knn_noalloc_max_queries(void);
  * NULL fills offsets_out[0..nq] (exclusive prefix of per-query counts).
  * Phase 2: ids_out with capacity ids_capacity >= offsets_out[nq] receives the
  * ids of query q at [offsets_out[q], offsets_out[q+1]), ordered by (d64, id).
- * offsets_out and ids_out follow ptr_flags. */
+ * offsets_out and ids_out follow ptr_flags.  Phase 2 checks every query's
+ * range against the hits it finds: offsets that do not belong to these
+ * queries and this radius give GPRT_ERR_ARG (nothing is written outside
+ * [0, ids_capacity)). */
 GPRT_HIP_API int gprt_hip_range(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const float* queries, int64_t nq,
                                 float radius, int64_t* offsets_out, uint32_t* ids_out, int64_t ids_capacity,
                                 int ptr_flags);

@@ -820,6 +820,9 @@ static int finish_stats(Ctx& c, int64_t nq, bool read_counters, const float* d_r
         if (h[CNT_TOTAL - 1]) return fail(GPRT_ERR_INPUT, "queries contain NaN/Inf or |coordinate| > 1e18");
         if (h[CNT_ERROR]) return fail(GPRT_ERR_STATE, "traversal step guard tripped: corrupt tree");
         if (h[CNT_SORTERR]) return fail(GPRT_ERR_STATE, "query sort: radix sort look-back did not complete");
+        if (h[CNT_OFFSETS])
+            return fail(GPRT_ERR_ARG, "range: offsets_out does not match these queries and radius (" +
+                                          std::to_string(h[CNT_OFFSETS]) + " queries)");
         if (h[CNT_BADID])
             return fail(GPRT_ERR_STATE, "k-NN id guard tripped: " + std::to_string(h[CNT_BADID]) +
                                             " queries re-ranked a candidate outside their cloud");

@@ -184,6 +184,26 @@ GPRT_HD double box2d(float qx, float qy, float qz, const float4& lo, const float
 #define GPRT_BAND_ABS 1e-36f
 GPRT_HD float band(float t) { return __builtin_fmaf(t, GPRT_BAND, GPRT_BAND_ABS); }
 
+// Squared FP32 distance from q to the farthest corner of a box: per axis
+// max(|q - lo|, |q - hi|), then the d2f sum.
+GPRT_HD float boxmax2f(float qx, float qy, float qz, const float4& lo, const float4& hi) {
+    const float mx = fmaxf(fabsf(qx - lo.x), fabsf(qx - hi.x));
+    const float my = fmaxf(fabsf(qy - lo.y), fabsf(qy - hi.y));
+    const float mz = fmaxf(fabsf(qz - lo.z), fabsf(qz - hi.z));
+    return __builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx));
+}
+
+// Inner band: boxmax2f(q, box) <= inner_band(fl32(r*r)) proves d64(q, p) <=
+// (double)r*(double)r for every point p of the box.  Per axis |q - p| <= M,
+// the exact farthest-corner offset, and fl32 rounding is monotone, so
+// boxmax2f >= sum M^2 * (1-u)^5 (u = 2^-24: the offsets, a product and two
+// fma roundings) while d64(p) <= sum M^2 * (1 + 2^-53)^5.  With s = fl32(r*r)
+// <= r^2 (1+u), a threshold s * (1 - 2^-19) = s * (1 - 32u) leaves a factor
+// above 4 over the 7u needed; the subtracted 1e-36 covers flushed subnormal
+// squares.  A threshold <= 0 (tiny radii) proves nothing.
+#define GPRT_INNER 0.9999980926513671875f
+GPRT_HD float inner_band(float s) { return __builtin_fmaf(s, GPRT_INNER, -GPRT_BAND_ABS); }
+
 // wave helpers
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {

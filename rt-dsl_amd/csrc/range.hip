@@ -7,10 +7,13 @@
 // flat array of at most 1e6 ids for all queries with no per-query split
 // (lib.rs:662-668); this one keeps the split.
 //
-// Pipeline: Morton-ordered queries -> FP32 traversal with the guard threshold
-// band(fl32(r*r)) collecting candidates, each confirmed in FP64 -> per-query
-// counts -> offsets (exclusive scan) -> the same traversal writing ids and
-// d64 -> per-query sort of (d64, id).
+// Pipeline: Morton-ordered queries -> FP32 traversal entering boxes within
+// the guard threshold band(fl32(r*r)), points confirmed in FP64, subtrees
+// proved inside the ball (inner_band) taken whole without a visit -> per-query
+// counts -> offsets (exclusive scan) -> the same traversal writing slots
+// (point positions, and (first, count) runs for whole subtrees) -> per-query
+// expansion of the runs, ids and d64 from the gathered points, sort by
+// (d64, id).
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
@@ -20,28 +23,85 @@
 
 namespace gprt {
 
-// per-lane walk with a private stack of EXACT_STACK entries (never overflows:
-// the radix-tree depth bound).  The count pass stores each query's number of
-// hits; the fill pass writes each hit's sorted point position (4 bytes) into
-// the query's output range, in walk order: the sort pass turns positions into
-// ids in (d64, id) order.  COUNT adds node visits and point tests to stats64.
+// Output slots of the fill walk.  A query's range [off[q], off[q+1]) holds one
+// entry per hit in the count, but the walk writes fewer slots: a hit found by
+// a point test is one slot holding its sorted point position; a subtree whose
+// box lies wholly inside the ball (every point of it a hit, see inner_band) is
+// two slots, RUN_START | first position and RUN_LEN | number of points.
+// Positions and lengths are below 2^30 (the radix sort's key limit).  The
+// number of slots written is kept per query (`used`); the sort kernels expand
+// the runs and overwrite the range with ids.
+constexpr uint32_t RUN_START = 0x80000000u, RUN_LEN = 0x40000000u, SLOT_MASK = 0x3FFFFFFFu;
+
+// elements a slot stands for: a position 1, a run start the length held by the
+// next slot, a length slot 0
+__device__ __forceinline__ uint32_t slot_elems(uint32_t v, uint32_t next) {
+    return (v & RUN_START) ? (next & SLOT_MASK) : ((v & RUN_LEN) ? 0u : 1u);
+}
+
+// query g's output range [b, e), clamped into [0, capacity): offsets that do
+// not match the walk never move a write out of the output array
+__device__ __forceinline__ void seg_bounds(const int64_t* __restrict__ off, int g, int64_t capacity, int64_t& b,
+                                           int64_t& e) {
+    b = min(max(off[g], (int64_t)0), capacity);
+    e = min(max(off[g + 1], b), capacity);
+}
+
+// Per-lane walk with a private stack of EXACT_STACK entries (never overflows:
+// the radix-tree depth bound).  A child box within band(r^2) is entered; one
+// whose farthest corner is within inner_band(r^2) is taken whole: its points
+// are the sorted positions [first, first + count), first being ~ref for a leaf
+// bucket, ref - count + 1 for an internal A child and ref for an internal B
+// child (a Karras node i covers [lo, hi] with its A child ending at split
+// gamma = ref A and its B child starting at gamma + 1 = ref B).  The count pass
+// stores each query's number of hits; the fill pass writes slots (above) in
+// walk order and checks that the hits fill the range exactly (a range that
+// does not match the queries and radius is counted in `bad`).  COUNT adds
+// node visits, point tests and slots to stats64.
 template <bool WRITE, bool COUNT>
 __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nodes, const float4* __restrict__ pts,
                                                     int2 root, const float4* __restrict__ qs, int nq, float r_all,
                                                     const float* __restrict__ radii,
                                                     int64_t* __restrict__ counts, const int64_t* __restrict__ off,
-                                                    uint32_t* __restrict__ out, int64_t capacity, int max_steps,
-                                                    unsigned* err, unsigned long long* stats64) {
+                                                    uint32_t* __restrict__ out, int64_t capacity,
+                                                    uint32_t* __restrict__ used, int max_steps, unsigned* err,
+                                                    unsigned* bad,
+                                                    unsigned long long* stats64) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t visits = 0, tests = 0;
+    uint64_t visits = 0, tests = 0, slots = 0;
     if (i < nq) {
         const float4 q = qs[i];
         const int g = __float_as_int(q.w);
         const float r = radii ? radii[g] : r_all;
         const double r2 = (double)r * (double)r;
-        const float tau = band(r * r);
-        int64_t cnt = 0;
-        int64_t pos = WRITE ? off[g] : 0, end = WRITE ? min(off[g + 1], capacity) : 0;
+        const float s2 = r * r;
+        const float tau = band(s2), tin = inner_band(s2);
+        int64_t hits = 0, beg = 0, end = 0;
+        if (WRITE) seg_bounds(off, g, capacity, beg, end);
+        int64_t pos = beg;
+        auto single = [&](int p) {
+            if (WRITE) {
+                if (pos < end) out[pos] = (uint32_t)p;
+                ++pos;
+            }
+            ++hits;
+            if (COUNT) ++slots;
+        };
+        auto run = [&](int f, int c) {
+            if (c == 1) {
+                single(f);
+                return;
+            }
+            if (WRITE) {
+                if (pos + 1 < end) {
+                    out[pos] = RUN_START | (uint32_t)f;
+                    out[pos + 1] = RUN_LEN | (uint32_t)c;
+                }
+                pos += 2;
+            }
+            hits += c;
+            if (COUNT) slots += 2;
+        };
         // a leaf range four points at a time: the loads of a group issue
         // together (indices past the range clamped to its last point, loaded
         // but not tested)
@@ -54,14 +114,8 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     if (t0 + u < m && d2f(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= tau &&
-                        d2d(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= r2) {
-                        if (WRITE) {
-                            if (pos < end) out[pos] = (uint32_t)(f + t0 + u);
-                            ++pos;
-                        } else {
-                            ++cnt;
-                        }
-                    }
+                        d2d(q.x, q.y, q.z, p[u].x, p[u].y, p[u].z) <= r2)
+                        single(f + t0 + u);
             }
         };
         if (root.y > 0) {
@@ -77,15 +131,24 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
                     const float dA = box2f(q.x, q.y, q.z, nd.alo, nd.ahi);
                     const float dB = box2f(q.x, q.y, q.z, nd.blo, nd.bhi);
                     const int rA = __float_as_int(nd.alo.w), rB = __float_as_int(nd.blo.w);
+                    const int nA = __float_as_int(nd.ahi.w), nB = __float_as_int(nd.bhi.w);
                     const bool inA = dA <= tau, inB = dB <= tau;
+                    const bool wA = inA && boxmax2f(q.x, q.y, q.z, nd.alo, nd.ahi) <= tin;
+                    const bool wB = inB && boxmax2f(q.x, q.y, q.z, nd.blo, nd.bhi) <= tin;
+                    if (wA || wB) {
+                        // A's points are followed by B's: both whole are one run
+                        const int fA = rA < 0 ? ~rA : rA - nA + 1;
+                        const int fB = rB < 0 ? ~rB : rB;
+                        run(wA ? fA : fB, (wA ? nA : 0) + (wB ? nB : 0));
+                    }
                     {
                         // the two buckets of a node are adjacent ranges of the sorted
                         // points (A's then B's): in-range ones are scanned as one range
-                        const bool sA = rA < 0 && inA, sB = rB < 0 && inB;
-                        const int mm = (sA ? __float_as_int(nd.ahi.w) : 0) + (sB ? __float_as_int(nd.bhi.w) : 0);
+                        const bool sA = rA < 0 && inA && !wA, sB = rB < 0 && inB && !wB;
+                        const int mm = (sA ? nA : 0) + (sB ? nB : 0);
                         if (mm > 0) visit(sA ? ~rA : ~rB, mm);
                     }
-                    const bool gA = rA >= 0 && inA, gB = rB >= 0 && inB;
+                    const bool gA = rA >= 0 && inA && !wA, gB = rB >= 0 && inB && !wB;
                     if (gA && gB) {
                         stk[sp++] = rB;
                         node = rA;
@@ -98,16 +161,23 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
                 }
             }
         }
-        if (!WRITE) counts[g] = cnt;
+        if (WRITE) {
+            used[g] = (uint32_t)min(pos - beg, end - beg);
+            if (hits != end - beg || off[g] != beg || off[g + 1] != end) atomicAdd(bad, 1u);
+        } else {
+            counts[g] = hits;
+        }
     }
     if (COUNT) {
         for (int o = 32; o > 0; o >>= 1) {
             visits += __shfl_xor(visits, o);
             tests += __shfl_xor(tests, o);
+            slots += __shfl_xor(slots, o);
         }
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&stats64[0], (unsigned long long)visits);
             atomicAdd(&stats64[1], (unsigned long long)tests);
+            atomicAdd(&stats64[2], (unsigned long long)slots);
         }
     }
 }
@@ -222,15 +292,17 @@ enum { SEG_L_SMALL = 0, SEG_L_MID, SEG_L_LARGE, SEG_L_HUGE };
 // longer hits (one-hit segments are converted here); huge ones also get a range
 // of the merge scratch (from the running total huge_total)
 __global__ __launch_bounds__(256) void seg_classify_kernel(const int64_t* __restrict__ off, int nq, int64_t capacity,
-                                                           const float4* __restrict__ pts, uint32_t* ids,
-                                                           int* lists, unsigned* counts, int64_t* huge_tmp,
-                                                           unsigned long long* huge_total) {
+                                                           const float4* __restrict__ pts, uint32_t npts,
+                                                           uint32_t* ids, int* lists, unsigned* counts,
+                                                           int64_t* huge_tmp, unsigned long long* huge_total) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     int64_t m = 0, b = 0;
     if (g < nq) {
-        b = off[g];
-        m = min(off[g + 1], capacity) - b;
-        if (m == 1) ids[b] = (uint32_t)__float_as_int(pts[ids[b]].w);
+        int64_t e;
+        seg_bounds(off, g, capacity, b, e);
+        m = e - b;
+        // a one-hit range holds one position (a run has two or more points)
+        if (m == 1) ids[b] = (uint32_t)__float_as_int(pts[min(ids[b] & SLOT_MASK, npts - 1)].w);
     }
     const int cls = m <= 1 ? -1 : (m <= 64 ? SEG_L_SMALL : (m <= SEG_MID ? SEG_L_MID : (m <= SEG_LARGE ? SEG_L_LARGE
                                                                                                    : SEG_L_HUGE)));
@@ -243,25 +315,45 @@ __global__ __launch_bounds__(256) void seg_classify_kernel(const int64_t* __rest
     }
 }
 
-// segments of 2..64 hits, one wavefront each (a persistent grid over the list)
+// segments of 2..64 hits, one wavefront each (a persistent grid over the
+// list): lane l reads slot l, a wave scan of the slots' element counts gives
+// each slot's first element, and lane e finds the slot of element e by a
+// binary search over the lanes' offsets
 __global__ __launch_bounds__(256) void segsort_small_kernel(const int64_t* __restrict__ off,
                                                             const int* __restrict__ list,
                                                             const unsigned* __restrict__ count,
+                                                            const uint32_t* __restrict__ used,
                                                             const float* __restrict__ qxyz,
                                                             const float* __restrict__ radii, float r_all,
-                                                            const float4* __restrict__ pts, uint32_t* ids,
-                                                            int64_t capacity) {
+                                                            const float4* __restrict__ pts, uint32_t npts,
+                                                            uint32_t* ids, int64_t capacity) {
     const int lane = threadIdx.x & 63;
     const unsigned n = *count, nw = (gridDim.x * blockDim.x) >> 6;
     for (unsigned li = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; li < n; li += nw) {
         const int seg = list[li];
-        const int64_t b = off[seg];
-        const int m = (int)(min(off[seg + 1], capacity) - b);
+        int64_t b, e;
+        seg_bounds(off, seg, capacity, b, e);
+        const int m = (int)(e - b);
+        const int u = (int)min(used[seg], (uint32_t)m);
         const SegQ q = seg_query(qxyz, radii, r_all, seg);
+        const uint32_t v = lane < u ? ids[b + lane] : RUN_LEN;
+        const uint32_t nx = __shfl_down(v, 1);
+        const uint32_t c = lane < u ? slot_elems(v, lane + 1 < u ? nx : 0u) : 0u;
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        const uint32_t ex = x - c;
+        int lo = 0;
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(ex, lo + step) <= (uint32_t)lane) lo += step;
+        const uint32_t sv = __shfl(v, lo), so = __shfl(ex, lo);
+        const uint32_t p0 = (sv & RUN_START) ? (sv & SLOT_MASK) + ((uint32_t)lane - so) : (sv & SLOT_MASK);
         double d = INFINITY;
         uint32_t id = 0xFFFFFFFFu;
         if (lane < m) {
-            const float4 p = pts[ids[b + lane]];
+            const float4 p = pts[min(p0, npts - 1)];
             d = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
             id = (uint32_t)__float_as_int(p.w);
         }
@@ -383,6 +475,82 @@ __device__ void block_bucket_sort(SegLds<CAP, T>& S, int m, double r2) {
     }
 }
 
+// Expands a range's slots[0..u) into element positions in walk order, calling
+// emit(e, position) for each element e < limit; returns the number of elements
+// emitted.  T slots at a time: a block scan of their element counts, then
+// element e of the batch lies in the last slot whose first element is <= e (a
+// binary search over the batch's offsets in LDS).  offs / vals: LDS scratch of
+// T words each; wsum: T / 64 words.  Called by the whole block; ends
+// synchronised.
+template <int T, class Emit>
+__device__ int expand_slots(const uint32_t* __restrict__ slots, int u, int limit, uint32_t* offs, uint32_t* vals,
+                            uint32_t* wsum, Emit emit) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int base = 0;
+    for (int s0 = 0; s0 < u; s0 += T) {
+        const int j = s0 + t;
+        const uint32_t v = j < u ? slots[j] : RUN_LEN;
+        const uint32_t c = j < u ? slot_elems(v, j + 1 < u ? slots[j + 1] : 0u) : 0u;
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int k = 0; k < T / 64; ++k) {
+            const uint32_t ws = wsum[k];
+            before += k < w ? ws : 0u;
+            total += ws;
+        }
+        offs[t] = before + x - c;
+        vals[t] = v;
+        __syncthreads();
+        const int nb = (int)min(total, (uint32_t)(limit - base));
+        for (int e = t; e < nb; e += T) {
+            int lo = 0;
+            for (int step = T / 2; step > 0; step >>= 1)
+                if (offs[lo + step] <= (uint32_t)e) lo += step;
+            const uint32_t sv = vals[lo];
+            emit(base + e, (sv & RUN_START) ? (sv & SLOT_MASK) + ((uint32_t)e - offs[lo]) : (sv & SLOT_MASK));
+        }
+        base += nb;
+        __syncthreads();
+    }
+    return base;
+}
+
+// Loads a range of m hits (u slots) into S.d / S.id: positions expanded into
+// LDS (over S.d), then each lane's points gathered and their d64 and ids
+// stored.  Positions are clamped to the point count, so ranges that do not
+// match the walk (reported by the fill walk) stay in bounds.
+template <int CAP, int T>
+__device__ __forceinline__ void seg_load(SegLds<CAP, T>& S, const uint32_t* slots, int u, int m, const SegQ& q,
+                                         const float4* __restrict__ pts, uint32_t npts) {
+    static_assert(CAP / 2 + 1 >= 2 * T, "expansion scratch lives in S.cnt");
+    static_assert(CAP % T == 0, "elements per lane");
+    constexpr int PER = CAP / T;
+    uint32_t* P = reinterpret_cast<uint32_t*>(S.d);
+    const int n = expand_slots<T>(slots, u, m, S.cnt, S.cnt + T, S.wsum, [&](int e, uint32_t p) { P[e] = p; });
+    uint32_t pos[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * T;
+        pos[i] = e < n ? min(P[e], npts - 1) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * T;
+        if (e < m) {
+            const float4 p = pts[pos[i]];
+            S.d[e] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+            S.id[e] = (uint32_t)__float_as_int(p.w);
+        }
+    }
+}
+
 // gathers positions pos[0..m) of the segment's query into S.d / S.id
 template <int CAP, int T>
 __device__ __forceinline__ void seg_gather(SegLds<CAP, T>& S, const uint32_t* pos, int m, const SegQ& q,
@@ -400,18 +568,20 @@ template <int CAP, int T>
 __global__ __launch_bounds__(T) void segsort_block_kernel(const int64_t* __restrict__ off,
                                                           const int* __restrict__ list,
                                                           const unsigned* __restrict__ count,
+                                                          const uint32_t* __restrict__ used,
                                                           const float* __restrict__ qxyz,
                                                           const float* __restrict__ radii, float r_all,
-                                                          const float4* __restrict__ pts, uint32_t* ids,
-                                                          int64_t capacity) {
+                                                          const float4* __restrict__ pts, uint32_t npts,
+                                                          uint32_t* ids, int64_t capacity) {
     __shared__ SegLds<CAP, T> S;
     const unsigned n = *count;
     for (unsigned li = blockIdx.x; li < n; li += gridDim.x) {
         const int g = list[li];
-        const int64_t b = off[g];
-        const int m = (int)(min(off[g + 1], capacity) - b);
+        int64_t b, e;
+        seg_bounds(off, g, capacity, b, e);
+        const int m = (int)(e - b);
         const SegQ q = seg_query(qxyz, radii, r_all, g);
-        seg_gather(S, ids + b, m, q, pts);
+        seg_load(S, ids + b, (int)min(used[g], (uint32_t)m), m, q, pts, npts);
         __syncthreads();
         block_bucket_sort(S, m, q.r2);
         __syncthreads();
@@ -432,8 +602,9 @@ __device__ __forceinline__ int rank_in(const double* __restrict__ rd, const uint
     return lo;
 }
 
-// One workgroup per segment of more than SEG_LARGE hits: SEG_LARGE-element
-// chunks bucket-sorted in LDS into the merge scratch A, then runs merged
+// One workgroup per segment of more than SEG_LARGE hits: the slots expanded
+// into positions in the merge scratch A, SEG_LARGE-element chunks of them
+// bucket-sorted in LDS back into A, then runs merged
 // pairwise (A -> B -> A ...); each element finds its output slot as its index
 // in its own run plus its rank in the partner run ((d64, id) keys are distinct
 // within a segment, so ranks never collide).  The last pass writes the ids.
@@ -442,24 +613,31 @@ __global__ __launch_bounds__(SEG_HUGE_T) void segsort_huge_kernel(const int64_t*
                                                                   const int* __restrict__ list,
                                                                   const int64_t* __restrict__ huge_tmp,
                                                                   const unsigned* count,
+                                                                  const uint32_t* __restrict__ used,
                                                                   const float* __restrict__ qxyz,
                                                                   const float* __restrict__ radii, float r_all,
-                                                                  const float4* __restrict__ pts, uint32_t* ids,
-                                                                  int64_t capacity, double* tmp_d, uint32_t* tmp_i,
-                                                                  int64_t tmp_half) {
+                                                                  const float4* __restrict__ pts, uint32_t npts,
+                                                                  uint32_t* ids, int64_t capacity, double* tmp_d,
+                                                                  uint32_t* tmp_i, int64_t tmp_half) {
     __shared__ SegLds<SEG_LARGE, SEG_HUGE_T> S;
     if (blockIdx.x >= *count) return;
     const int g = list[blockIdx.x];
-    const int64_t b = off[g];
-    const int m = (int)(min(off[g + 1], capacity) - b);
+    int64_t b, e;
+    seg_bounds(off, g, capacity, b, e);
+    const int m = (int)(e - b);
     const SegQ q = seg_query(qxyz, radii, r_all, g);
     double* ad = tmp_d + huge_tmp[blockIdx.x];
     uint32_t* ai = tmp_i + huge_tmp[blockIdx.x];
     double* bd = ad + tmp_half;
     uint32_t* bi = ai + tmp_half;
+    const int nexp = expand_slots<SEG_HUGE_T>(ids + b, (int)min(used[g], (uint32_t)m), m, S.cnt,
+                                              S.cnt + SEG_HUGE_T, S.wsum,
+                                              [&](int e, uint32_t p) { ai[e] = min(p, npts - 1); });
+    for (int e = nexp + threadIdx.x; e < m; e += SEG_HUGE_T) ai[e] = 0u;
+    __syncthreads();
     for (int c0 = 0; c0 < m; c0 += SEG_LARGE) {
         const int cm = min(SEG_LARGE, m - c0);
-        seg_gather(S, ids + b + c0, cm, q, pts);
+        seg_gather(S, ai + c0, cm, q, pts);
         __syncthreads();
         block_bucket_sort(S, cm, q.r2);
         __syncthreads();
@@ -538,12 +716,12 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     if (!d_ids) {
         if (count)
             hipLaunchKernelGGL((range_kernel<false, true>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
-                               d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, max_steps,
-                               counters + CNT_ERROR, stats64);
+                               d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, nullptr, max_steps,
+                               counters + CNT_ERROR, counters + CNT_OFFSETS, stats64);
         else
             hipLaunchKernelGGL((range_kernel<false, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq,
-                               r, d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, max_steps,
-                               counters + CNT_ERROR, stats64);
+                               r, d_radii, counts_buf.as<int64_t>(), nullptr, nullptr, (int64_t)0, nullptr, max_steps,
+                               counters + CNT_ERROR, counters + CNT_OFFSETS, stats64);
         if (time_walk) {
             GPRT_HIP_TRY(hipEventRecord(ctx->ev3, s));
             ctx->kernel_timed = true;
@@ -563,12 +741,16 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     GPRT_TRY(big_buf.reserve(16ull * nq + 8ull * nq + 16));
     int* lists = big_buf.as<int>();
     int64_t* huge_tmp = (int64_t*)(lists + 4 * (size_t)nq);
+    // slots written per query, in the count buffer (unused by the fill call)
+    uint32_t* used = counts_buf.as<uint32_t>();
     if (count)
         hipLaunchKernelGGL((range_kernel<true, true>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
-                           d_radii, nullptr, d_off, d_ids, capacity, max_steps, counters + CNT_ERROR, stats64);
+                           d_radii, nullptr, d_off, d_ids, capacity, used, max_steps, counters + CNT_ERROR,
+                           counters + CNT_OFFSETS, stats64);
     else
         hipLaunchKernelGGL((range_kernel<true, false>), dim3(grid), dim3(256), 0, s, nodes, pts, root, qs4, nq, r,
-                           d_radii, nullptr, d_off, d_ids, capacity, max_steps, counters + CNT_ERROR, stats64);
+                           d_radii, nullptr, d_off, d_ids, capacity, used, max_steps, counters + CNT_ERROR,
+                           counters + CNT_OFFSETS, stats64);
     if (time_walk) {
         GPRT_HIP_TRY(hipEventRecord(ctx->ev3, s));
         ctx->kernel_timed = true;
@@ -577,20 +759,21 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     // stats64[3]: the huge segments' total
     unsigned* cnt = counters + CNT_AUX;
     unsigned long long* huge_total = stats64 + 3;
-    hipLaunchKernelGGL(seg_classify_kernel, dim3(grid), dim3(256), 0, s, d_off, nq, capacity, pts, d_ids, lists, cnt,
-                       huge_tmp, huge_total);
+    const uint32_t npts = (uint32_t)T.n;
+    hipLaunchKernelGGL(seg_classify_kernel, dim3(grid), dim3(256), 0, s, d_off, nq, capacity, pts, npts, d_ids, lists,
+                       cnt, huge_tmp, huge_total);
     static std::atomic<int> g_small[64], g_mid[64], g_large[64];
     hipLaunchKernelGGL(segsort_small_kernel, dim3(persistent_blocks(segsort_small_kernel, 256, ctx->num_cus, g_small)),
-                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_SMALL * nq, cnt + SEG_L_SMALL, d_q, d_radii, r,
-                       pts, d_ids, capacity);
+                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_SMALL * nq, cnt + SEG_L_SMALL, used, d_q,
+                       d_radii, r, pts, npts, d_ids, capacity);
     hipLaunchKernelGGL((segsort_block_kernel<SEG_MID, 256>),
                        dim3(persistent_blocks(segsort_block_kernel<SEG_MID, 256>, 256, ctx->num_cus, g_mid)),
-                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_MID * nq, cnt + SEG_L_MID, d_q, d_radii, r, pts,
-                       d_ids, capacity);
+                       dim3(256), 0, s, d_off, lists + (size_t)SEG_L_MID * nq, cnt + SEG_L_MID, used, d_q, d_radii, r,
+                       pts, npts, d_ids, capacity);
     hipLaunchKernelGGL((segsort_block_kernel<SEG_LARGE, 1024>),
                        dim3(persistent_blocks(segsort_block_kernel<SEG_LARGE, 1024>, 1024, ctx->num_cus, g_large)),
-                       dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_LARGE * nq, cnt + SEG_L_LARGE, d_q, d_radii, r,
-                       pts, d_ids, capacity);
+                       dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_LARGE * nq, cnt + SEG_L_LARGE, used, d_q,
+                       d_radii, r, pts, npts, d_ids, capacity);
     struct {
         unsigned nhuge;
         unsigned long long total;
@@ -602,8 +785,9 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
         GPRT_TRY(ctx->sort_tmp[0].reserve(16ull * h.total));
         GPRT_TRY(ctx->sort_tmp[1].reserve(8ull * h.total));
         hipLaunchKernelGGL(segsort_huge_kernel, dim3(h.nhuge), dim3(SEG_HUGE_T), 0, s, d_off,
-                           lists + (size_t)SEG_L_HUGE * nq, huge_tmp, cnt + SEG_L_HUGE, d_q, d_radii, r, pts, d_ids,
-                           capacity, ctx->sort_tmp[0].as<double>(), ctx->sort_tmp[1].as<uint32_t>(), (int64_t)h.total);
+                           lists + (size_t)SEG_L_HUGE * nq, huge_tmp, cnt + SEG_L_HUGE, used, d_q, d_radii, r, pts,
+                           npts, d_ids, capacity, ctx->sort_tmp[0].as<double>(), ctx->sort_tmp[1].as<uint32_t>(),
+                           (int64_t)h.total);
     }
     GPRT_HIP_TRY(hipGetLastError());
     return GPRT_OK;

@@ -242,6 +242,68 @@ def test_range_segment_tiers(ctx):
         t.close()
 
 
+def test_range_whole_subtrees(ctx):
+    """Subtrees inside the ball are emitted as runs and expanded by the sorts:
+    a radius covering the whole cloud (the root's two children taken whole, one
+    run of every point per query, lists in the chunked-merge tier), a dense
+    lattice where most hits come from whole subtrees, radii so small that the
+    inner threshold proves nothing, and points on the ball's surface."""
+    rng = np.random.default_rng(31)
+    pts = rng.random((12_000, 3)).astype(np.float32)
+    q = np.concatenate([rng.random((6, 3)), [[0.5, 0.5, 0.5], [3.0, 3.0, 3.0]]]).astype(np.float32)
+    t = ctx.build(pts, leaf_size=8)
+    try:
+        for r in (2.0, 5.0):  # every point of the cloud for the queries in the unit cube
+            off, ids = t.range_query(q[:7], r)
+            assert np.all(np.diff(off) == len(pts))
+            _check_range(off, ids, *oracle.range_brute(pts, q[:7], r))
+        for r in (1e-20, 1e-19, 0.0):
+            _check_range(*t.range_query(q, r), *oracle.range_brute(pts, q, r))
+    finally:
+        t.close()
+    g = np.arange(40, dtype=np.float32) * np.float32(0.25)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).copy()
+    ql = np.concatenate([lat[::997], lat[::1311] + np.float32(0.125)]).astype(np.float32)
+    t = ctx.build(lat, leaf_size=4)
+    try:
+        for r in (0.5, 1.0, 1.75, 2.5):  # lattice points exactly on the sphere, long lists
+            rr = float(np.float32(r))
+            off, ids = t.range_query(ql, rr, stats=(st := []))
+            assert st[1]["rerank_reads"] == 0  # counters off
+            _check_range(off, ids, *oracle.range_brute(lat, ql, rr))
+        st = []
+        off, ids = t.range_query(ql, 1.75, flags=gprt_hip.COLLECT_COUNTERS, stats=st)
+        # slots written (positions and two-slot runs) are far fewer than hits
+        assert 0 < st[1]["rerank_reads"] < off[-1] // 2, (st[1]["rerank_reads"], off[-1])
+        assert st[0]["rerank_reads"] == st[1]["rerank_reads"]
+    finally:
+        t.close()
+
+
+def test_range_fill_offsets_mismatch(ctx):
+    """A fill call whose offsets do not match the queries and radius fails with
+    GPRT_ERR_ARG (the walk checks every range it fills) and writes nowhere
+    outside the ids array; the context keeps working."""
+    pts, q = D.uniform_cloud(20_000, 500, seed=8)
+    t = ctx.build(pts)
+    L = gprt_hip.lib()
+    try:
+        off, ids = t.range_query(q, 0.08)
+        qq = np.ascontiguousarray(q, np.float32)
+        for bad_off in (off // 2, np.concatenate([[0], np.full(len(q), off[-1] // 2)]).astype(np.int64)):
+            bad_off = np.ascontiguousarray(bad_off, np.int64)
+            total = int(bad_off[-1])
+            buf = np.full(total + 64, 0xA5A5A5A5, np.uint32)
+            rc = L.gprt_hip_range(ctx.h, t.h, qq.ctypes.data, len(q), ctypes.c_float(0.08), bad_off.ctypes.data,
+                                  buf.ctypes.data, total, gprt_hip.PTR_HOST)
+            assert rc == 1, rc  # GPRT_ERR_ARG
+            assert "offsets_out does not match" in L.gprt_hip_last_error().decode()
+            assert np.all(buf[total:] == 0xA5A5A5A5)
+        _check_range(*t.range_query(q, 0.08), off, ids)
+    finally:
+        t.close()
+
+
 def test_range_lidar_subset(ctx):
     pts, q = D.lidar_scan(frames=4, nq=20_000)
     t = ctx.build(pts)

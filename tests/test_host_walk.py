@@ -92,7 +92,9 @@ def test_host_tree_layout(kind, n, leaf):
     """Keys are the 21-bit-per-axis Morton codes over the cube key box, stably
     sorted; every child box is the min/max of its point range and the child
     counts partition the node's range (a walk of the tree reaches every point
-    once)."""
+    once).  An internal child's first sorted point follows from its reference
+    alone, as the range walk's whole-subtree runs assume: ref - count + 1 for
+    an A child (a Karras node ending at the split), ref for a B child."""
     rng = np.random.default_rng(n)
     pts = _cloud(kind, n, rng)
     keys, perm, nodes, root = H.build_tree(pts, leaf)
@@ -115,7 +117,9 @@ def test_host_tree_layout(kind, n, leaf):
         ca, cb = int(nd[7:8].view(np.int32)[0]), int(nd[15:16].view(np.int32)[0])
         ra, rb = int(nd[3:4].view(np.int32)[0]), int(nd[11:12].view(np.int32)[0])
         assert ca + cb == cnt
-        for (r, c, f, box) in ((ra, ca, first, nd[0:8]), (rb, cb, first + ca, nd[8:16])):
+        for (r, c, f, box, side) in ((ra, ca, first, nd[0:8], 0), (rb, cb, first + ca, nd[8:16], 1)):
+            if r >= 0:
+                assert (r if side else r - c + 1) == f
             seg = sp[f:f + c]
             assert np.array_equal(box[0:3], seg.min(0)) and np.array_equal(box[4:7], seg.max(0))
             stack.append((r, c, f))
