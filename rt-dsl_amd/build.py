@@ -1,6 +1,6 @@
 """Build libgprt_hip.so (all HIP translation units, gfx950 only) in-tree.
 
-    python rt-dsl_amd/build.py [-j N]
+    python rt-dsl_amd/build.py [-j N] [--incremental]
 
 Objects go to rt-dsl_amd/_build/, the library to rt-dsl_amd/gprt_hip/, together
 with libgprt_host.so (the k-NN walk built for the host, for CPU tests).
@@ -52,18 +52,36 @@ def compile_one(src, build_dir=BUILD, defines=(), extra=()):
     return obj, err + f"\n[{src}: {time.time() - t0:.0f} s]"
 
 
-def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=(), sanitize=False):
+def _up_to_date(src, build_dir):
+    """True when the object of `src` is newer than it and than every header
+    (csrc/*.hpp and include/gprt_hip.h)."""
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    if not os.path.exists(obj):
+        return False
+    deps = [os.path.join(CSRC, src), os.path.join(HERE, "..", "include", "gprt_hip.h")]
+    deps += [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    return os.path.getmtime(obj) > max(os.path.getmtime(d) for d in deps)
+
+
+def build(jobs=None, verbose=True, out=OUT, build_dir=BUILD, defines=(), sanitize=False, incremental=False):
     """Compile every translation unit and link the library.  `defines` are
     extra -D options (tuning constants of the kernels), for builds of the
     same library into another `out` / `build_dir`.  sanitize=True builds the
     host code with ASan + UBSan (load it with the shared ASan runtime
-    preloaded)."""
+    preloaded).  incremental=True skips translation units whose object is
+    newer than their source and every header (same defines assumed)."""
     os.makedirs(build_dir, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
     extra = SANITIZE if sanitize else []
+
+    def one(src):
+        if incremental and _up_to_date(src, build_dir):
+            return os.path.join(build_dir, os.path.splitext(src)[0] + ".o"), ""
+        return compile_one(src, build_dir, defines, extra)
+
     with cf.ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(lambda src: compile_one(src, build_dir, defines, extra), srcs))
+        results = list(ex.map(one, srcs))
     for obj, err in results:
         if err.strip() and verbose:
             print(err, file=sys.stderr)
@@ -120,9 +138,11 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=OUT)
     ap.add_argument("--build-dir", default=BUILD)
     ap.add_argument("--sanitize", action="store_true", help="ASan + UBSan host build into _build_asan/")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources changed since their object (or all, after a header change)")
     a = ap.parse_args()
     if a.sanitize:
         print(build_sanitized(a.j))
     else:
-        build(a.j, out=a.out, build_dir=a.build_dir, defines=a.defines)
+        build(a.j, out=a.out, build_dir=a.build_dir, defines=a.defines, incremental=a.incremental)
         build_host()
