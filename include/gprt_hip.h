@@ -82,6 +82,13 @@ extern "C" {
 /* gprt_hip_range: time the traversal kernel of the call (count walk in the
  * first phase, fill walk in the second) alone, into ms_kernel */
 #define GPRT_TIME_KERNELS 32
+/* tree build: split the Morton-ordered points top-down by binned SAH (8 bins
+ * per axis, stable partitions) instead of the Karras radix hierarchy; same
+ * node layout, so every query runs on either tree with the same results.
+ * Fewer node visits on surface-like data, at a higher build cost.  No
+ * counterpart in the reference (its GAS build is OptiX's,
+ * gprt-optix/host/main.cpp:200-248). */
+#define GPRT_BUILD_SAH 64
 
 #define GPRT_ID_NONE 0xFFFFFFFFu
 

@@ -29,7 +29,7 @@ HIPFLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-SOURCES = ["lbvh.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hip", "knn_qwave.hip", "api.hip", "range.hip",
+SOURCES = ["lbvh.hip", "sah.hip", "knn.hip", "knn_lane.hip", "knn_wide.hip", "knn_packet.hip", "knn_qwave.hip", "api.hip", "range.hip",
            "bh.hip", "legacy.hip", "autotune.hip", "codegen.hip", "io.hip"]
 
 

@@ -428,6 +428,7 @@ static int build_common(gprt_hip_ctx* h, const float* pts, const int64_t* off, i
         for (int i = 0; i < TREE_BUFS; ++i) std::swap(*bufs[i], c.tree_pool[i]);
     }
     GPRT_HIP_TRY(hipEventRecord(c.ev2, c.stream));
+    t->t.sah = (flags & GPRT_BUILD_SAH) != 0;
     int rc = tree_build(&c, d_pts, off, nclouds, leaf_size, &t->t, (flags & GPRT_KEYS_REF30) ? 10 : 0);
     if (rc != GPRT_OK) {
         gprt_hip_tree_destroy(t);

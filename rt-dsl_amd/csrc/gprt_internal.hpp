@@ -107,6 +107,7 @@ struct Tree {
     std::vector<int64_t> h_cloud_off;
     float h_bbox[6] = {0, 0, 0, 0, 0, 0};
     bool build_wide = true;  // also build the 4-wide view (k-NN trees)
+    bool sah = false;        // split by binned SAH (sah.hip) instead of the Karras hierarchy
     int key_bits = 30;  // significant bits of the sort key
     int axis_bits = 10; // Morton bits per axis
     int max_depth_bound = 64;
@@ -231,6 +232,10 @@ __device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
 int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int nclouds, int leaf_size, Tree* t,
                int axis_bits = 0);
 int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
+// the SAH hierarchy over the Morton-sorted points of t (sah.hip): reorders
+// pts4 / perm / codes inside each cloud, writes node references and counts,
+// parent links, ranges and roots; the refit fills the boxes afterwards
+int sah_build(Ctx* ctx, Tree* t);
 int radix_sort_pairs_u64(Ctx* ctx, uint64_t* keys, uint32_t* vals, int n, int begin_bit, int end_bit);
 // with res_keys/res_vals the result may stay in the context's sort temporaries
 // (see radix_sort_impl); without them it is copied back into keys/vals

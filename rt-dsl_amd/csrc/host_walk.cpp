@@ -19,6 +19,7 @@
 #include "bh_walk.hpp"
 #include "knn_internal.hpp"
 #include "lane_walk.hpp"
+#include "sah.hpp"
 
 namespace {
 
@@ -35,6 +36,7 @@ struct HostTree {
     std::vector<float4> pts4;
     std::vector<Node> nodes;
     std::vector<int2> ranges;  // sorted point range [x, y] of each internal node
+    std::vector<int> parent;   // (node << 1 | side) of internal nodes, then of points; -1 at the root
     int2 root{0, 0};
 };
 
@@ -55,6 +57,9 @@ int delta(const std::vector<uint64_t>& k, int n, int i, int j) {
     if (k[i] == k[j]) return 64 + __builtin_clz((unsigned)(i ^ j));
     return __builtin_clzll(k[i] ^ k[j]);
 }
+
+void child_boxes(HostTree& T);
+[[noreturn]] void oob(const char* what, long long idx, long long size);
 
 void build_tree(const float* p, int n, int leaf, HostTree& T) {
     T.n = n;
@@ -127,10 +132,18 @@ void build_tree(const float* p, int n, int leaf, HostTree& T) {
         parent[(hi == gamma + 1 ? (n - 1) : 0) + gamma + 1] = (i << 1) | 1;
     }
     T.ranges = ranges;
-    // boxes: every child box is the min/max of the points of its range
+    T.parent = parent;
+    child_boxes(T);
+}
+
+// every child box of every internal node is the min/max of the points of its
+// range (what the refit computes)
+void child_boxes(HostTree& T) {
+    const int n = T.n;
     for (int i = 0; i < n - 1; ++i) {
+        if (T.ranges[i].y < T.ranges[i].x) continue;  // unused id
         Node& nd = T.nodes[i];
-        const int lo = ranges[i].x, hi = ranges[i].y, cA = as_int(nd.ahi.w);
+        const int lo = T.ranges[i].x, hi = T.ranges[i].y, cA = as_int(nd.ahi.w);
         for (int side = 0; side < 2; ++side) {
             const int f = side ? lo + cA : lo, l = side ? hi : lo + cA - 1;
             float4 bl = make_float4(3.402823466e38f, 3.402823466e38f, 3.402823466e38f, 0.f);
@@ -146,6 +159,109 @@ void build_tree(const float* p, int n, int leaf, HostTree& T) {
             H.x = bh.x; H.y = bh.y; H.z = bh.z;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// SAH tree, host restatement of sah.hip: the LBVH order of build_tree, then
+// segments split by sah_choose (sah.hpp) with stable partitions, down to single
+// points (subtrees of at most `leaf` points, and segments at depth
+// SAH_MAX_DEPTH or more, split at their index median; the walk scans buckets);
+// node ids in the Karras numbering (sah_seg_id).
+// ---------------------------------------------------------------------------
+void build_tree_sah(const float* p, int n, int leaf, HostTree& T) {
+    build_tree(p, n, leaf, T);
+    if (n < 2) return;
+    T.nodes.assign(n - 1, Node{});
+    T.ranges.assign(n - 1, make_int2(0, -1));
+    T.parent.assign(2 * n - 1, -1);
+    struct Seg { int lo, hi, plink, depth; float blo[3], bhi[3]; };
+    std::vector<Seg> stack;
+    Seg root{0, n, -2, 0, {3.402823466e38f, 3.402823466e38f, 3.402823466e38f},
+             {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f}};
+    for (int i = 0; i < n; ++i) {
+        const float v[3] = {T.pts4[i].x, T.pts4[i].y, T.pts4[i].z};
+        for (int a = 0; a < 3; ++a) {
+            root.blo[a] = sah_dec(std::min(sah_enc(root.blo[a]), sah_enc(v[a])));
+            root.bhi[a] = sah_dec(std::max(sah_enc(root.bhi[a]), sah_enc(v[a])));
+        }
+    }
+    T.root = n > leaf ? make_int2(0, n) : make_int2(~0, n);
+    stack.push_back(root);
+    std::vector<uint32_t> w(SAH_WORDS);
+    std::vector<float4> tp;
+    std::vector<uint32_t> tperm;
+    std::vector<uint64_t> tkey;
+    while (!stack.empty()) {
+        const Seg g = stack.back();
+        stack.pop_back();
+        const int m = g.hi - g.lo;
+        const int link = g.plink >= 0 ? g.plink : -1;
+        if (m < 2) {
+            if (m == 1) T.parent[(n - 1) + g.lo] = link;
+            continue;
+        }
+        int mid = g.lo + m / 2;
+        float llo[3], lhi[3], rlo[3], rhi[3];
+        for (int a = 0; a < 3; ++a) { llo[a] = rlo[a] = g.blo[a]; lhi[a] = rhi[a] = g.bhi[a]; }
+        if (m > leaf && g.depth < SAH_MAX_DEPTH) {
+            float sc[3];
+            for (int a = 0; a < 3; ++a) sc[a] = sah_scale(g.blo[a], g.bhi[a]);
+            for (int j = 0; j < SAH_WORDS; ++j) w[j] = sah_init_word(j);
+            for (int i = g.lo; i < g.hi; ++i) {
+                const float v[3] = {T.pts4[i].x, T.pts4[i].y, T.pts4[i].z};
+                for (int a = 0; a < 3; ++a) {
+                    const int b = sah_bin(v[a], g.blo[a], sc[a]);
+                    ++w[sah_cnt_word(a, b)];
+                    for (int k = 0; k < 3; ++k) {
+                        uint32_t& l = w[sah_lo_word(a, b, k)];
+                        uint32_t& h = w[sah_hi_word(a, b, k)];
+                        l = std::min(l, sah_enc(v[k]));
+                        h = std::max(h, sah_enc(v[k]));
+                    }
+                }
+            }
+            SahSplit sp;
+            if (sah_choose(w.data(), sp)) {
+                tp.clear(); tperm.clear(); tkey.clear();
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int i = g.lo; i < g.hi; ++i) {
+                        const float4 q = T.pts4[i];
+                        const float c = sp.axis == 0 ? q.x : (sp.axis == 1 ? q.y : q.z);
+                        const bool left = sah_bin(c, g.blo[sp.axis], sc[sp.axis]) <= sp.bin;
+                        if (left == (pass == 0)) {
+                            tp.push_back(q);
+                            tperm.push_back(T.perm[i]);
+                            tkey.push_back(T.keys[i]);
+                        }
+                    }
+                for (int i = 0; i < m; ++i) {
+                    T.pts4[g.lo + i] = tp[i];
+                    T.perm[g.lo + i] = tperm[i];
+                    T.keys[g.lo + i] = tkey[i];
+                }
+                mid = g.lo + (int)sp.nleft;
+                for (int a = 0; a < 3; ++a) {
+                    llo[a] = sp.llo[a]; lhi[a] = sp.lhi[a];
+                    rlo[a] = sp.rlo[a]; rhi[a] = sp.rhi[a];
+                }
+            }
+        }
+        const int id = sah_seg_id(g.lo, g.hi, g.plink);
+        if (id < 0 || id >= n - 1) oob("sah node", id, n - 1);
+        T.ranges[id] = make_int2(g.lo, g.hi - 1);
+        T.parent[id] = link;
+        Node& nd = T.nodes[id];
+        nd.ahi.w = as_float(mid - g.lo);
+        nd.bhi.w = as_float(g.hi - mid);
+        nd.alo.w = as_float(mid - g.lo <= leaf ? ~g.lo : mid - 1);
+        nd.blo.w = as_float(g.hi - mid <= leaf ? ~mid : mid);
+        if (m > leaf && g.plink < 0) T.root = make_int2(id, m);
+        Seg b{mid, g.hi, (id << 1) | 1, g.depth + 1, {rlo[0], rlo[1], rlo[2]}, {rhi[0], rhi[1], rhi[2]}};
+        Seg a{g.lo, mid, (id << 1) | 0, g.depth + 1, {llo[0], llo[1], llo[2]}, {lhi[0], lhi[1], lhi[2]}};
+        stack.push_back(b);
+        stack.push_back(a);
+    }
+    child_boxes(T);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,22 +553,56 @@ __attribute__((visibility("default"))) int gprt_host_build_tree(const float* pts
     return 0;
 }
 
+// Either tree with its refit links: mode 0 the LBVH, 1 the SAH tree of
+// build_tree_sah.  keys / perm / nodes / root as gprt_host_build_tree; parent
+// (2n-1 ints) and ranges (2 ints per internal node; [0, -1] for an unused id).
+// Any output may be NULL.
+__attribute__((visibility("default"))) int gprt_host_build_tree_ex(const float* pts, int64_t n, int leaf_size,
+                                                                   int mode, uint64_t* keys, uint32_t* perm,
+                                                                   float* nodes, int32_t* root, int32_t* parent,
+                                                                   int32_t* ranges) {
+    g_msg.clear();
+    try {
+        if (n < 0 || n > 0x7FFFFFF0 || leaf_size < 1 || leaf_size > 32 || mode < 0 || mode > 1)
+            throw std::invalid_argument("bad argument");
+        HostTree T;
+        if (mode == 1) build_tree_sah(pts, (int)n, leaf_size, T);
+        else build_tree(pts, (int)n, leaf_size, T);
+        if (keys) std::memcpy(keys, T.keys.data(), 8 * (size_t)n);
+        if (perm) std::memcpy(perm, T.perm.data(), 4 * (size_t)n);
+        if (nodes && !T.nodes.empty() && n > 1) std::memcpy(nodes, T.nodes.data(), sizeof(Node) * (size_t)(n - 1));
+        if (root) {
+            root[0] = T.root.x;
+            root[1] = T.root.y;
+        }
+        if (parent && n > 1) std::memcpy(parent, T.parent.data(), sizeof(int) * (size_t)(2 * n - 1));
+        if (ranges && n > 1) std::memcpy(ranges, T.ranges.data(), sizeof(int2) * (size_t)(n - 1));
+    } catch (const std::exception& e) {
+        g_msg = e.what();
+        return 1;
+    }
+    return 0;
+}
+
 // The batched form's per-lane path for each query (see run_query) with a list
 // of `cap` slots (8, 16, 24, 32, 40, 48, 64 or 67).  ids_out: nq*k (rows of
 // status != 0 are left untouched); status_out: nq; stats_out[3]: node visits,
 // point tests, walk trips.  Returns 0, or 1 with gprt_host_last_error() naming
 // the first out-of-bounds access or bad argument.
-__attribute__((visibility("default"))) int gprt_host_lane_knn(const float* pts, int64_t n, int leaf_size,
-                                                              const float* qs, int64_t nq, int k, int cap,
-                                                              float radius0, float mult, int rounds, int stack_depth,
-                                                              uint32_t* ids_out, int32_t* status_out,
-                                                              int64_t* stats_out) {
+// tree_mode: 0 the LBVH, 1 the SAH tree.
+__attribute__((visibility("default"))) int gprt_host_lane_knn_ex(const float* pts, int64_t n, int leaf_size,
+                                                                 int tree_mode, const float* qs, int64_t nq, int k,
+                                                                 int cap, float radius0, float mult, int rounds,
+                                                                 int stack_depth, uint32_t* ids_out,
+                                                                 int32_t* status_out, int64_t* stats_out) {
     g_msg.clear();
     try {
-        if (n < 0 || n > 0x7FFFFFF0 || nq < 0 || k < 1 || rounds < 1 || stack_depth < 8 || !(mult > 1.0f))
+        if (n < 0 || n > 0x7FFFFFF0 || nq < 0 || k < 1 || rounds < 1 || stack_depth < 8 || !(mult > 1.0f) ||
+            tree_mode < 0 || tree_mode > 1)
             throw std::invalid_argument("bad argument");
         HostTree T;
-        build_tree(pts, (int)n, leaf_size, T);
+        if (tree_mode == 1) build_tree_sah(pts, (int)n, leaf_size, T);
+        else build_tree(pts, (int)n, leaf_size, T);
         int64_t st[3] = {0, 0, 0};
         for (int64_t i = 0; i < nq; ++i) {
             const float* q = qs + 3 * i;
@@ -477,6 +627,15 @@ __attribute__((visibility("default"))) int gprt_host_lane_knn(const float* pts, 
         return 1;
     }
     return 0;
+}
+
+__attribute__((visibility("default"))) int gprt_host_lane_knn(const float* pts, int64_t n, int leaf_size,
+                                                              const float* qs, int64_t nq, int k, int cap,
+                                                              float radius0, float mult, int rounds, int stack_depth,
+                                                              uint32_t* ids_out, int32_t* status_out,
+                                                              int64_t* stats_out) {
+    return gprt_host_lane_knn_ex(pts, n, leaf_size, 0, qs, nq, k, cap, radius0, mult, rounds, stack_depth, ids_out,
+                                 status_out, stats_out);
 }
 
 // Barnes-Hut for the bodies `sel` (original indices; NULL = all n) as

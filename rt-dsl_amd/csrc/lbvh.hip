@@ -393,9 +393,12 @@ int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_off, int nclouds, 
         hipLaunchKernelGGL(fill_int_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->flags.as<int>(), n - 1, 0);
         int* ready = t->ready.as<int>();
         hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, ready + n, 1, 0);
-        hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), 3 * b, n,
-                           leaf_size, t->cloud_off.as<int64_t>(), t->nodes.as<Node>(), t->parent.as<int>(),
-                           t->roots.as<int2>(), t->ranges.as<int2>());
+        if (t->sah)
+            GPRT_TRY(sah_build(ctx, t));
+        else
+            hipLaunchKernelGGL(karras_kernel, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, t->codes.as<uint64_t>(), 3 * b,
+                               n, leaf_size, t->cloud_off.as<int64_t>(), t->nodes.as<Node>(), t->parent.as<int>(),
+                               t->roots.as<int2>(), t->ranges.as<int2>());
         BoxSlots slots{t->nodes.as<Node>(), t->pts4.as<float4>()};
         launch_refit<BoxP>(s, slots, t->parent.as<int>(), t->ranges.as<int2>(), n, t->flags.as<int>(), ready,
                            (unsigned*)(ready + n));

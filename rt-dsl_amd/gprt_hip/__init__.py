@@ -22,6 +22,7 @@ LIB_PATH = os.environ.get("GPRT_HIP_LIBRARY", LIB_PATH)
 OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
 PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30, BH_REFERENCE_LAW = 0, 1, 2, 4, 8, 16
+BUILD_SAH = 64
 TIME_KERNELS = 32
 ID_NONE = 0xFFFFFFFF
 K_MAX = 4096  # GPRT_HIP_K_MAX: largest k of the k-NN entry points
@@ -289,21 +290,23 @@ class Context:
         return s.as_dict()
 
     # ---- trees ----
-    def build(self, pts, leaf_size=8, ref_keys=False):
-        """LBVH over pts; ref_keys sorts by the reference's 30-bit Morton codes."""
+    def build(self, pts, leaf_size=8, ref_keys=False, sah=False):
+        """LBVH over pts; ref_keys sorts by the reference's 30-bit Morton codes;
+        sah splits the Morton-ordered points by binned SAH instead of the Karras
+        hierarchy (GPRT_BUILD_SAH)."""
         pts = _points(pts)
         p, dev = _ptr(pts)
         t = ctypes.c_void_p()
-        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0)
+        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0) | (BUILD_SAH if sah else 0)
         _check(lib().gprt_hip_tree_build(self.h, p, len(pts), leaf_size, flags, ctypes.byref(t)))
         return Tree(self, t, pts)
 
-    def build_batched(self, pts, cloud_offsets, leaf_size=8, ref_keys=False):
+    def build_batched(self, pts, cloud_offsets, leaf_size=8, ref_keys=False, sah=False):
         pts = _points(pts)
         off = np.ascontiguousarray(cloud_offsets, np.int64)
         p, dev = _ptr(pts)
         t = ctypes.c_void_p()
-        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0)
+        flags = (PTR_DEVICE if dev else PTR_HOST) | (KEYS_REF30 if ref_keys else 0) | (BUILD_SAH if sah else 0)
         _check(lib().gprt_hip_tree_build_batched(self.h, p, off.ctypes.data, len(off) - 1, leaf_size, flags,
                                                  ctypes.byref(t)))
         return Tree(self, t, pts)

@@ -35,6 +35,11 @@ def lib():
         L.gprt_host_lane_knn.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, _f32p, ctypes.c_int64, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                          _u32p, _i32p, _i64p]
+        L.gprt_host_build_tree_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _u64p, _u32p,
+                                              _f32p, _i32p, _i32p, _i32p]
+        L.gprt_host_lane_knn_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _f32p,
+                                            ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_int, ctypes.c_int, _u32p, _i32p, _i64p]
         L.gprt_host_bh.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, _f32p, _f64p,
                                    _f64p, _f64p, _i64p]
@@ -55,7 +60,26 @@ def build_tree(pts, leaf_size=8):
     return keys, perm, nodes[: max(n - 1, 0)], root
 
 
-def lane_knn(pts, queries, k, cap, leaf_size=8, radius0=1.0, mult=3.0, rounds=3, stack_depth=64):
+def build_tree_ex(pts, leaf_size=8, sah=False):
+    """(keys, perm, nodes (n-1, 16), root (2,), parent (2n-1,), ranges (n-1, 2)) of
+    the LBVH or (sah=True) of the SAH tree of sah.hip, as the GPU lays them out;
+    ranges of unused node ids are [0, -1]."""
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = len(pts)
+    keys = np.zeros(n, np.uint64)
+    perm = np.zeros(n, np.uint32)
+    nodes = np.zeros((max(n - 1, 1), 16), np.float32)
+    root = np.zeros(2, np.int32)
+    parent = np.full(max(2 * n - 1, 1), -1, np.int32)
+    ranges = np.zeros((max(n - 1, 1), 2), np.int32)
+    rc = lib().gprt_host_build_tree_ex(pts, n, leaf_size, 1 if sah else 0, keys, perm, nodes, root, parent, ranges)
+    if rc:
+        raise RuntimeError(lib().gprt_host_last_error().decode())
+    m = max(n - 1, 0)
+    return keys, perm, nodes[:m], root, parent[: max(2 * n - 1, 0)], ranges[:m]
+
+
+def lane_knn(pts, queries, k, cap, leaf_size=8, radius0=1.0, mult=3.0, rounds=3, stack_depth=64, sah=False):
     """The batched form's per-lane path, one query at a time.  Returns (ids
     (nq, k), status (nq,): 0 written by the re-rank, 1 sent to the exact kernel,
     2 id guard tripped, stats: node visits, point tests, trips).  Raises on an
@@ -65,8 +89,8 @@ def lane_knn(pts, queries, k, cap, leaf_size=8, radius0=1.0, mult=3.0, rounds=3,
     ids = np.full((len(q), k), 0xFFFFFFFF, np.uint32)
     st = np.zeros(len(q), np.int32)
     stats = np.zeros(3, np.int64)
-    rc = lib().gprt_host_lane_knn(pts, len(pts), leaf_size, q, len(q), k, cap, radius0, mult, rounds, stack_depth,
-                                  ids, st, stats)
+    rc = lib().gprt_host_lane_knn_ex(pts, len(pts), leaf_size, 1 if sah else 0, q, len(q), k, cap, radius0, mult,
+                                     rounds, stack_depth, ids, st, stats)
     if rc:
         raise RuntimeError(lib().gprt_host_last_error().decode())
     return ids, st, stats
