@@ -1,29 +1,31 @@
 """Headline benchmark: exact k-NN, k = 5, on the LiDAR-like workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--leaf L] [--no-cpu] [--no-extra] [--all]
-                    [--traffic auto|off] [--only range|barnes-hut|build]
+                    [--traffic auto|off] [--only range|barnes-hut|build] [--sah]
 
 Workload: 1,310,720 float32 points (20 frames x 64 rings x 1,024 azimuths of a
 ground-plane + boxes + 120 m cylinder scene, 2 cm range noise) and 400,000
 queries (dataset points + 2 cm jitter), k = 5, generated with fixed seeds (the
 sha256 of the generated arrays is part of the output).  One step = one exact
-k-NN pass of all queries over a prebuilt LBVH, with the queries already
+k-NN pass of all queries over a prebuilt tree, with the queries already
 resident in HBM and the ids left resident in HBM.
 
 Prints one JSON line.  `value` is queries/s of the batched FP32 + FP64-recheck
 form (device-resident inputs, K steps enqueued back to back between two
-synchronisations), with the default schedule or the autotuner's pick when that
-is faster at identical ids; the plain single-launch FP64 form is timed the same
-way on the same tree.  The roofline entry is for the dominant kernel (the first
-traversal round): algorithmic bytes counted by an instrumented run, divided by
-the kernel's duration timed with HIP events on the library stream; `traffic` is
+synchronisations), with the default schedule (LBVH, leaf 8) or the autotuner's
+pick (schedule, leaf size and tree: LBVH or SAH build) when that is faster at
+identical ids; `config` names the tree and its build time.  The plain
+single-launch FP64 form is timed on the default tree.  The roofline entry is
+for the dominant kernel (the first traversal round) of the configuration
+`value` used: algorithmic bytes counted by an instrumented run, divided by the
+kernel's duration timed with HIP events on the library stream; `traffic` is
 the kernel's HBM bytes per launch from two rocprofv3 --pmc passes (FETCH_SIZE,
-WRITE_SIZE) run as child processes of this run, scaled by what the same counters
-read for a 64 MiB streaming copy kernel in those passes (null when rocprofv3 is absent or
---traffic off).  The CPU baseline is the oracle's multi-threaded KD-tree on this
+WRITE_SIZE) over the same configuration in child processes of this run, scaled
+by what the same counters read for a 64 MiB streaming copy kernel in those
+passes (null when rocprofv3 is absent or --traffic off).  The CPU baseline is the oracle's multi-threaded KD-tree on this
 host's cores.  `details` adds, on the same data: the end-to-end time with the
-build and host transfers, k = 32, the r = 0.5 m range query and the LBVH
-rebuild, each with its own roofline (skipped with --no-extra); --all adds the
+build and host transfers, k = 32, the r = 0.5 m range query, the LBVH
+rebuild and the SAH build, with rooflines (skipped with --no-extra); --all adds the
 road-network, batched multi-cloud and Barnes-Hut workloads.  --only runs one
 secondary workload alone and prints its own line (for profiling).
 """
@@ -89,12 +91,14 @@ def _roof(alg_bytes, ms, **extra):
 
 
 # ---------------------------------------------------------------- HBM traffic
-def _pmc_pass(counter, tmp):
+def _pmc_pass(counter, tmp, head):
     """One rocprofv3 --pmc pass over `bench.py --only knn-round0` in a child
-    process; returns {kernel name: mean counter value (KiB) per dispatch}."""
+    process running the head configuration `head` (leaf size, tree, schedule);
+    returns {kernel name: mean counter value (KiB) per dispatch}."""
     d = os.path.join(tmp, counter.lower())
+    hargs = ["--leaf", str(head["leaf"]), "--schedule", json.dumps(head["schedule"])] + (["--sah"] if head["sah"] else [])
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
-           sys.executable, os.path.abspath(__file__), "--only", "knn-round0", "--traffic", "off"]
+           sys.executable, os.path.abspath(__file__), "--only", "knn-round0", "--traffic", "off"] + hargs
     env = dict(os.environ, TMPDIR=tmp)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=env, cwd=tmp)
     files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -107,19 +111,21 @@ def _pmc_pass(counter, tmp):
                 continue
             acc.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
     shown = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", "<tmp>", "-o", "run", "--", "python3",
-             "bench.py", "--only", "knn-round0", "--traffic", "off"]
+             "bench.py", "--only", "knn-round0", "--traffic", "off", "--leaf", str(head["leaf"])] + \
+        (["--sah"] if head["sah"] else []) + ["--schedule", "'<head schedule>'"]
     return {k: sum(v) / len(v) for k, v in acc.items()}, " ".join(shown)
 
 
-def _traffic():
-    """HBM bytes per launch of the first traversal round from two counter
-    passes, each scaled by the counter's reading for a plain 64 MiB copy."""
+def _traffic(head):
+    """HBM bytes per launch of the first traversal round of the head
+    configuration from two counter passes, each scaled by the counter's reading
+    for a plain 64 MiB copy."""
     if shutil.which("rocprofv3") is None:
         return None, "rocprofv3 not found"
     tmp = tempfile.mkdtemp(prefix="gprt_pmc_")
     try:
-        fetch, c1 = _pmc_pass("FETCH_SIZE", tmp)
-        write, c2 = _pmc_pass("WRITE_SIZE", tmp)
+        fetch, c1 = _pmc_pass("FETCH_SIZE", tmp, head)
+        write, c2 = _pmc_pass("WRITE_SIZE", tmp, head)
         kern = [k for k in fetch if ROUND0 in k]
         copy = [k for k in fetch if "stream_copy_kernel" in k]
         if not kern or not copy:
@@ -209,6 +215,18 @@ def _build_line(ctx, dp, n, leaf, reps=7):
     alg = n * (12 + 24 + 8 * 24 + 48 + 8 * np.log2(n) + 40 + 64 + 192)
     return {"workload": f"LBVH rebuild, {n} device-resident points, leaf {leaf}", "ms": med,
             "roofline": _roof(alg, med, kernel="LBVH build (all kernels)")}
+
+
+def _sah_build_line(ctx, dp, n, leaf, reps=5):
+    """SAH tree build (GPRT_BUILD_SAH) from device-resident points: the LBVH's
+    Morton sort, then the SAH levels, the refit and the 4-wide view.  Build
+    time only (its passes are many small latency-bound launches)."""
+    ms = []
+    for _ in range(reps + 1):
+        t = ctx.build(dp, leaf_size=leaf, sah=True)
+        ms.append(ctx.stats()["ms_build"])
+        t.close()
+    return {"workload": f"SAH tree build, {n} device-resident points, leaf {leaf}", "ms": float(np.median(ms[1:]))}
 
 
 def _bh_line(ctx, pm, oracle, thetas=(0.3, 0.5, 0.7), check=True):
@@ -303,18 +321,20 @@ def _only(args):
     ctx = gprt_hip.Context(0)
     if args.only == "range":
         pts, q = D.lidar_scan()
-        tree = ctx.build(pts, leaf_size=args.leaf)
+        tree = ctx.build(pts, leaf_size=args.leaf, sah=args.sah)
         line = _range_line(ctx, tree, pts, q, oracle, reps=max(2, args.steps))
     elif args.only == "build":
         pts, _ = D.lidar_scan()
         dp = gprt_hip.DeviceArray.from_host(ctx, pts)
-        line = _build_line(ctx, dp, len(pts), args.leaf, reps=max(3, args.steps))
+        line = _sah_build_line(ctx, dp, len(pts), args.leaf, reps=max(3, args.steps)) if args.sah else \
+            _build_line(ctx, dp, len(pts), args.leaf, reps=max(3, args.steps))
     elif args.only == "knn-round0":
         pts, q = D.lidar_scan()
         dp, dq = gprt_hip.DeviceArray.from_host(ctx, pts), gprt_hip.DeviceArray.from_host(ctx, q)
         ids = gprt_hip.DeviceArray(ctx, (len(q), K), np.uint32)
-        tree = ctx.build(dp, leaf_size=args.leaf)
-        cfg = gprt_hip.default_config(use_graph=0)
+        tree = ctx.build(dp, leaf_size=args.leaf, sah=args.sah)
+        cfg = gprt_hip.default_config(**(json.loads(args.schedule) if args.schedule else {}))
+        cfg.use_graph = 0
         for _ in range(max(2, args.steps)):
             tree.k_nn(dq, K, cfg, ids_out=ids)
         a = gprt_hip.DeviceArray(ctx, (COPY_BYTES // 4,), np.uint32)
@@ -341,6 +361,8 @@ def main():
                     help="measure the round kernel's HBM bytes with two rocprofv3 --pmc child passes")
     ap.add_argument("--only", choices=["range", "barnes-hut", "build", "knn-round0"],
                     help="run one workload alone (for profiling) and print its JSON line")
+    ap.add_argument("--sah", action="store_true", help="with --only: build the SAH tree (GPRT_BUILD_SAH)")
+    ap.add_argument("--schedule", default=None, help="with --only knn-round0: the schedule as a JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -348,10 +370,6 @@ def main():
         return 2
     if args.only:
         return _only(args)
-
-    # the counter passes run first, as child processes, before this process
-    # touches the GPU
-    traffic, traffic_note = (None, "--traffic off") if args.traffic == "off" else _traffic()
 
     import gprt_hip
     from gprt_hip import datasets as D
@@ -385,24 +403,36 @@ def main():
     tree.k_nn_plain(dq, K, ids_out=ids_p)
     ms_p_dev = ctx.stats()["ms_query"]
 
-    # the autotuner's pick, timed the same way; adopted when faster at equal ids
+    # the autotuner's pick (schedule, leaf size and tree: LBVH or SAH), timed
+    # the same way; adopted when faster at equal ids
     auto = None
-    head_tree, head_cfg, head_leaf, head_ms = tree, cfg, leaf, ms_b
+    head_tree, head_cfg, head_leaf, head_sah, head_ms, head_build_ms = tree, cfg, leaf, False, ms_b, build_ms
     if not args.no_extra:
         t0 = time.perf_counter()
-        acfg, aleaf = ctx.autotune(dp, dq, K, budget_trials=14)
+        acfg, aleaf, asah = ctx.autotune(dp, dq, K, budget_trials=16, with_tree=True)
         tune_s = time.perf_counter() - t0
-        atree = tree if aleaf == leaf else ctx.build(dp, leaf_size=aleaf)
+        if aleaf == leaf and not asah:
+            atree, abuild_ms = tree, build_ms
+        else:
+            atree = ctx.build(dp, leaf_size=aleaf, sah=asah)
+            abuild_ms = ctx.stats()["ms_build"]
         ms_a = _time_steps(ctx, batched_on(atree, acfg), args.steps, args.warmup)
         atree.k_nn(dq, K, acfg, ids_out=ids_b)
         same = bool(np.array_equal(ids_b.to_host(), got_default))
         adopt = same and ms_a < ms_b * 0.99
-        auto = {"seconds": tune_s, "leaf_size": aleaf, "schedule": acfg.as_dict(), "ms_per_step": ms_a,
+        auto = {"seconds": tune_s, "leaf_size": aleaf, "tree": "sah" if asah else "lbvh", "build_ms": abuild_ms,
+                "schedule": acfg.as_dict(), "ms_per_step": ms_a,
                 "default_ms_per_step": ms_b, "ids_equal_default": same, "adopted": adopt}
         if adopt:
-            head_tree, head_cfg, head_leaf, head_ms = atree, acfg, aleaf, ms_a
+            head_tree, head_cfg, head_leaf, head_sah, head_ms, head_build_ms = atree, acfg, aleaf, asah, ms_a, abuild_ms
         elif atree is not tree:
             atree.close()
+
+    # HBM traffic of the head configuration's round kernel: two counter passes
+    # in child processes (this process idles meanwhile)
+    ctx.sync()
+    traffic, traffic_note = (None, "--traffic off") if args.traffic == "off" else _traffic(
+        {"leaf": head_leaf, "sah": head_sah, "schedule": head_cfg.as_dict()})
 
     # roofline of the dominant kernel (first traversal round): counts from an
     # instrumented run, duration from HIP events around the uninstrumented kernel
@@ -472,6 +502,7 @@ def main():
                               "plain_eq_oracle": bool(np.array_equal(ip, ref32))}
         details["range_r0.5"] = _range_line(ctx, tree, pts, q, oracle)
         details["lbvh_rebuild"] = _build_line(ctx, dp, len(pts), leaf)
+        details["sah_build"] = _sah_build_line(ctx, dp, len(pts), head_leaf)
     if args.all:
         details.update(_extra_all(ctx, args, oracle))
 
@@ -495,7 +526,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "lidar_scan: 1,310,720 pts (20x64x1024) x 400,000 queries, k=5",
-                   "leaf_size": head_leaf, "schedule": head_cfg.as_dict(),
+                   "leaf_size": head_leaf, "tree": "sah" if head_sah else "lbvh",
+                   "tree_build_ms": head_build_ms, "schedule": head_cfg.as_dict(),
                    "schedule_source": "autotuner" if head_cfg is not cfg else "default",
                    "form": "batched FP32 + FP64 re-rank"},
         "plain": {"value": nq / (ms_p * 1e-3), "ms_per_step": ms_p, "ms_device": ms_p_dev,

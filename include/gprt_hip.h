@@ -316,6 +316,14 @@ GPRT_HIP_API int gprt_hip_sort_pairs(gprt_hip_ctx* ctx, uint64_t* keys, uint32_t
 GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t n, const float* queries, int64_t nq,
                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
                                    const char* json_path, int ptr_flags);
+/* As gprt_hip_autotune, also choosing the tree: the LBVH or the SAH build
+ * (the Schedule's build_strategy, gprt-ir/src/lib.rs:46-49, 64), returned as
+ * the build flags to pass to gprt_hip_tree_build (GPRT_BUILD_SAH or 0) and
+ * written to the JSON file as build_strategy PreferFastTrace / PreferFastBuild.
+ * With build_flags_out NULL it is gprt_hip_autotune (LBVH only). */
+GPRT_HIP_API int gprt_hip_autotune_ex(gprt_hip_ctx* ctx, const float* pts, int64_t n, const float* queries,
+                                      int64_t nq, int k, int budget_trials, gprt_hip_config* cfg_out,
+                                      int* leaf_size_out, int* build_flags_out, const char* json_path, int ptr_flags);
 
 /* ---------------- testing hooks ---------------- */
 /* Morton bits (8..30, default 16) by which the batched k-NN orders its queries
@@ -362,6 +370,11 @@ GPRT_HIP_API int gprt_hip_read_points(const char* path, int format, float* out, 
  * geom_type load; a "gprt_hip" object (as gprt_hip_autotune writes) sets the
  * config fields it names and *leaf_size (default 8).  Host only. */
 GPRT_HIP_API int gprt_hip_schedule_from_json(const char* json, size_t size, gprt_hip_config* cfg, int* leaf_size);
+/* As gprt_hip_schedule_from_json, also reading build_strategy into the tree
+ * build flags: PreferFastTrace (the reference's default, gprt-ir/src/lib.rs:85)
+ * gives GPRT_BUILD_SAH, PreferFastBuild 0.  build_flags may be NULL. */
+GPRT_HIP_API int gprt_hip_schedule_from_json_ex(const char* json, size_t size, gprt_hip_config* cfg, int* leaf_size,
+                                                int* build_flags);
 
 /* ---------------- device programs from the shader IR ---------------- */
 /* Lower a program of the reference's shader IR (ShaderNode trees,

@@ -28,6 +28,7 @@ namespace {
 struct Cand {
     gprt_hip_config cfg;
     int leaf;
+    int sah;  // 1: the SAH tree (GPRT_BUILD_SAH), 0: the LBVH
 };
 
 int time_config(gprt_hip_ctx* h, gprt_hip_tree* t, const float* d_q, int64_t nq, int k, const gprt_hip_config& cfg,
@@ -44,7 +45,7 @@ int time_config(gprt_hip_ctx* h, gprt_hip_tree* t, const float* d_q, int64_t nq,
     return GPRT_OK;
 }
 
-std::string schedule_json(const gprt_hip_config& c, int leaf, double ms) {
+std::string schedule_json(const gprt_hip_config& c, int leaf, int sah, double ms) {
     char heur[96];
     if (c.radius_heuristic == GPRT_RADIUS_PERCENTILE) snprintf(heur, sizeof heur, "{\"SampledPercentile\": %.6g}", c.radius_param);
     else if (c.radius_heuristic == GPRT_RADIUS_FIXED) snprintf(heur, sizeof heur, "{\"Fixed\": %.9g}", c.radius_param);
@@ -53,12 +54,13 @@ std::string schedule_json(const gprt_hip_config& c, int leaf, double ms) {
     snprintf(buf, sizeof buf,
              "{\"radius_increment_mult\": %.6g, \"max_hits_per_query\": 2000, \"use_morton_lbv\": %s, "
              "\"radius_heuristic\": %s, \"memory_strategy\": \"PayloadRegisterHeap\", "
-             "\"build_strategy\": \"PreferFastTrace\", \"geom_type\": \"Spheres\", "
+             "\"build_strategy\": \"%s\", \"geom_type\": \"Spheres\", "
              "\"gprt_hip\": {\"sampler\": %d, \"radius_heuristic\": %d, \"radius_param\": %.6g, "
              "\"radius_mult\": %.6g, \"max_rounds\": %d, \"traversal\": %d, \"stack_depth\": %d, "
              "\"guard_slots\": %d, \"sort_queries\": %d, \"use_graph\": %d, \"grid_blocks\": %d, \"lds_stack\": %d, \"compact_rounds\": %d, "
              "\"leaf_size\": %d, \"ms_query\": %.6g}}\n",
-             c.radius_mult, c.sort_queries ? "true" : "false", heur, c.sampler, c.radius_heuristic, c.radius_param,
+             c.radius_mult, c.sort_queries ? "true" : "false", heur, sah ? "PreferFastTrace" : "PreferFastBuild",
+             c.sampler, c.radius_heuristic, c.radius_param,
              c.radius_mult, c.max_rounds, c.traversal, c.stack_depth, c.guard_slots, c.sort_queries, c.use_graph,
              c.grid_blocks, c.lds_stack, c.compact_rounds, leaf, ms);
     return buf;
@@ -66,10 +68,11 @@ std::string schedule_json(const gprt_hip_config& c, int leaf, double ms) {
 
 }  // namespace
 
-extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
-                                 int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
-                                 const char* json_path, int flags) {
+extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
+                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
+                                    int* build_flags_out, const char* json_path, int flags) {
     if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
+    const bool with_sah = build_flags_out != nullptr;
     GPRT_CTX_LOCK(h);
     if (n <= 0 || nq <= 0 || k < 1 || k > 64) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 64");
     Ctx& c = h->c;
@@ -84,9 +87,12 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
     GPRT_TRY(dids.reserve(4ull * nq * k));
     GPRT_TRY(dref.reserve(4ull * nq * k));
     const int leaves[3] = {4, 8, 16};
-    gprt_hip_tree* trees[3] = {nullptr, nullptr, nullptr};
+    // trees[s * 3 + i]: leaf size leaves[i], s = 1 the SAH tree
+    gprt_hip_tree* trees[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int rc = GPRT_OK;
-    for (int i = 0; i < 3 && rc == GPRT_OK; ++i) rc = gprt_hip_tree_build(h, pts, n, leaves[i], flags & GPRT_PTR_DEVICE, &trees[i]);
+    for (int i = 0; i < (with_sah ? 6 : 3) && rc == GPRT_OK; ++i)
+        rc = gprt_hip_tree_build(h, pts, n, leaves[i % 3], (flags & GPRT_PTR_DEVICE) | (i >= 3 ? GPRT_BUILD_SAH : 0),
+                                 &trees[i]);
     // reference answer: default schedule on the leaf-8 tree
     gprt_hip_config base;
     gprt_hip_config_default(&base);
@@ -99,15 +105,15 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
     }
     // coordinate descent from the default, one knob at a time (like the
     // reference's hill climb), bounded by budget_trials timed candidates
-    Cand best{base, 8};
+    Cand best{base, 8, 0};
     double best_ms = 1e30;
     int trials = 0;
-    auto tree_of = [&](int leaf) { return trees[leaf == 4 ? 0 : (leaf == 8 ? 1 : 2)]; };
+    auto tree_of = [&](int leaf, int sah) { return trees[3 * sah + (leaf == 4 ? 0 : (leaf == 8 ? 1 : 2))]; };
     auto try_cand = [&](const Cand& cd) -> int {
         if (trials >= std::max(budget_trials, 1)) return GPRT_OK;
         ++trials;
         double ms = 0;
-        GPRT_TRY(time_config(h, tree_of(cd.leaf), d_q, nq, k, cd.cfg, dids.as<uint32_t>(), &ms));
+        GPRT_TRY(time_config(h, tree_of(cd.leaf, cd.sah), d_q, nq, k, cd.cfg, dids.as<uint32_t>(), &ms));
         got.resize((size_t)nq * k);
         GPRT_HIP_TRY(hipMemcpy(got.data(), dids.ptr, 4ull * nq * k, hipMemcpyDeviceToHost));
         if (got != ref) return fail(GPRT_ERR_STATE, "autotune: a candidate schedule changed the result");
@@ -120,6 +126,12 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
     if (rc == GPRT_OK) rc = try_cand(best);
     for (int pass = 0; pass < 2 && rc == GPRT_OK; ++pass) {
         const Cand start = best;
+        if (with_sah && rc == GPRT_OK) {
+            // the tree: SAH splits (slower build, fewer node visits) or the LBVH
+            Cand cd = best;
+            cd.sah = best.sah ? 0 : 1;
+            rc = try_cand(cd);
+        }
         for (int leaf : leaves) {
             if (leaf == start.leaf || rc != GPRT_OK) continue;
             Cand cd = best;
@@ -169,25 +181,35 @@ extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, c
     if (rc != GPRT_OK) return rc;
     *cfg_out = best.cfg;
     *leaf_size_out = best.leaf;
+    if (build_flags_out) *build_flags_out = best.sah ? GPRT_BUILD_SAH : 0;
     if (json_path) {
         FILE* f = fopen(json_path, "w");
         if (!f) return fail(GPRT_ERR_ARG, std::string("cannot write ") + json_path);
-        const std::string js = schedule_json(best.cfg, best.leaf, best_ms);
+        const std::string js = schedule_json(best.cfg, best.leaf, best.sah, best_ms);
         fputs(js.c_str(), f);
         fclose(f);
     }
     return GPRT_OK;
 }
 
+extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
+                                 int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
+                                 const char* json_path, int flags) {
+    return gprt_hip_autotune_ex(h, pts, n, queries, nq, k, budget_trials, cfg_out, leaf_size_out, nullptr, json_path,
+                                flags);
+}
+
 // Schedule / wisdom JSON (gprt-ir/src/lib.rs:58-89) -> config.  Fields missing
 // from the file take Schedule::default() (lib.rs:76-87): the reference's own
 // *_wisdom.json files lack build_strategy and geom_type and therefore fail its
 // serde reader; here they load.  max_hits_per_query, memory_strategy,
-// build_strategy and geom_type have no counterpart (candidates live in
-// registers, the tree is the LBVH) and are ignored.  An object "gprt_hip"
-// (written by gprt_hip_autotune) sets the config fields it names and the leaf
-// size.
-extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_config* cfg, int* leaf_size) {
+// build_strategy selects the tree (PreferFastTrace, Schedule::default()'s:
+// the SAH build; PreferFastBuild: the LBVH) through *build_flags of the _ex
+// form; geom_type has no counterpart (the points are spheres of radius 0).  An
+// object "gprt_hip" (written by gprt_hip_autotune) sets the config fields it
+// names and the leaf size.
+extern "C" int gprt_hip_schedule_from_json_ex(const char* json, size_t n, gprt_hip_config* cfg, int* leaf_size,
+                                              int* build_flags) {
     if (!json || !cfg) return fail(GPRT_ERR_ARG, "null argument");
     JParser jp(json, n);
     JVal v;
@@ -238,6 +260,12 @@ extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_
                                       "or {\"Fixed\": r}");
         }
     }
+    int bflags = GPRT_BUILD_SAH;  // Schedule::default(): PreferFastTrace
+    if (const JVal* bs = v.get("build_strategy")) {
+        if (bs->kind == JVal::Str && bs->str == "PreferFastTrace") bflags = GPRT_BUILD_SAH;
+        else if (bs->kind == JVal::Str && bs->str == "PreferFastBuild") bflags = 0;
+        else return fail(GPRT_ERR_ARG, "schedule: build_strategy must be \"PreferFastTrace\" or \"PreferFastBuild\"");
+    }
     if (const JVal* g = v.get("gprt_hip")) {
         if (g->kind != JVal::Obj) return fail(GPRT_ERR_ARG, "schedule: gprt_hip must be an object");
         struct {
@@ -265,5 +293,10 @@ extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_
     }
     *cfg = c;
     if (leaf_size) *leaf_size = leaf;
+    if (build_flags) *build_flags = bflags;
     return GPRT_OK;
+}
+
+extern "C" int gprt_hip_schedule_from_json(const char* json, size_t n, gprt_hip_config* cfg, int* leaf_size) {
+    return gprt_hip_schedule_from_json_ex(json, n, cfg, leaf_size, nullptr);
 }

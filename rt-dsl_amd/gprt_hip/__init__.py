@@ -133,10 +133,14 @@ SIGNATURES = [
     ("gprt_hip_sort_pairs", _int, [_vp, _vp, _vp, _i64, _int, _int, _int]),
     ("gprt_hip_autotune", _int, [_vp, _vp, _i64, _vp, _i64, _int, _int, ctypes.POINTER(Config),
                                  ctypes.POINTER(_int), ctypes.c_char_p, _int]),
+    ("gprt_hip_autotune_ex", _int, [_vp, _vp, _i64, _vp, _i64, _int, _int, ctypes.POINTER(Config),
+                                    ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.c_char_p, _int]),
     ("gprt_hip_read_points", _int, [ctypes.c_char_p, _int, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.POINTER(ctypes.c_int64)]),
     ("gprt_hip_schedule_from_json", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Config),
                                            ctypes.POINTER(_int)]),
+    ("gprt_hip_schedule_from_json_ex", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Config),
+                                              ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     ("gprt_hip_codegen", _int, [ctypes.c_char_p, ctypes.c_size_t, _vp, ctypes.c_size_t,
                                 ctypes.POINTER(ctypes.c_size_t)]),
     ("gprt_hip_codegen_compile", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p,
@@ -353,9 +357,11 @@ class Context:
         _check(lib().gprt_hip_bh_forces(self.h, p, n, theta, eps, G, leaf_size, o, flags))
         return out
 
-    def autotune(self, pts, queries, k, budget_trials=24, json_path=None):
+    def autotune(self, pts, queries, k, budget_trials=24, json_path=None, with_tree=False):
         """The project's autotuner (gprt_autotune! of the reference): returns
-        (Config, leaf_size) and optionally writes the Schedule-superset JSON."""
+        (Config, leaf_size) and optionally writes the Schedule-superset JSON;
+        with_tree also chooses between the LBVH and the SAH tree and returns
+        (Config, leaf_size, sah)."""
         pts, queries = _points(pts), _points(queries)
         p, dev = _ptr(pts)
         qp, qdev = _ptr(queries)
@@ -363,9 +369,14 @@ class Context:
             raise ValueError("points and queries must both be host arrays or both device arrays")
         cfg = Config()
         leaf = ctypes.c_int()
-        _check(lib().gprt_hip_autotune(self.h, p, len(pts), qp, len(queries), k, budget_trials, ctypes.byref(cfg),
-                                       ctypes.byref(leaf), json_path.encode() if json_path else None,
-                                       PTR_DEVICE if dev else PTR_HOST))
+        bflags = ctypes.c_int()
+        _check(lib().gprt_hip_autotune_ex(self.h, p, len(pts), qp, len(queries), k, budget_trials,
+                                          ctypes.byref(cfg), ctypes.byref(leaf),
+                                          ctypes.byref(bflags) if with_tree else None,
+                                          json_path.encode() if json_path else None,
+                                          PTR_DEVICE if dev else PTR_HOST))
+        if with_tree:
+            return cfg, leaf.value, bool(bflags.value & BUILD_SAH)
         return cfg, leaf.value
 
 
@@ -504,19 +515,24 @@ def read_points(path, fmt=CSV_POINTS):
     return out[:rows.value].copy()
 
 
-def parse_schedule(text):
-    """(Config, leaf_size) from the text of a Schedule / wisdom JSON object."""
+def parse_schedule(text, with_tree=False):
+    """(Config, leaf_size) from the text of a Schedule / wisdom JSON object;
+    with_tree adds sah (build_strategy PreferFastTrace, the default)."""
     raw = text.encode() if isinstance(text, str) else bytes(text)
     cfg = Config()
     leaf = _int()
-    _check(lib().gprt_hip_schedule_from_json(raw, len(raw), ctypes.byref(cfg), ctypes.byref(leaf)))
+    bflags = _int()
+    _check(lib().gprt_hip_schedule_from_json_ex(raw, len(raw), ctypes.byref(cfg), ctypes.byref(leaf),
+                                                ctypes.byref(bflags)))
+    if with_tree:
+        return cfg, leaf.value, bool(bflags.value & BUILD_SAH)
     return cfg, leaf.value
 
 
-def load_schedule(path):
-    """(Config, leaf_size) from a Schedule / wisdom JSON file."""
+def load_schedule(path, with_tree=False):
+    """(Config, leaf_size[, sah]) from a Schedule / wisdom JSON file."""
     with open(path, "rb") as f:
-        return parse_schedule(f.read())
+        return parse_schedule(f.read(), with_tree)
 
 
 def codegen(program):

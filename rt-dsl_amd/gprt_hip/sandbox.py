@@ -70,18 +70,22 @@ def parse_args(argv):
 
 
 def choose_schedule(ctx, data, queries, k, percentile, mult, tune, out=print):
-    """(Config, leaf_size) per main.rs:50-76."""
+    """(Config, leaf_size, sah) per main.rs:50-76.  The tree follows the
+    schedule's build_strategy (PreferFastTrace: the SAH build): the autotuner
+    chooses it, a loaded file names it (Schedule::default() when absent); the
+    command-line defaults build the LBVH, whose build is the faster one inside
+    the timed region."""
     if tune:
         out("[TUNE] Starting Native Full-Dataset Autotuner...")
-        cfg, leaf = ctx.autotune(data, queries, k, json_path=WISDOM)
+        cfg, leaf, sah = ctx.autotune(data, queries, k, json_path=WISDOM, with_tree=True)
         out(f"[TUNE] Saved optimal schedule to {WISDOM}")
-        return cfg, leaf
+        return cfg, leaf, sah
     cfg, leaf = gprt_hip.parse_schedule("{}")  # Schedule::default()
     if os.path.exists(WISDOM):
         try:
-            cfg, leaf = gprt_hip.load_schedule(WISDOM)
+            cfg, leaf, sah = gprt_hip.load_schedule(WISDOM, with_tree=True)
             out(f"[LOAD] Loaded auto-tuned schedule from {WISDOM}")
-            return cfg, leaf
+            return cfg, leaf, sah
         except (gprt_hip.GprtError, OSError, UnicodeDecodeError):
             pass
     else:
@@ -89,7 +93,7 @@ def choose_schedule(ctx, data, queries, k, percentile, mult, tune, out=print):
     cfg.radius_heuristic = gprt_hip.RADIUS_PERCENTILE
     cfg.radius_param = percentile
     cfg.radius_mult = mult
-    return cfg, leaf
+    return cfg, leaf, False
 
 
 def format_results(ids):
@@ -109,9 +113,9 @@ def run(argv, out=print):
     out(f"[IO] Loaded {len(data)} points and {len(queries)} queries in {(time.perf_counter() - t0) * 1e3:.2f}ms")
     ctx = gprt_hip.Context(0)
     try:
-        cfg, leaf = choose_schedule(ctx, data, queries, k, percentile, mult, tune, out)
+        cfg, leaf, sah = choose_schedule(ctx, data, queries, k, percentile, mult, tune, out)
         t1 = time.perf_counter()
-        tree = ctx.build(data, leaf_size=leaf)
+        tree = ctx.build(data, leaf_size=leaf, sah=sah)
         ids = tree.k_nn(queries, k, cfg)
         total_ms = (time.perf_counter() - t1) * 1e3
         st = ctx.stats()

@@ -526,3 +526,23 @@ def test_autotune_writes_schedule(ctx, tmp_path):
     assert leaf2 == leaf
     for f, _ in gprt_hip.Config._fields_:
         assert getattr(cfg2, f) == pytest.approx(getattr(cfg, f), rel=1e-5), f
+    assert js["build_strategy"] == "PreferFastBuild"  # the LBVH-only form
+
+
+def test_autotune_chooses_tree(ctx, tmp_path):
+    """The autotuner with the tree knob: its pick (LBVH or SAH build) gives the
+    oracle's ids, and the file's build_strategy reads back to the same tree."""
+    rng = np.random.default_rng(21)
+    pts = (rng.random((60_000, 3)) * [50.0, 50.0, 0.0]).astype(np.float32)  # a noisy plane
+    pts[:, 2] = rng.normal(size=60_000).astype(np.float32) * 0.02
+    q = pts[rng.integers(0, len(pts), 20_000)] + rng.normal(size=(20_000, 3)).astype(np.float32) * 0.01
+    path = str(tmp_path / "gprt_wisdom.json")
+    cfg, leaf, sah = ctx.autotune(pts, q, 5, budget_trials=10, json_path=path, with_tree=True)
+    js = json.load(open(path))
+    assert js["build_strategy"] == ("PreferFastTrace" if sah else "PreferFastBuild")
+    assert gprt_hip.load_schedule(path, with_tree=True)[1:] == (leaf, sah)
+    t = ctx.build(pts, leaf_size=leaf, sah=sah)
+    try:
+        assert np.array_equal(t.k_nn(q, 5, cfg), oracle.knn_brute(pts, q, 5))
+    finally:
+        t.close()

@@ -127,22 +127,24 @@ def test_parse_args():
 def test_choose_schedule(tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     lines = []
-    cfg, leaf = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
+    cfg, leaf, sah = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
     assert lines == ["[LOAD] Using CLI defaults: P=0.25, M=2.0"]
     assert cfg.radius_param == 0.25 and cfg.radius_mult == 2.0 and cfg.sort_queries == 0 and leaf == 8
+    assert sah is False
     # a reference wisdom file (no build_strategy / geom_type) loads
     (tmp_path / "gprt_wisdom.json").write_text('{"radius_increment_mult": 3.0, "max_hits_per_query": 2000, '
                                                '"use_morton_lbv": true, "radius_heuristic": '
                                                '{"SampledPercentile": 0.5}, "memory_strategy": '
                                                '"PayloadRegisterHeap"}')
     lines.clear()
-    cfg, leaf = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
+    cfg, leaf, sah = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
     assert lines == ["[LOAD] Loaded auto-tuned schedule from gprt_wisdom.json"]
     assert cfg.radius_param == 0.5 and cfg.sort_queries == 1
+    assert sah is True  # build_strategy absent: Schedule::default(), PreferFastTrace
     # an unreadable file falls back to the command line without a message
     (tmp_path / "gprt_wisdom.json").write_text("not json")
     lines.clear()
-    cfg, _ = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
+    cfg, _, _ = sandbox.choose_schedule(None, None, None, 5, 0.25, 2.0, False, lines.append)
     assert lines == [] and cfg.radius_param == 0.25 and cfg.radius_mult == 2.0
 
 

@@ -43,11 +43,22 @@ def test_reference_wisdom_files(name, p, mult):
 
 def test_schedule_defaults():
     """An empty object is Schedule::default(): SampledPercentile(0.10), mult 3.0,
-    use_morton_lbv false."""
+    use_morton_lbv false, build_strategy PreferFastTrace (the SAH build)."""
     cfg, leaf = gprt_hip.parse_schedule("{}")
     assert cfg.radius_heuristic == gprt_hip.RADIUS_PERCENTILE
     assert cfg.radius_param == pytest.approx(0.10)
     assert cfg.radius_mult == 3.0 and cfg.sort_queries == 0 and leaf == 8
+    assert gprt_hip.parse_schedule("{}", with_tree=True)[2] is True
+
+
+def test_schedule_build_strategy():
+    """build_strategy picks the tree: PreferFastTrace the SAH build,
+    PreferFastBuild the LBVH; anything else is rejected."""
+    assert gprt_hip.parse_schedule('{"build_strategy": "PreferFastTrace"}', with_tree=True)[2] is True
+    assert gprt_hip.parse_schedule('{"build_strategy": "PreferFastBuild"}', with_tree=True)[2] is False
+    for bad in ('{"build_strategy": "Fast"}', '{"build_strategy": 1}'):
+        with pytest.raises(gprt_hip.GprtError):
+            gprt_hip.parse_schedule(bad)
 
 
 def test_schedule_heuristics():
@@ -63,8 +74,9 @@ def test_schedule_full_reference_form():
                        "max_hits_per_query": 2000, "use_morton_lbv": True,
                        "memory_strategy": "PayloadRegisterHeap", "build_strategy": "PreferFastBuild",
                        "geom_type": "Triangles"})
-    cfg, leaf = gprt_hip.parse_schedule(text)
+    cfg, leaf, sah = gprt_hip.parse_schedule(text, with_tree=True)
     assert (cfg.radius_param, cfg.radius_mult, cfg.sort_queries, leaf) == (pytest.approx(0.8), 5.0, 1, 8)
+    assert sah is False
 
 
 def test_schedule_extension_object_round_trip():
