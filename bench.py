@@ -361,7 +361,8 @@ def main():
                     help="measure the round kernel's HBM bytes with two rocprofv3 --pmc child passes")
     ap.add_argument("--only", choices=["range", "barnes-hut", "build", "knn-round0"],
                     help="run one workload alone (for profiling) and print its JSON line")
-    ap.add_argument("--sah", action="store_true", help="with --only: build the SAH tree (GPRT_BUILD_SAH)")
+    ap.add_argument("--sah", action="store_true",
+                    help="build the SAH tree (GPRT_BUILD_SAH) as the default tree (with --only: its tree)")
     ap.add_argument("--schedule", default=None, help="with --only knn-round0: the schedule as a JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -388,7 +389,7 @@ def main():
 
     leaf = args.leaf
     cfg = gprt_hip.default_config()
-    tree = ctx.build(dp, leaf_size=leaf)
+    tree = ctx.build(dp, leaf_size=leaf, sah=args.sah)
     build_ms = ctx.stats()["ms_build"]
 
     def batched_on(t, c):
@@ -406,12 +407,12 @@ def main():
     # the autotuner's pick (schedule, leaf size and tree: LBVH or SAH), timed
     # the same way; adopted when faster at equal ids
     auto = None
-    head_tree, head_cfg, head_leaf, head_sah, head_ms, head_build_ms = tree, cfg, leaf, False, ms_b, build_ms
+    head_tree, head_cfg, head_leaf, head_sah, head_ms, head_build_ms = tree, cfg, leaf, args.sah, ms_b, build_ms
     if not args.no_extra:
         t0 = time.perf_counter()
         acfg, aleaf, asah = ctx.autotune(dp, dq, K, budget_trials=16, with_tree=True)
         tune_s = time.perf_counter() - t0
-        if aleaf == leaf and not asah:
+        if aleaf == leaf and asah == args.sah:
             atree, abuild_ms = tree, build_ms
         else:
             atree = ctx.build(dp, leaf_size=aleaf, sah=asah)

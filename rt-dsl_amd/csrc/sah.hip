@@ -29,7 +29,10 @@
 
 namespace gprt {
 
-constexpr int SAH_SMALL = 1024;  // largest segment the wavefront phase takes
+#ifndef GPRT_SAH_SMALL
+#define GPRT_SAH_SMALL 1024
+#endif
+constexpr int SAH_SMALL = GPRT_SAH_SMALL;  // largest segment the wavefront phase takes
 constexpr int SAH_CH = 2048;     // points per chunk of the large-segment passes
 constexpr int SAH_CT = 256;      // threads of a chunk workgroup (8 points each)
 constexpr int SAH_STACK = 24;    // segment stack of a wavefront (log2(1024) + 1 used at most)

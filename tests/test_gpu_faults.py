@@ -62,7 +62,15 @@ def _radius(ctx):
     return np.float32(ctx.sample_radius(PTS, Q))
 
 
-OPS = [_knn, _knn_nograph, _plain, _large_k, _batched, _range, _bh, _radius]
+def _sah(ctx):
+    return ctx.build(PTS, sah=True).k_nn(Q, 5)
+
+
+def _sah_batched(ctx):
+    return ctx.build_batched(PTS, POFF, sah=True).k_nn_batched(Q, QOFF, np.array([3, 50], np.int32))[0]
+
+
+OPS = [_knn, _knn_nograph, _plain, _large_k, _batched, _range, _bh, _radius, _sah, _sah_batched]
 
 
 @pytest.mark.parametrize("op", OPS, ids=[f.__name__ for f in OPS])
