@@ -1189,14 +1189,34 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
     if (sampler != GPRT_SAMPLER_LCG && sampler != GPRT_SAMPLER_STRIDED) return fail(GPRT_ERR_ARG, "bad sampler");
     if (heuristic < 0 || heuristic > 3) return fail(GPRT_ERR_ARG, "bad heuristic");
     if (sampler == GPRT_SAMPLER_STRIDED && k < 1) return fail(GPRT_ERR_ARG, "k must be positive");
-    // the sampler reads points through a tree (sorted points + inverse permutation)
-    gprt_hip_tree* t = nullptr;
-    GPRT_TRY(gprt_hip_tree_build(h, pts, n, 8, flags, &t));
+    if (n > (int64_t)0x7FFFFFF0) return fail(GPRT_ERR_ARG, "too many points");
+    // the samplers draw input indices and read the points in input order as
+    // float4 (one 16-byte load per draw); no tree is needed
     Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
     hipStream_t s = c.stream;
     CtxExtra* x = extra(&c);
-    int rc = GPRT_OK;
-    const float* d_q = queries;
+    const float* d_pts = pts;
+    if (!(flags & GPRT_PTR_DEVICE) && n > 0) {
+        GPRT_TRY(x->q_in.reserve(sizeof(float) * 3 * n));
+        GPRT_HIP_TRY(hipMemcpyAsync(x->q_in.ptr, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        d_pts = x->q_in.as<float>();
+    }
+    GPRT_TRY(x->flag.reserve(16));
+    GPRT_HIP_TRY(hipMemsetAsync(x->flag.ptr, 0, 16, s));
+    launch_validate(s, d_pts, 3 * n, x->flag.as<unsigned>());
+    unsigned bad = 0;
+    GPRT_HIP_TRY(hipMemcpyAsync(&bad, x->flag.ptr, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return fail(GPRT_ERR_INPUT, "points contain NaN/Inf or |coordinate| > 1e18");
+    // orig4 (16 B per point) then the one cloud's offsets {0, n}
+    GPRT_TRY(c.sort_tmp[0].reserve(16ull * (size_t)std::max<int64_t>(n, 1) + 64));
+    float4* orig4 = c.sort_tmp[0].as<float4>();
+    int64_t* poff = (int64_t*)(c.sort_tmp[0].as<char>() + 16ull * (size_t)std::max<int64_t>(n, 1));
+    const int64_t po[2] = {0, n};
+    GPRT_HIP_TRY(hipMemcpyAsync(poff, po, 16, hipMemcpyHostToDevice, s));
+    GPRT_TRY(launch_input_points(s, d_pts, orig4, (int)n));
+    int rc = GPRT_OK;    const float* d_q = queries;
     DevBuf qtmp;
     if (!(flags & GPRT_PTR_DEVICE) && nq > 0) {
         rc = qtmp.reserve(12ull * nq);
@@ -1212,8 +1232,8 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
         float* rad = kth + 100;
         (void)hipMemcpyAsync(x->qoff.ptr, qo, 16, hipMemcpyHostToDevice, s);
         if (n > 0 && nq > 0)
-            rc = launch_sampler(s, sampler, t->t.orig4.as<float4>(), t->t.cloud_off.as<int64_t>(), d_q,
-                                x->qoff.as<int64_t>(), 1, k, heuristic, param, kth, rad);
+            rc = launch_sampler(s, sampler, orig4, poff, d_q, x->qoff.as<int64_t>(), 1, k, heuristic, param, kth,
+                                rad);
         else
             rc = launch_sampler(s, sampler, nullptr, nullptr, nullptr, nullptr, 1, k, GPRT_RADIUS_INFINITE,
                                 param, kth, rad);
@@ -1232,7 +1252,6 @@ extern "C" int gprt_hip_sample_radius(gprt_hip_ctx* h, const float* pts, int64_t
     }
     (void)hipStreamSynchronize(s);
     qtmp.release();
-    gprt_hip_tree_destroy(t);
     return rc;
 }
 

@@ -232,6 +232,8 @@ __device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
 int tree_build(Ctx* ctx, const float* d_pts, const int64_t* h_cloud_off, int nclouds, int leaf_size, Tree* t,
                int axis_bits = 0);
 int morton_codes_dev(Ctx* ctx, const float* d_pts, int n, uint32_t* d_codes, float* d_bbox6);
+// the points in input order as float4 (x, y, z, 0), as the radius samplers read them
+int launch_input_points(hipStream_t s, const float* pts, float4* out, int n);
 // the SAH hierarchy over the Morton-sorted points of t (sah.hip): reorders
 // pts4 / perm / codes inside each cloud, writes node references and counts,
 // parent links, ranges and roots; the refit fills the boxes afterwards

@@ -315,6 +315,11 @@ __global__ void input_points_kernel(const float* __restrict__ pts, float4* __res
     if (i < n) out[i] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], 0.0f);
 }
 
+int launch_input_points(hipStream_t s, const float* pts, float4* out, int n) {
+    if (n > 0) hipLaunchKernelGGL(input_points_kernel, dim3((n + 255) / 256), dim3(256), 0, s, pts, out, n);
+    return hipGetLastError() == hipSuccess ? GPRT_OK : fail(GPRT_ERR_HIP, "input points launch failed");
+}
+
 __global__ void fill_int_kernel(int* p, int n, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

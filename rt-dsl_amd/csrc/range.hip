@@ -256,13 +256,15 @@ __global__ void set_last_kernel(int64_t* out, int n, const int64_t* total) { out
 // A classifying pass lists the segments by length (wave-aggregated appends):
 //   * up to 64 hits: one wavefront, a bitonic network in registers;
 //   * up to SEG_MID (2,048) / SEG_LARGE (8,192): one workgroup of 256 / 1,024
-//     lanes, a bucket sort in LDS.  With B = ceil(m/2) buckets, bucket b holds
-//     the elements with floor(d64 * B / r^2) = b, a monotone function of d64,
-//     so buckets come out in d64 order and only each bucket's few elements
-//     (two on average) need ordering by (d64, id), done by insertion; a bucket
-//     of more than SEG_RUN elements (many equal distances: lattices,
-//     duplicates) switches the segment to a bitonic network over the whole
-//     segment;
+//     lanes, a bucket sort.  With B = ceil(m/2) buckets, bucket b holds the
+//     elements with floor(d64 * B / r^2) = b, a monotone function of d64, so
+//     buckets come out in d64 order and only each bucket's few elements (two
+//     on average) need ordering by (d64, id).  Each lane keeps its elements'
+//     (d64, id) in registers, scatters them into bucket order in LDS, counts
+//     the smaller keys of its bucket (keys are distinct: ids differ) and
+//     writes each id straight to its output slot; a bucket of more than
+//     SEG_RUN elements (many equal distances: lattices, duplicates) switches
+//     the segment to a bitonic network over the whole segment;
 //   * longer: SEG_LARGE-element chunks sorted the same way, then merged
 //     pairwise in global memory by rank.
 // ---------------------------------------------------------------------------
@@ -272,8 +274,7 @@ __device__ __forceinline__ bool pair_gt(double da, uint32_t ia, double db, uint3
 
 constexpr int SEG_MID = 2048;
 constexpr int SEG_LARGE = 8192;
-constexpr int SEG_RUN = 32;       // longest bucket ordered by insertion
-constexpr uint16_t SEG_NONE = 0xFFFF;
+constexpr int SEG_RUN = 32;       // longest bucket ordered by rank counting
 
 struct SegQ {
     float x, y, z;
@@ -396,83 +397,9 @@ __device__ __forceinline__ void block_exscan(uint32_t* a, int n, uint32_t* wsum)
     }
 }
 
-template <int CAP, int T>
-struct SegLds {
-    double d[CAP];
-    uint32_t id[CAP];
-    uint32_t cnt[CAP / 2 + 1];
-    uint16_t perm[CAP];
-    uint32_t wsum[T / 64];
-    int flag;
-};
-
-template <class L>
-__device__ __forceinline__ bool perm_gt(const L& S, uint16_t a, uint16_t b) {
-    if (a == SEG_NONE) return b != SEG_NONE;
-    if (b == SEG_NONE) return false;
-    return pair_gt(S.d[a], S.id[a], S.d[b], S.id[b]);
-}
-
 __device__ __forceinline__ int seg_bucket(double d, double scale, int nb) {
     const double f = d * scale;
     return f < (double)(nb - 1) ? (int)f : nb - 1;
-}
-
-// Sorts S.d/S.id[0..m) (m <= CAP, every d <= r2) by (d64, id): on return
-// S.perm[0..m) lists the element indices in order.  Called by the whole block,
-// which must be synchronised on entry.
-template <int CAP, int T>
-__device__ void block_bucket_sort(SegLds<CAP, T>& S, int m, double r2) {
-    const int t = threadIdx.x;
-    const int nb = (m + 1) / 2;
-    const double scale = r2 > 0.0 ? (double)nb / r2 : 0.0;
-    for (int j = t; j <= nb; j += T) S.cnt[j] = 0;
-    if (t == 0) S.flag = 0;
-    __syncthreads();
-    for (int j = t; j < m; j += T) atomicAdd(&S.cnt[seg_bucket(S.d[j], scale, nb)], 1u);
-    __syncthreads();
-    block_exscan<T>(S.cnt, nb, S.wsum);
-    __syncthreads();
-    for (int j = t; j < m; j += T) S.perm[atomicAdd(&S.cnt[seg_bucket(S.d[j], scale, nb)], 1u)] = (uint16_t)j;
-    __syncthreads();
-    // cnt[b] is now the end of bucket b (the start of bucket b + 1)
-    for (int b = t; b < nb; b += T) {
-        const int st = b ? (int)S.cnt[b - 1] : 0, en = (int)S.cnt[b];
-        if (en - st > SEG_RUN) {
-            S.flag = 1;
-        } else {
-            for (int x = st + 1; x < en; ++x) {
-                const uint16_t v = S.perm[x];
-                int y = x - 1;
-                while (y >= st && perm_gt(S, S.perm[y], v)) {
-                    S.perm[y + 1] = S.perm[y];
-                    --y;
-                }
-                S.perm[y + 1] = v;
-            }
-        }
-    }
-    __syncthreads();
-    if (S.flag) {
-        // many equal distances: a bitonic network over the whole segment
-        int P = 1;
-        while (P < m) P <<= 1;
-        for (int j = t; j < P; j += T) S.perm[j] = j < m ? (uint16_t)j : SEG_NONE;
-        __syncthreads();
-        for (int size = 2; size <= P; size <<= 1)
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int u = t; u < P / 2; u += T) {
-                    const int j = 2 * u - (u & (stride - 1)), p = j + stride;
-                    const bool up = (j & size) == 0;
-                    const uint16_t a = S.perm[j], c = S.perm[p];
-                    if (perm_gt(S, a, c) == up) {
-                        S.perm[j] = c;
-                        S.perm[p] = a;
-                    }
-                }
-                __syncthreads();
-            }
-    }
 }
 
 // Expands a range's slots[0..u) into element positions in walk order, calling
@@ -521,72 +448,147 @@ __device__ int expand_slots(const uint32_t* __restrict__ slots, int u, int limit
     return base;
 }
 
-// Loads a range of m hits (u slots) into S.d / S.id: positions expanded into
-// LDS (over S.d), then each lane's points gathered and their d64 and ids
-// stored.  Positions are clamped to the point count, so ranges that do not
-// match the walk (reported by the fill walk) stay in bounds.
 template <int CAP, int T>
-__device__ __forceinline__ void seg_load(SegLds<CAP, T>& S, const uint32_t* slots, int u, int m, const SegQ& q,
-                                         const float4* __restrict__ pts, uint32_t npts) {
-    static_assert(CAP / 2 + 1 >= 2 * T, "expansion scratch lives in S.cnt");
-    static_assert(CAP % T == 0, "elements per lane");
+struct SegRankLds {
+    double d[CAP];     // the expanded positions (as uint32), then the keys in bucket order
+    uint32_t id[CAP];
+    uint32_t cnt[CAP / 2 + 1];  // bucket counts, then bucket ends; first the expansion scratch
+    uint32_t wsum[T / 64];
+    int flag;
+};
+
+// Sorts the m <= CAP keys (dv, iv)[i] of lane t (element t + i*T) by (d64, id)
+// and calls out(j, d64, id) once for the element of sorted position j.  All
+// d64 <= r2.  Bucket counts, an exclusive scan, the keys scattered into bucket
+// order in LDS, then each element's rank among the smaller keys of its bucket;
+// a bucket of more than SEG_RUN elements switches to a bitonic network over the
+// bucket-ordered keys.  Called by the whole block, which must be synchronised
+// on entry (S.cnt and S.d / S.id free); ends synchronised.
+template <int CAP, int T, class Out>
+__device__ void rank_sort(SegRankLds<CAP, T>& S, int m, double r2, const double* dv, const uint32_t* iv, Out out) {
     constexpr int PER = CAP / T;
-    uint32_t* P = reinterpret_cast<uint32_t*>(S.d);
-    const int n = expand_slots<T>(slots, u, m, S.cnt, S.cnt + T, S.wsum, [&](int e, uint32_t p) { P[e] = p; });
-    uint32_t pos[PER];
+    const int t = threadIdx.x;
+    const int nb = (m + 1) / 2;
+    const double scale = r2 > 0.0 ? (double)nb / r2 : 0.0;
+    for (int j = t; j <= nb; j += T) S.cnt[j] = 0;
+    if (t == 0) S.flag = 0;
+    int bk[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * T;
-        pos[i] = e < n ? min(P[e], npts - 1) : 0u;
-    }
+    for (int i = 0; i < PER; ++i) bk[i] = t + i * T < m ? seg_bucket(dv[i], scale, nb) : 0;
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * T;
-        if (e < m) {
-            const float4 p = pts[pos[i]];
-            S.d[e] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
-            S.id[e] = (uint32_t)__float_as_int(p.w);
+    for (int i = 0; i < PER; ++i)
+        if (t + i * T < m) atomicAdd(&S.cnt[bk[i]], 1u);
+    __syncthreads();
+    block_exscan<T>(S.cnt, nb, S.wsum);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + i * T < m) {
+            const uint32_t slot = atomicAdd(&S.cnt[bk[i]], 1u);
+            S.d[slot] = dv[i];
+            S.id[slot] = iv[i];
         }
+    __syncthreads();
+    // S.cnt[k] is now the end of bucket k (the start of bucket k + 1)
+    bool big = false;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + i * T < m) {
+            const int st = bk[i] ? (int)S.cnt[bk[i] - 1] : 0, en = (int)S.cnt[bk[i]];
+            big |= en - st > SEG_RUN;
+        }
+    if (big) S.flag = 1;
+    __syncthreads();
+    if (!S.flag) {
+        // keys are distinct (ids differ), so the ranks place every element once
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            if (t + i * T < m) {
+                const int st = bk[i] ? (int)S.cnt[bk[i] - 1] : 0, en = (int)S.cnt[bk[i]];
+                int r = 0;
+                for (int y = st; y < en; ++y) r += pair_gt(dv[i], iv[i], S.d[y], S.id[y]);
+                out(st + r, dv[i], iv[i]);
+            }
+    } else {
+        // many equal distances: a bitonic network over all the keys
+        int Pn = 1;
+        while (Pn < m) Pn <<= 1;
+        for (int j = m + t; j < Pn; j += T) {
+            S.d[j] = INFINITY;
+            S.id[j] = 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (int size = 2; size <= Pn; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int u = t; u < Pn / 2; u += T) {
+                    const int j = 2 * u - (u & (stride - 1)), pj = j + stride;
+                    const bool up = (j & size) == 0;
+                    const double da = S.d[j], dc = S.d[pj];
+                    const uint32_t ia = S.id[j], ic = S.id[pj];
+                    if (pair_gt(da, ia, dc, ic) == up) {
+                        S.d[j] = dc;
+                        S.id[j] = ic;
+                        S.d[pj] = da;
+                        S.id[pj] = ia;
+                    }
+                }
+                __syncthreads();
+            }
+        for (int j = t; j < m; j += T) out(j, S.d[j], S.id[j]);
     }
-}
-
-// gathers positions pos[0..m) of the segment's query into S.d / S.id
-template <int CAP, int T>
-__device__ __forceinline__ void seg_gather(SegLds<CAP, T>& S, const uint32_t* pos, int m, const SegQ& q,
-                                           const float4* __restrict__ pts) {
-    for (int j = threadIdx.x; j < m; j += T) {
-        const float4 p = pts[pos[j]];
-        S.d[j] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
-        S.id[j] = (uint32_t)__float_as_int(p.w);
-    }
+    __syncthreads();
 }
 
 // one workgroup per listed segment of up to CAP hits (a persistent grid
-// striding over the list)
+// striding over the list): positions expanded, points gathered into
+// registers, bucket counts, scatter of the keys into bucket order, then each
+// element's rank inside its bucket gives its output slot
 template <int CAP, int T>
-__global__ __launch_bounds__(T) void segsort_block_kernel(const int64_t* __restrict__ off,
-                                                          const int* __restrict__ list,
-                                                          const unsigned* __restrict__ count,
-                                                          const uint32_t* __restrict__ used,
-                                                          const float* __restrict__ qxyz,
-                                                          const float* __restrict__ radii, float r_all,
-                                                          const float4* __restrict__ pts, uint32_t npts,
-                                                          uint32_t* ids, int64_t capacity) {
-    __shared__ SegLds<CAP, T> S;
+__global__ __launch_bounds__(T) void segsort_rank_kernel(const int64_t* __restrict__ off,
+                                                         const int* __restrict__ list,
+                                                         const unsigned* __restrict__ count,
+                                                         const uint32_t* __restrict__ used,
+                                                         const float* __restrict__ qxyz,
+                                                         const float* __restrict__ radii, float r_all,
+                                                         const float4* __restrict__ pts, uint32_t npts,
+                                                         uint32_t* ids, int64_t capacity) {
+    static_assert(CAP / 2 + 1 >= 2 * T, "expansion scratch lives in S.cnt");
+    static_assert(CAP % T == 0, "elements per lane");
+    constexpr int PER = CAP / T;
+    __shared__ SegRankLds<CAP, T> S;
+    const int t = threadIdx.x;
     const unsigned n = *count;
+    uint32_t* P = reinterpret_cast<uint32_t*>(S.d);
     for (unsigned li = blockIdx.x; li < n; li += gridDim.x) {
         const int g = list[li];
         int64_t b, e;
         seg_bounds(off, g, capacity, b, e);
         const int m = (int)(e - b);
         const SegQ q = seg_query(qxyz, radii, r_all, g);
-        seg_load(S, ids + b, (int)min(used[g], (uint32_t)m), m, q, pts, npts);
-        __syncthreads();
-        block_bucket_sort(S, m, q.r2);
-        __syncthreads();
-        for (int j = threadIdx.x; j < m; j += T) ids[b + j] = S.id[S.perm[j]];
-        __syncthreads();
+        // positions of the segment's hits (ends synchronised)
+        const int ne = expand_slots<T>(ids + b, (int)min(used[g], (uint32_t)m), m, S.cnt, S.cnt + T, S.wsum,
+                                       [&](int x, uint32_t p) { P[x] = p; });
+        uint32_t pos[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int x = t + i * T;
+            pos[i] = x < ne ? min(P[x], npts - 1) : 0u;
+        }
+        __syncthreads();  // P (over S.d) read
+        double dv[PER];
+        uint32_t iv[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            dv[i] = INFINITY;
+            iv[i] = 0xFFFFFFFFu;
+            if (t + i * T < m) {
+                const float4 p = pts[pos[i]];
+                dv[i] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+                iv[i] = (uint32_t)__float_as_int(p.w);
+            }
+        }
+        rank_sort<CAP, T>(S, m, q.r2, dv, iv, [&](int j, double, uint32_t id) { ids[b + j] = id; });
     }
 }
 
@@ -619,7 +621,7 @@ __global__ __launch_bounds__(SEG_HUGE_T) void segsort_huge_kernel(const int64_t*
                                                                   const float4* __restrict__ pts, uint32_t npts,
                                                                   uint32_t* ids, int64_t capacity, double* tmp_d,
                                                                   uint32_t* tmp_i, int64_t tmp_half) {
-    __shared__ SegLds<SEG_LARGE, SEG_HUGE_T> S;
+    __shared__ SegRankLds<SEG_LARGE, SEG_HUGE_T> S;
     if (blockIdx.x >= *count) return;
     const int g = list[blockIdx.x];
     int64_t b, e;
@@ -635,17 +637,27 @@ __global__ __launch_bounds__(SEG_HUGE_T) void segsort_huge_kernel(const int64_t*
                                               [&](int e, uint32_t p) { ai[e] = min(p, npts - 1); });
     for (int e = nexp + threadIdx.x; e < m; e += SEG_HUGE_T) ai[e] = 0u;
     __syncthreads();
+    constexpr int PER = SEG_LARGE / SEG_HUGE_T;
     for (int c0 = 0; c0 < m; c0 += SEG_LARGE) {
         const int cm = min(SEG_LARGE, m - c0);
-        seg_gather(S, ai + c0, cm, q, pts);
-        __syncthreads();
-        block_bucket_sort(S, cm, q.r2);
-        __syncthreads();
-        for (int j = threadIdx.x; j < cm; j += SEG_HUGE_T) {
-            ad[c0 + j] = S.d[S.perm[j]];
-            ai[c0 + j] = S.id[S.perm[j]];
+        double dv[PER];
+        uint32_t iv[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int x = (int)threadIdx.x + i * SEG_HUGE_T;
+            dv[i] = INFINITY;
+            iv[i] = 0xFFFFFFFFu;
+            if (x < cm) {
+                const float4 p = pts[ai[c0 + x]];
+                dv[i] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+                iv[i] = (uint32_t)__float_as_int(p.w);
+            }
         }
-        __syncthreads();
+        __syncthreads();  // the chunk's positions are read before its sorted keys replace them
+        rank_sort<SEG_LARGE, SEG_HUGE_T>(S, cm, q.r2, dv, iv, [&](int j, double d, uint32_t id) {
+            ad[c0 + j] = d;
+            ai[c0 + j] = id;
+        });
     }
     for (int w = SEG_LARGE; w < m; w <<= 1) {
         for (int j = threadIdx.x; j < m; j += SEG_HUGE_T) {
@@ -766,12 +778,12 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
     hipLaunchKernelGGL(segsort_small_kernel, dim3(persistent_blocks(segsort_small_kernel, 256, ctx->num_cus, g_small)),
                        dim3(256), 0, s, d_off, lists + (size_t)SEG_L_SMALL * nq, cnt + SEG_L_SMALL, used, d_q,
                        d_radii, r, pts, npts, d_ids, capacity);
-    hipLaunchKernelGGL((segsort_block_kernel<SEG_MID, 256>),
-                       dim3(persistent_blocks(segsort_block_kernel<SEG_MID, 256>, 256, ctx->num_cus, g_mid)),
+    hipLaunchKernelGGL((segsort_rank_kernel<SEG_MID, 256>),
+                       dim3(persistent_blocks(segsort_rank_kernel<SEG_MID, 256>, 256, ctx->num_cus, g_mid)),
                        dim3(256), 0, s, d_off, lists + (size_t)SEG_L_MID * nq, cnt + SEG_L_MID, used, d_q, d_radii, r,
                        pts, npts, d_ids, capacity);
-    hipLaunchKernelGGL((segsort_block_kernel<SEG_LARGE, 1024>),
-                       dim3(persistent_blocks(segsort_block_kernel<SEG_LARGE, 1024>, 1024, ctx->num_cus, g_large)),
+    hipLaunchKernelGGL((segsort_rank_kernel<SEG_LARGE, 1024>),
+                       dim3(persistent_blocks(segsort_rank_kernel<SEG_LARGE, 1024>, 1024, ctx->num_cus, g_large)),
                        dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_LARGE * nq, cnt + SEG_L_LARGE, used, d_q,
                        d_radii, r, pts, npts, d_ids, capacity);
     struct {
