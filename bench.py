@@ -231,11 +231,14 @@ def _sah_build_line(ctx, dp, n, leaf, reps=5):
 
 def _bh_line(ctx, pm, oracle, thetas=(0.3, 0.5, 0.7), check=True):
     """1M Plummer bodies: force kernel time per theta (HIP events), counts from
-    an instrumented run: bytes = per body 16 + 96 per node opened (64 B node +
-    32 B moments) + 20 per body interaction (float4 + mass) + 12 written;
-    FP32 work = 30 flops per interaction (body or far node: 3 sub, 3 fma,
-    sqrt, div, 4 mul, 3 mul, 3 compensated adds of 4) + 32 per node opened
-    (two opening tests)."""
+    an instrumented run.  The kernel walks the tree once per wavefront of 64
+    Morton-adjacent bodies, so its bytes are per wavefront: 96 per node it
+    loads (64 B node + 32 B moments) + 20 per bucket body it loads (float4 +
+    mass), plus per body 16 read and 12 written.  The per-body count of the
+    same walks (96 per node opened + 20 per body interaction, as if every body
+    walked alone) is reported beside it.  FP32 work = 30 flops per interaction
+    (body or far node: 3 sub, 3 fma, sqrt, div, 4 mul, 3 mul, 3 compensated
+    adds of 4) + 32 per node opened (two opening tests)."""
     import gprt_hip
     n = len(pm)
     dpm = gprt_hip.DeviceArray.from_host(ctx, pm)
@@ -252,12 +255,16 @@ def _bh_line(ctx, pm, oracle, thetas=(0.3, 0.5, 0.7), check=True):
         acc = ctx.barnes_hut(dpm, theta=theta, eps=1e-3, G=1.0, flags=gprt_hip.COLLECT_COUNTERS)
         st = ctx.stats()
         V, L, F = st["node_visits"], st["point_tests"], st["rerank_reads"]
+        Vw, Lw = st["wave_steps"], st["lane_steps"]
         t = float(np.median(ms[1:]))
-        alg = 16 * n + 96 * V + 20 * L + 12 * n
+        alg = 16 * n + 96 * Vw + 20 * Lw + 12 * n
+        per_body = 16 * n + 96 * V + 20 * L + 12 * n
         flops = 30 * (L + F) + 32 * V
         line = {"ms_force": t, "bodies_per_s": n / (t * 1e-3),
-                "roofline": _roof(alg, t, kernel="bh_force_kernel", nodes_opened=V, body_interactions=L,
-                                  far_interactions=F, fp32_tflops=flops / (t * 1e-3) / 1e12,
+                "roofline": _roof(alg, t, kernel="bh_wave_kernel (one walk per wavefront of 64 bodies)",
+                                  wave_node_loads=Vw, wave_body_loads=Lw, nodes_opened=V, body_interactions=L,
+                                  far_interactions=F, per_body_walk_bytes=per_body,
+                                  fp32_tflops=flops / (t * 1e-3) / 1e12,
                                   fp32_frac=flops / (t * 1e-3) / 1e12 / FP32_PEAK_TFLOPS)}
         if check:
             a = acc.to_host()[sample].astype(np.float64)

@@ -89,6 +89,12 @@ extern "C" {
  * counterpart in the reference (its GAS build is OptiX's,
  * gprt-optix/host/main.cpp:200-248). */
 #define GPRT_BUILD_SAH 64
+/* gprt_hip_bh_forces: walk the tree one body per lane, each lane on its own
+ * path, instead of the default walk of 64 Morton-adjacent bodies per
+ * wavefront over the union of their paths with wave-uniform node loads (each
+ * body keeps its own opening decisions there too).  Both give the same bits;
+ * the per-lane walk is kept for comparison. */
+#define GPRT_BH_LANE_WALK 128
 
 #define GPRT_ID_NONE 0xFFFFFFFFu
 
@@ -158,8 +164,11 @@ typedef struct {
     double ms_kernel;          /* first traversal round alone (time_kernels; range: GPRT_TIME_KERNELS) */
     int64_t wave_steps;        /* per-lane rounds: sum over wavefront tasks of their longest walk in
                                   loop iterations (counters enabled); lane steps / (64 * this) is the
-                                  SIMD efficiency of the walk loop */
-    int64_t lane_steps;        /* per-lane rounds: loop iterations summed over lanes (counters enabled) */
+                                  SIMD efficiency of the walk loop.  Barnes-Hut (wavefront walk):
+                                  nodes loaded, summed over wavefronts */
+    int64_t lane_steps;        /* per-lane rounds: loop iterations summed over lanes (counters enabled).
+                                  Barnes-Hut (wavefront walk): bucket bodies loaded, summed over
+                                  wavefronts */
 } gprt_hip_run_stats;
 
 typedef struct {
@@ -285,7 +294,7 @@ GPRT_HIP_API int gprt_hip_range(gprt_hip_ctx* ctx, gprt_hip_tree* tree, const fl
  * leaf_size: bodies per leaf bucket (1..32).  A bucket that is opened is
  * summed body by body, so larger buckets trade time for accuracy: on a
  * 1M-body Plummer sphere at theta 0.5, leaf 8 gives a median relative error
- * of 1.1e-3 in 26 ms, leaf 1 gives 1.3e-3 in 20 ms.  The Plummer law needs
+ * of 1.1e-3 in 10.3 ms, leaf 1 gives 1.3e-3 in 10.2 ms.  The Plummer law needs
  * eps > 0 (GPRT_ERR_ARG otherwise: every body's own term would be 0/0); the
  * reference law takes eps = 0.
  * Replaces barnes_hut! (gprt-macros/src/lib.rs:752-834). */

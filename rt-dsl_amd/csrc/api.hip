@@ -171,7 +171,7 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
               int64_t capacity, DevBuf& counts_buf, DevBuf& d2_buf, DevBuf& big_buf, DevBuf& keys_buf,
               DevBuf& perm_buf, DevBuf& qs4_buf, DevBuf& sums_buf, int64_t* h_total, const float* d_radii = nullptr, bool count = false, bool time_walk = false);
 int bh_run(Ctx* ctx, Tree& T, const float4* d_pm, int n, float theta, float eps, float G, float* d_acc,
-           DevBuf& mass_buf, bool count, bool ref_law);
+           DevBuf& mass_buf, bool count, bool ref_law, bool lane_walk);
 int bh_prepare(Ctx* ctx, const float4* d_pm, int n, float* d_xyz, unsigned* flag);
 
 }  // namespace gprt
@@ -1416,7 +1416,7 @@ extern "C" int gprt_hip_bh_forces(gprt_hip_ctx* h, const float* pos_mass, int64_
     x->bh_tree.build_wide = false;
     GPRT_TRY(tree_build(&c, x->bh_xyz.as<float>(), off, 1, leaf_size, &x->bh_tree));
     GPRT_TRY(bh_run(&c, x->bh_tree, d_pm, (int)n, theta, eps, G, d_acc, x->bh_mass, (flags & GPRT_COLLECT_COUNTERS) != 0,
-                          (flags & GPRT_BH_REFERENCE_LAW) != 0));
+                          (flags & GPRT_BH_REFERENCE_LAW) != 0, (flags & GPRT_BH_LANE_WALK) != 0));
     GPRT_HIP_TRY(hipEventRecord(c.ev3, s));
     if (!dev) GPRT_HIP_TRY(hipMemcpyAsync(acc_out, d_acc, 12ull * n, hipMemcpyDeviceToHost, s));
     GPRT_TRY(finish_stats(c, n, true));

@@ -98,6 +98,56 @@ struct WaveList {
     }
 };
 
+// (d64, id) of the lane `lane ^ j`
+__device__ __forceinline__ void xor_lane(double d, uint32_t i, int j, double& od, uint32_t& oi) {
+    od = __hiloint2double(__shfl_xor((int)__double2hiint(d), j), __shfl_xor((int)__double2loint(d), j));
+    oi = (uint32_t)__shfl_xor((int)i, j);
+}
+
+// One compare-exchange stage of a bitonic network over the 64 lanes: the lane
+// keeps the smaller of its pair's two (d64, id) keys when keep_min is set,
+// else the larger.  Keys are distinct pairs except for equal sentinels, whose
+// exchange changes nothing.
+__device__ __forceinline__ void bitonic_stage(double& d, uint32_t& i, int j, bool keep_min) {
+    double od;
+    uint32_t oi;
+    xor_lane(d, i, j, od, oi);
+    const bool other_lt = lex_lt(od, oi, d, i);
+    if (other_lt == keep_min) {
+        d = od;
+        i = oi;
+    }
+}
+
+// Merges up to 64 candidates (one per lane; (+inf, NONE) where a lane has
+// none) into a one-slot-per-lane list (M = 1): the candidates are sorted into
+// descending order by a bitonic network, the lane-wise minimum of the
+// ascending list and the descending candidates is the bitonic sequence of the
+// 64 smallest keys of both (the half-cleaner of Batcher's merge), and a
+// bitonic merge puts it in ascending order.  The result equals inserting the
+// candidates one by one.  About 27 stages of 3 lane exchanges each, which is
+// less than sequential inserts from about 8 candidates on.
+__device__ __forceinline__ void merge_batch(double& ld, uint32_t& li, double cd, uint32_t ci, int lane) {
+#pragma unroll
+    for (int s = 2; s <= 64; s <<= 1) {
+#pragma unroll
+        for (int j = s >> 1; j > 0; j >>= 1) {
+            // pairs in blocks with (lane & s) set ascend, the others descend, so
+            // the final block (s = 64) descends
+            const bool asc = (lane & s) != 0, lower = (lane & j) == 0;
+            bitonic_stage(cd, ci, j, lower == asc);
+        }
+    }
+    if (lex_lt(cd, ci, ld, li)) {
+        ld = cd;
+        li = ci;
+    }
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) bitonic_stage(ld, li, j, (lane & j) == 0);
+}
+
+constexpr int QW_BATCH = 8;  // candidates of one scan from which M = 1 lists merge them as a batch
+
 // one query's walk state
 template <int M>
 struct WaveQuery {
@@ -118,7 +168,13 @@ struct WaveQuery {
             const float4 p = pts[f + (v ? t : m - 1)];
             const double dd = d2d(qx, qy, qz, p.x, p.y, p.z);
             const uint32_t ii = (uint32_t)__float_as_int(p.w);
-            uint64_t mask = __ballot(v && dd <= T0 && lex_lt(dd, ii, wd, wi));
+            const bool cand = v && dd <= T0 && lex_lt(dd, ii, wd, wi);
+            uint64_t mask = __ballot(cand);
+            if (M == 1 && __popcll((unsigned long long)mask) >= QW_BATCH) {
+                merge_batch(L.d[0], L.id[0], cand ? dd : INFINITY, cand ? ii : GPRT_ID_NONE, lane);
+                L.worst(wd, wi);
+                mask = 0;
+            }
             while (mask) {
                 const int j = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)mask) - 1);
                 mask &= mask - 1;

@@ -23,6 +23,7 @@ OK = 0
 ERR_ARG, ERR_INPUT, ERR_HIP, ERR_ALLOC, ERR_CAPACITY, ERR_STATE = 1, 2, 3, 4, 5, 6
 PTR_HOST, PTR_DEVICE, NO_SYNC, COLLECT_COUNTERS, KEYS_REF30, BH_REFERENCE_LAW = 0, 1, 2, 4, 8, 16
 BUILD_SAH = 64
+BH_LANE_WALK = 128  # Barnes-Hut: one independent walk per lane (default: one walk per wavefront)
 TIME_KERNELS = 32
 ID_NONE = 0xFFFFFFFF
 K_MAX = 4096  # GPRT_HIP_K_MAX: largest k of the k-NN entry points

@@ -4,7 +4,8 @@
 
 Per case, none of the checks fitted to observed errors:
 - the kernel's accelerations equal, bit for bit, those of the host build of
-  the same walk and mass refit (gprt_hip.hostwalk.barnes_hut);
+  the same walk and mass refit (gprt_hip.hostwalk.barnes_hut), and the
+  wavefront walk (the default) equals the per-lane walk for every body;
 - they lie within the rounding allowance of the same terms summed in FP64;
 - Plummer law: they lie within the derived monopole bound of the FP64 direct
   sum plus that allowance (test_host_bh.py states both); at theta = 0 the bound
@@ -35,6 +36,10 @@ def test_bh_matrix(ctx, seed, kind, n, leaf, law):
     for theta in (0.0, 0.5, 1.0):
         acc = ctx.barnes_hut(pm, theta=theta, eps=EPS, G=1.0, leaf_size=leaf, law=law)
         assert np.all(np.isfinite(acc))
+        # the per-lane walk gives the wavefront walk's bits for every body
+        lane = ctx.barnes_hut(pm, theta=theta, eps=EPS, G=1.0, leaf_size=leaf, law=law,
+                              flags=gprt_hip.BH_LANE_WALK)
+        assert np.array_equal(acc.view(np.uint32), lane.view(np.uint32)), (kind, n, leaf, law, theta)
         h = H.barnes_hut(pm, theta, eps=EPS, G=1.0, leaf_size=leaf, law=law, sample=sample)
         a = acc[sample]
         diff = np.nonzero((a.view(np.uint32) != h["acc32"].view(np.uint32)).any(axis=1))[0]
@@ -55,10 +60,15 @@ def test_bh_counters_match_host(ctx):
     rng = np.random.default_rng(77)
     pm = bodies("plummer", 20_000, rng)
     for theta in (0.3, 0.7):
-        ctx.barnes_hut(pm, theta=theta, eps=EPS, flags=gprt_hip.COLLECT_COUNTERS)
-        s = ctx.stats()
         h = H.barnes_hut(pm, theta, eps=EPS)
-        assert (s["node_visits"], s["point_tests"], s["rerank_reads"]) == tuple(int(x) for x in h["stats"])
+        for walk in (0, gprt_hip.BH_LANE_WALK):
+            ctx.barnes_hut(pm, theta=theta, eps=EPS, flags=gprt_hip.COLLECT_COUNTERS | walk)
+            s = ctx.stats()
+            assert (s["node_visits"], s["point_tests"], s["rerank_reads"]) == tuple(int(x) for x in h["stats"])
+            if walk == 0:
+                # a wavefront loads a node (a bucket) once for all of its lanes that visit it
+                assert 0 < s["wave_steps"] <= s["node_visits"]
+                assert 0 < s["lane_steps"] <= s["point_tests"]
 
 
 def test_bh_plummer_eps_zero_rejected(ctx):
