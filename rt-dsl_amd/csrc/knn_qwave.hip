@@ -146,7 +146,10 @@ __device__ __forceinline__ void merge_batch(double& ld, uint32_t& li, double cd,
     for (int j = 32; j > 0; j >>= 1) bitonic_stage(ld, li, j, (lane & j) == 0);
 }
 
-constexpr int QW_BATCH = 8;  // candidates of one scan from which M = 1 lists merge them as a batch
+#ifndef GPRT_QW_BATCH
+#define GPRT_QW_BATCH 8
+#endif
+constexpr int QW_BATCH = GPRT_QW_BATCH;  // candidates of one scan from which M = 1 lists merge them as a batch
 
 // one query's walk state
 template <int M>
