@@ -296,21 +296,39 @@ def _extra_all(ctx, args, oracle):
     dp.free()
     dq.free()
     pts, poff, q, qoff, kc = D.clustered_batch()
+
+    def clouds_run(tree, cfg):
+        ids, out_off = tree.k_nn_batched(q, qoff, kc, cfg)
+        times = []
+        for _ in range(steps):
+            tree.k_nn_batched(q, qoff, kc, cfg)
+            times.append(ctx.stats()["ms_query"])
+        return ids, out_off, float(np.median(times))
+
     tree = ctx.build_batched(pts, poff, leaf_size=8)
-    cfg = gprt_hip.default_config()
-    ids, out_off = tree.k_nn_batched(q, qoff, kc, cfg)
-    times = []
-    for _ in range(steps):
-        tree.k_nn_batched(q, qoff, kc, cfg)
-        times.append(ctx.stats()["ms_query"])
-    ms = float(np.median(times))
+    ids, out_off, ms_default = clouds_run(tree, gprt_hip.default_config())
+    tree.close()
+    # the batched autotuner's pick (schedule, leaf size and tree), adopted when
+    # faster at identical ids
+    t0 = time.perf_counter()
+    tcfg, tleaf, tsah = ctx.autotune_batched(pts, poff, q, qoff, kc, budget_trials=24, with_tree=True)
+    tune_s = time.perf_counter() - t0
+    tree = ctx.build_batched(pts, poff, leaf_size=tleaf, sah=tsah)
+    tids, _, ms_tuned = clouds_run(tree, tcfg)
+    adopt = bool(np.array_equal(tids, ids)) and ms_tuned < 0.99 * ms_default
+    if adopt:
+        ids = tids
+    ms = ms_tuned if adopt else ms_default
     ok = True
     for c in range(len(kc)):
         k = int(kc[c])
         r, _, _, _ = oracle.kdtree_knn(pts[poff[c]:poff[c + 1]], q[qoff[c]:qoff[c + 1]], k)
         ok &= bool(np.array_equal(ids[out_off[c]:out_off[c + 1]].reshape(-1, k), r))
     ex["batched_clouds"] = {"clouds": len(kc), "points": len(pts), "queries": len(q), "ms_device": ms,
-                            "qps": len(q) / (ms * 1e-3), "all_clouds_eq_oracle": ok}
+                            "qps": len(q) / (ms * 1e-3), "all_clouds_eq_oracle": ok,
+                            "ms_default_schedule": ms_default,
+                            "autotune": {"seconds": tune_s, "leaf_size": tleaf, "tree": "sah" if tsah else "lbvh",
+                                         "schedule": tcfg.as_dict(), "ms": ms_tuned, "adopted": adopt}}
     tree.close()
     ex["barnes_hut_plummer_1m"] = _bh_line(ctx, D.plummer(), oracle)
     return ex

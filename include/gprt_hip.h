@@ -333,6 +333,18 @@ GPRT_HIP_API int gprt_hip_autotune(gprt_hip_ctx* ctx, const float* pts, int64_t 
 GPRT_HIP_API int gprt_hip_autotune_ex(gprt_hip_ctx* ctx, const float* pts, int64_t n, const float* queries,
                                       int64_t nq, int k, int budget_trials, gprt_hip_config* cfg_out,
                                       int* leaf_size_out, int* build_flags_out, const char* json_path, int ptr_flags);
+/* The autotuner for uneven batches: the same search over the batched call of
+ * gprt_hip_knn_batched on a tree of the given clouds (gprt_hip_tree_build_batched),
+ * every candidate's ids equal to the default schedule's over all clouds.
+ * cloud_offsets, query_offsets, k_per_cloud: host arrays as in those calls;
+ * pts and queries host or device by ptr_flags.  build_flags_out may be NULL
+ * (LBVH only).  Replaces gprt_autotune! (gprt-macros/src/lib.rs:120-214) for
+ * the batched workload, which the reference tunes one data set at a time. */
+GPRT_HIP_API int gprt_hip_autotune_batched(gprt_hip_ctx* ctx, const float* pts, const int64_t* cloud_offsets,
+                                           int nclouds, const float* queries, const int64_t* query_offsets,
+                                           const int* k_per_cloud, int budget_trials, gprt_hip_config* cfg_out,
+                                           int* leaf_size_out, int* build_flags_out, const char* json_path,
+                                           int ptr_flags);
 
 /* ---------------- testing hooks ---------------- */
 /* Morton bits (8..30, default 16) by which the batched k-NN orders its queries

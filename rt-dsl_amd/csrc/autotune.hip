@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -31,11 +32,20 @@ struct Cand {
     int sah;  // 1: the SAH tree (GPRT_BUILD_SAH), 0: the LBVH
 };
 
-int time_config(gprt_hip_ctx* h, gprt_hip_tree* t, const float* d_q, int64_t nq, int k, const gprt_hip_config& cfg,
-                uint32_t* d_ids, double* ms_out) {
+// One tuning problem: builds the tree of a candidate (leaf size, SAH or not)
+// and runs one k-NN call with a config into d_ids (nout ids).
+struct Problem {
+    std::function<int(int leaf, int sah, gprt_hip_tree** out)> build;
+    std::function<int(gprt_hip_tree* t, const gprt_hip_config& cfg, uint32_t* d_ids)> run;
+    size_t nout = 0;
+};
+
+// median query-phase time of three calls after a warm-up
+int time_config(gprt_hip_ctx* h, const Problem& P, gprt_hip_tree* t, const gprt_hip_config& cfg, uint32_t* d_ids,
+                double* ms_out) {
     std::vector<double> ts;
     for (int r = 0; r < 4; ++r) {
-        GPRT_TRY(gprt_hip_knn(h, t, d_q, nq, k, &cfg, d_ids, nullptr, GPRT_PTR_DEVICE));
+        GPRT_TRY(P.run(t, cfg, d_ids));
         gprt_hip_run_stats st;
         gprt_hip_stats(h, &st);
         if (r > 0) ts.push_back(st.ms_query);
@@ -66,45 +76,30 @@ std::string schedule_json(const gprt_hip_config& c, int leaf, int sah, double ms
     return buf;
 }
 
-}  // namespace
 
-extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
-                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
-                                    int* build_flags_out, const char* json_path, int flags) {
-    if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
-    const bool with_sah = build_flags_out != nullptr;
-    GPRT_CTX_LOCK(h);
-    if (n <= 0 || nq <= 0 || k < 1 || k > 64) return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= 64");
-    Ctx& c = h->c;
-    GPRT_HIP_TRY(hipSetDevice(c.device));
-    DevBuf dq, dids, dref;
-    const float* d_q = queries;
-    if (!(flags & GPRT_PTR_DEVICE)) {
-        GPRT_TRY(dq.reserve(12ull * nq));
-        GPRT_HIP_TRY(hipMemcpy(dq.ptr, queries, 12ull * nq, hipMemcpyHostToDevice));
-        d_q = dq.as<float>();
-    }
-    GPRT_TRY(dids.reserve(4ull * nq * k));
-    GPRT_TRY(dref.reserve(4ull * nq * k));
+// Coordinate descent from the default schedule on the leaf-8 LBVH, one knob at
+// a time (like the reference's hill climb), bounded by budget_trials timed
+// candidates; every candidate's ids must equal the default's.
+int autotune_core(gprt_hip_ctx* h, const Problem& P, int budget_trials, bool with_sah, Cand* best_out,
+                  double* best_ms_out) {
+    DevBuf dids, dref;
+    GPRT_TRY(dids.reserve(4 * std::max<size_t>(P.nout, 1)));
+    GPRT_TRY(dref.reserve(4 * std::max<size_t>(P.nout, 1)));
     const int leaves[3] = {4, 8, 16};
     // trees[s * 3 + i]: leaf size leaves[i], s = 1 the SAH tree
     gprt_hip_tree* trees[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int rc = GPRT_OK;
-    for (int i = 0; i < (with_sah ? 6 : 3) && rc == GPRT_OK; ++i)
-        rc = gprt_hip_tree_build(h, pts, n, leaves[i % 3], (flags & GPRT_PTR_DEVICE) | (i >= 3 ? GPRT_BUILD_SAH : 0),
-                                 &trees[i]);
+    for (int i = 0; i < (with_sah ? 6 : 3) && rc == GPRT_OK; ++i) rc = P.build(leaves[i % 3], i >= 3, &trees[i]);
     // reference answer: default schedule on the leaf-8 tree
     gprt_hip_config base;
     gprt_hip_config_default(&base);
-    if (rc == GPRT_OK) rc = gprt_hip_knn(h, trees[1], d_q, nq, k, &base, dref.as<uint32_t>(), nullptr, GPRT_PTR_DEVICE);
+    if (rc == GPRT_OK) rc = P.run(trees[1], base, dref.as<uint32_t>());
     std::vector<uint32_t> ref, got;
     if (rc == GPRT_OK) {
-        ref.resize((size_t)nq * k);
-        if (hipMemcpy(ref.data(), dref.ptr, 4ull * nq * k, hipMemcpyDeviceToHost) != hipSuccess)
+        ref.resize(P.nout);
+        if (hipMemcpy(ref.data(), dref.ptr, 4 * P.nout, hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(GPRT_ERR_HIP, "copy failed");
     }
-    // coordinate descent from the default, one knob at a time (like the
-    // reference's hill climb), bounded by budget_trials timed candidates
     Cand best{base, 8, 0};
     double best_ms = 1e30;
     int trials = 0;
@@ -113,9 +108,9 @@ extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n
         if (trials >= std::max(budget_trials, 1)) return GPRT_OK;
         ++trials;
         double ms = 0;
-        GPRT_TRY(time_config(h, tree_of(cd.leaf, cd.sah), d_q, nq, k, cd.cfg, dids.as<uint32_t>(), &ms));
-        got.resize((size_t)nq * k);
-        GPRT_HIP_TRY(hipMemcpy(got.data(), dids.ptr, 4ull * nq * k, hipMemcpyDeviceToHost));
+        GPRT_TRY(time_config(h, P, tree_of(cd.leaf, cd.sah), cd.cfg, dids.as<uint32_t>(), &ms));
+        got.resize(P.nout);
+        GPRT_HIP_TRY(hipMemcpy(got.data(), dids.ptr, 4 * P.nout, hipMemcpyDeviceToHost));
         if (got != ref) return fail(GPRT_ERR_STATE, "autotune: a candidate schedule changed the result");
         if (ms < best_ms) {
             best_ms = ms;
@@ -175,10 +170,16 @@ extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n
         }
     }
     for (auto* t : trees) gprt_hip_tree_destroy(t);
-    dq.release();
     dids.release();
     dref.release();
     if (rc != GPRT_OK) return rc;
+    *best_out = best;
+    *best_ms_out = best_ms;
+    return GPRT_OK;
+}
+
+int finish_tune(const Cand& best, double best_ms, gprt_hip_config* cfg_out, int* leaf_size_out, int* build_flags_out,
+                const char* json_path) {
     *cfg_out = best.cfg;
     *leaf_size_out = best.leaf;
     if (build_flags_out) *build_flags_out = best.sah ? GPRT_BUILD_SAH : 0;
@@ -190,6 +191,85 @@ extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n
         fclose(f);
     }
     return GPRT_OK;
+}
+
+}  // namespace
+
+extern "C" int gprt_hip_autotune_ex(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,
+                                    int k, int budget_trials, gprt_hip_config* cfg_out, int* leaf_size_out,
+                                    int* build_flags_out, const char* json_path, int flags) {
+    if (!h || !pts || !queries || !cfg_out || !leaf_size_out) return fail(GPRT_ERR_ARG, "null argument");
+    GPRT_CTX_LOCK(h);
+    if (n <= 0 || nq <= 0 || k < 1 || k > GPRT_HIP_K_MAX)
+        return fail(GPRT_ERR_ARG, "need n > 0, nq > 0, 1 <= k <= " + std::to_string(GPRT_HIP_K_MAX));
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    DevBuf dq;
+    const float* d_q = queries;
+    if (!(flags & GPRT_PTR_DEVICE)) {
+        GPRT_TRY(dq.reserve(12ull * nq));
+        GPRT_HIP_TRY(hipMemcpy(dq.ptr, queries, 12ull * nq, hipMemcpyHostToDevice));
+        d_q = dq.as<float>();
+    }
+    Problem P;
+    P.nout = (size_t)nq * k;
+    P.build = [&](int leaf, int sah, gprt_hip_tree** out) {
+        return gprt_hip_tree_build(h, pts, n, leaf, (flags & GPRT_PTR_DEVICE) | (sah ? GPRT_BUILD_SAH : 0), out);
+    };
+    P.run = [&](gprt_hip_tree* t, const gprt_hip_config& cfg, uint32_t* d_ids) {
+        return gprt_hip_knn(h, t, d_q, nq, k, &cfg, d_ids, nullptr, GPRT_PTR_DEVICE);
+    };
+    Cand best{};
+    double best_ms = 0;
+    const int rc = autotune_core(h, P, budget_trials, build_flags_out != nullptr, &best, &best_ms);
+    dq.release();
+    if (rc != GPRT_OK) return rc;
+    return finish_tune(best, best_ms, cfg_out, leaf_size_out, build_flags_out, json_path);
+}
+
+extern "C" int gprt_hip_autotune_batched(gprt_hip_ctx* h, const float* pts, const int64_t* cloud_offsets,
+                                         int nclouds, const float* queries, const int64_t* query_offsets,
+                                         const int* k_per_cloud, int budget_trials, gprt_hip_config* cfg_out,
+                                         int* leaf_size_out, int* build_flags_out, const char* json_path,
+                                         int flags) {
+    if (!h || !pts || !cloud_offsets || !queries || !query_offsets || !k_per_cloud || !cfg_out || !leaf_size_out)
+        return fail(GPRT_ERR_ARG, "null argument");
+    if (nclouds < 1) return fail(GPRT_ERR_ARG, "nclouds must be >= 1");
+    GPRT_CTX_LOCK(h);
+    size_t nout = 0;
+    for (int c = 0; c < nclouds; ++c) {
+        const int64_t qn = query_offsets[c + 1] - query_offsets[c];
+        if (qn < 0 || k_per_cloud[c] < 1 || k_per_cloud[c] > GPRT_HIP_K_MAX)
+            return fail(GPRT_ERR_ARG, "bad query offsets or k of cloud " + std::to_string(c));
+        nout += (size_t)qn * k_per_cloud[c];
+    }
+    const int64_t nq = query_offsets[nclouds] - query_offsets[0];
+    if (query_offsets[0] != 0 || nq <= 0 || cloud_offsets[0] != 0 || cloud_offsets[nclouds] <= 0)
+        return fail(GPRT_ERR_ARG, "need offsets from 0, some points and some queries");
+    Ctx& c = h->c;
+    GPRT_HIP_TRY(hipSetDevice(c.device));
+    DevBuf dq;
+    const float* d_q = queries;
+    if (!(flags & GPRT_PTR_DEVICE)) {
+        GPRT_TRY(dq.reserve(12ull * nq));
+        GPRT_HIP_TRY(hipMemcpy(dq.ptr, queries, 12ull * nq, hipMemcpyHostToDevice));
+        d_q = dq.as<float>();
+    }
+    Problem P;
+    P.nout = nout;
+    P.build = [&](int leaf, int sah, gprt_hip_tree** out) {
+        return gprt_hip_tree_build_batched(h, pts, cloud_offsets, nclouds, leaf,
+                                           (flags & GPRT_PTR_DEVICE) | (sah ? GPRT_BUILD_SAH : 0), out);
+    };
+    P.run = [&](gprt_hip_tree* t, const gprt_hip_config& cfg, uint32_t* d_ids) {
+        return gprt_hip_knn_batched(h, t, d_q, query_offsets, k_per_cloud, &cfg, d_ids, nullptr, GPRT_PTR_DEVICE);
+    };
+    Cand best{};
+    double best_ms = 0;
+    const int rc = autotune_core(h, P, budget_trials, build_flags_out != nullptr, &best, &best_ms);
+    dq.release();
+    if (rc != GPRT_OK) return rc;
+    return finish_tune(best, best_ms, cfg_out, leaf_size_out, build_flags_out, json_path);
 }
 
 extern "C" int gprt_hip_autotune(gprt_hip_ctx* h, const float* pts, int64_t n, const float* queries, int64_t nq,

@@ -136,6 +136,8 @@ SIGNATURES = [
                                  ctypes.POINTER(_int), ctypes.c_char_p, _int]),
     ("gprt_hip_autotune_ex", _int, [_vp, _vp, _i64, _vp, _i64, _int, _int, ctypes.POINTER(Config),
                                     ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.c_char_p, _int]),
+    ("gprt_hip_autotune_batched", _int, [_vp, _vp, _vp, _int, _vp, _vp, _vp, _int, ctypes.POINTER(Config),
+                                         ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.c_char_p, _int]),
     ("gprt_hip_read_points", _int, [ctypes.c_char_p, _int, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.POINTER(ctypes.c_int64)]),
     ("gprt_hip_schedule_from_json", _int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Config),
@@ -376,6 +378,32 @@ class Context:
                                           ctypes.byref(bflags) if with_tree else None,
                                           json_path.encode() if json_path else None,
                                           PTR_DEVICE if dev else PTR_HOST))
+        if with_tree:
+            return cfg, leaf.value, bool(bflags.value & BUILD_SAH)
+        return cfg, leaf.value
+
+    def autotune_batched(self, pts, cloud_offsets, queries, query_offsets, k_per_cloud, budget_trials=24,
+                         json_path=None, with_tree=False):
+        """The autotuner over an uneven batch (gprt_hip_autotune_batched):
+        (Config, leaf_size[, sah]) for gprt_hip_knn_batched on these clouds."""
+        pts, queries = _points(pts), _points(queries)
+        p, dev = _ptr(pts)
+        qp, qdev = _ptr(queries)
+        if dev != qdev:
+            raise ValueError("points and queries must both be host arrays or both device arrays")
+        poff = np.ascontiguousarray(cloud_offsets, np.int64)
+        qoff = np.ascontiguousarray(query_offsets, np.int64)
+        kc = np.ascontiguousarray(k_per_cloud, np.int32)
+        if not (len(poff) == len(qoff) == len(kc) + 1):
+            raise ValueError("need nclouds + 1 cloud and query offsets and nclouds k values")
+        cfg = Config()
+        leaf = ctypes.c_int()
+        bflags = ctypes.c_int()
+        _check(lib().gprt_hip_autotune_batched(self.h, p, poff.ctypes.data, len(kc), qp, qoff.ctypes.data,
+                                               kc.ctypes.data, budget_trials, ctypes.byref(cfg),
+                                               ctypes.byref(leaf), ctypes.byref(bflags) if with_tree else None,
+                                               json_path.encode() if json_path else None,
+                                               PTR_DEVICE if dev else PTR_HOST))
         if with_tree:
             return cfg, leaf.value, bool(bflags.value & BUILD_SAH)
         return cfg, leaf.value

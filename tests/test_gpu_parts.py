@@ -546,3 +546,33 @@ def test_autotune_chooses_tree(ctx, tmp_path):
         assert np.array_equal(t.k_nn(q, 5, cfg), oracle.knn_brute(pts, q, 5))
     finally:
         t.close()
+
+
+def test_autotune_batched_uneven_clouds(ctx, tmp_path):
+    """The autotuner over an uneven batch (clouds of different sizes, k in
+    {1, 5, 16, 64}): its pick, on the tree it chose, gives every cloud the
+    oracle's ids, and the written file reads back to the same schedule."""
+    rng = np.random.default_rng(31)
+    sizes = [300, 5_000, 1_200, 20_000, 64, 9_000]
+    ks = [1, 5, 16, 64, 5, 16]
+    clouds = [rng.normal(size=(n, 3)).astype(np.float32) * (1 + i) for i, n in enumerate(sizes)]
+    qs = [c[rng.integers(0, len(c), max(1, len(c) // 10))] + rng.normal(size=(max(1, len(c) // 10), 3)).astype(
+        np.float32) * 0.01 for c in clouds]
+    pts = np.concatenate(clouds)
+    q = np.concatenate(qs)
+    poff = np.r_[0, np.cumsum(sizes)].astype(np.int64)
+    qoff = np.r_[0, np.cumsum([len(x) for x in qs])].astype(np.int64)
+    kc = np.array(ks, np.int32)
+    path = str(tmp_path / "gprt_wisdom.json")
+    cfg, leaf, sah = ctx.autotune_batched(pts, poff, q, qoff, kc, budget_trials=10, json_path=path,
+                                          with_tree=True)
+    assert leaf in (4, 8, 16)
+    assert gprt_hip.load_schedule(path, with_tree=True)[1:] == (leaf, sah)
+    t = ctx.build_batched(pts, poff, leaf_size=leaf, sah=sah)
+    try:
+        ids, out_off = t.k_nn_batched(q, qoff, kc, cfg)
+    finally:
+        t.close()
+    for c in range(len(sizes)):
+        want = oracle.knn_brute(clouds[c], qs[c], ks[c])
+        assert np.array_equal(ids[out_off[c]:out_off[c + 1]].reshape(-1, ks[c]), want), c
