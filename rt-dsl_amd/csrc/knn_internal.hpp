@@ -15,6 +15,7 @@ constexpr int CNT_ERROR = 42;     // traversals stopped by the step guard (corru
 constexpr int CNT_BADID = 43;     // re-ranked candidates outside their query's cloud (must stay 0)
 constexpr int CNT_AUX = 44;       // [0..3]: scratch counters of the range pipeline (segment lists)
 constexpr int CNT_OFFSETS = 48;   // range fill: queries whose hits do not fill their offsets range
+constexpr int CNT_RETRY = 49;     // range fill: long segments left to the chunked merge sort
 constexpr int CNT_TOTAL = 56;   // the last slot (CNT_TOTAL - 1) is the input-validation flag
 constexpr int CNT_STATS64 = 6;   // u64 statistics after the counters: visits, tests, re-rank reads,
                                  // (spare), wave steps, lane steps

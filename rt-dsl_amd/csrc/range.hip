@@ -274,6 +274,7 @@ __device__ __forceinline__ bool pair_gt(double da, uint32_t ia, double db, uint3
 
 constexpr int SEG_MID = 2048;
 constexpr int SEG_LARGE = 8192;
+constexpr int SEG_PARTS = 12288;  // longest segment sorted in LDS parts (longer: chunks merged in global memory)
 constexpr int SEG_RUN = 32;       // longest bucket ordered by rank counting
 
 struct SegQ {
@@ -592,6 +593,188 @@ __global__ __launch_bounds__(T) void segsort_rank_kernel(const int64_t* __restri
     }
 }
 
+// Segments of CAP + 1 .. LIM hits (LIM <= 2 CAP), one workgroup each (a
+// persistent grid over the list of longer segments): positions expanded into
+// LDS (2 CAP of them fit over S.d), keys gathered into registers (LIM / T per
+// lane), one
+// bucket count over CAP / 2 buckets by distance and its scan, then the buckets
+// in consecutive parts of at most CAP elements: each part's keys scattered into
+// LDS in bucket order, ranked inside their buckets and written (a bitonic
+// network over the part when one of its buckets holds more than SEG_RUN).  The
+// parts cover increasing distance ranges, so sorting each part sorts the
+// segment.  A longer segment, or one with a single bucket of more than CAP
+// elements, is appended untouched to the retry list (with its merge-scratch
+// offset) for segsort_huge_kernel.
+template <int CAP, int T, int LIM>
+__global__ __launch_bounds__(T) void segsort_parts_kernel(const int64_t* __restrict__ off,
+                                                          const int* __restrict__ list,
+                                                          const unsigned* __restrict__ count,
+                                                          const uint32_t* __restrict__ used,
+                                                          const float* __restrict__ qxyz,
+                                                          const float* __restrict__ radii, float r_all,
+                                                          const float4* __restrict__ pts, uint32_t npts,
+                                                          uint32_t* ids, int64_t capacity,
+                                                          const int64_t* __restrict__ huge_tmp, int* retry_list,
+                                                          int64_t* retry_tmp, unsigned* retry_count) {
+    static_assert(CAP / 2 + 1 >= 2 * T, "expansion scratch lives in S.cnt");
+    static_assert(LIM % T == 0 && LIM <= 2 * CAP, "elements per lane");
+    constexpr int PER = LIM / T;
+    constexpr int NB = CAP / 2;  // buckets
+    __shared__ SegRankLds<CAP, T> S;
+    __shared__ int part_end[NB + 2];  // end buckets of the parts; [NB + 1]: their number
+    const int t = threadIdx.x;
+    const unsigned n = *count;
+    uint32_t* P = reinterpret_cast<uint32_t*>(S.d);
+    for (unsigned li = blockIdx.x; li < n; li += gridDim.x) {
+        const int g = list[li];
+        int64_t b, e;
+        seg_bounds(off, g, capacity, b, e);
+        const int m = (int)(e - b);
+        if (m > LIM) {
+            if (t == 0) {
+                const unsigned j = atomicAdd(retry_count, 1u);
+                retry_list[j] = g;
+                retry_tmp[j] = huge_tmp[li];
+            }
+            continue;
+        }
+        const SegQ q = seg_query(qxyz, radii, r_all, g);
+        const int ne = expand_slots<T>(ids + b, (int)min(used[g], (uint32_t)m), m, S.cnt, S.cnt + T, S.wsum,
+                                       [&](int x, uint32_t p) { P[x] = p; });
+        uint32_t pos[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int x = t + i * T;
+            pos[i] = x < ne ? min(P[x], npts - 1) : 0u;
+        }
+        __syncthreads();  // P (over S.d) read
+        double dv[PER];
+        uint32_t iv[PER];
+        const double scale = q.r2 > 0.0 ? (double)NB / q.r2 : 0.0;
+        // an element's bucket is recomputed where needed (registers are short
+        // with 2 CAP / T keys per lane)
+        auto bkt = [&](int i) { return seg_bucket(dv[i], scale, NB); };
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            dv[i] = INFINITY;
+            iv[i] = 0xFFFFFFFFu;
+            if (t + i * T < m) {
+                const float4 p = pts[pos[i]];
+                dv[i] = d2d(q.x, q.y, q.z, p.x, p.y, p.z);
+                iv[i] = (uint32_t)__float_as_int(p.w);
+            }
+        }
+        for (int j = t; j <= NB; j += T) S.cnt[j] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            if (t + i * T < m) atomicAdd(&S.cnt[bkt(i)], 1u);
+        __syncthreads();
+        block_exscan<T>(S.cnt, NB, S.wsum);
+        __syncthreads();
+        if (t == 0) {
+            // S.cnt[k]: start of bucket k, S.cnt[NB] = m.  Parts: maximal runs
+            // of buckets holding at most CAP elements; 0 parts if a bucket alone
+            // holds more
+            S.cnt[NB] = (uint32_t)m;
+            int np = 0, b1 = 0;
+            while (b1 < NB) {
+                int lo = b1, hi = NB;  // largest b2 in [b1, NB] with start(b2) - start(b1) <= CAP
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (S.cnt[mid] - S.cnt[b1] <= (uint32_t)CAP) lo = mid;
+                    else hi = mid - 1;
+                }
+                if (lo == b1) {
+                    np = 0;
+                    break;
+                }
+                part_end[np++] = lo;
+                b1 = lo;
+            }
+            part_end[NB + 1] = np;
+        }
+        __syncthreads();
+        const int np = part_end[NB + 1];
+        if (np == 0) {
+            if (t == 0) {
+                const unsigned j = atomicAdd(retry_count, 1u);
+                retry_list[j] = g;
+                retry_tmp[j] = huge_tmp[li];
+            }
+            __syncthreads();
+            continue;
+        }
+        int b1 = 0;
+        for (int pi = 0; pi < np; ++pi) {
+            const int b2 = part_end[pi];
+            const uint32_t base = S.cnt[b1];  // still the start of bucket b1
+            const int psize = (int)(S.cnt[b2] - base);
+            if (t == 0) S.flag = 0;
+            __syncthreads();  // every lane read base and psize before the scatter moves the counts
+#pragma unroll
+            for (int i = 0; i < PER; ++i)
+                if (t + i * T < m && bkt(i) >= b1 && bkt(i) < b2) {
+                    const uint32_t slot = atomicAdd(&S.cnt[bkt(i)], 1u) - base;
+                    S.d[slot] = dv[i];
+                    S.id[slot] = iv[i];
+                }
+            __syncthreads();
+            // S.cnt[k] is now the end of bucket k for the part's buckets, and
+            // S.cnt[b1 - 1] the end of the previous part's last bucket
+            bool big = false;
+#pragma unroll
+            for (int i = 0; i < PER; ++i)
+                if (t + i * T < m && bkt(i) >= b1 && bkt(i) < b2) {
+                    const int bi = bkt(i);
+                    const int st = bi ? (int)S.cnt[bi - 1] : 0, en = (int)S.cnt[bi];
+                    big |= en - st > SEG_RUN;
+                }
+            if (big) S.flag = 1;
+            __syncthreads();
+            if (!S.flag) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i)
+                    if (t + i * T < m && bkt(i) >= b1 && bkt(i) < b2) {
+                        const int bi = bkt(i);
+                        const int st = bi ? (int)S.cnt[bi - 1] : 0, en = (int)S.cnt[bi];
+                        int r = 0;
+                        for (int y = st; y < en; ++y)
+                            r += pair_gt(dv[i], iv[i], S.d[y - (int)base], S.id[y - (int)base]);
+                        ids[b + st + r] = iv[i];
+                    }
+            } else {
+                int Pn = 1;
+                while (Pn < psize) Pn <<= 1;
+                for (int j = psize + t; j < Pn; j += T) {
+                    S.d[j] = INFINITY;
+                    S.id[j] = 0xFFFFFFFFu;
+                }
+                __syncthreads();
+                for (int size = 2; size <= Pn; size <<= 1)
+                    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                        for (int u = t; u < Pn / 2; u += T) {
+                            const int j = 2 * u - (u & (stride - 1)), pj = j + stride;
+                            const bool up = (j & size) == 0;
+                            const double da = S.d[j], dc = S.d[pj];
+                            const uint32_t ia = S.id[j], ic = S.id[pj];
+                            if (pair_gt(da, ia, dc, ic) == up) {
+                                S.d[j] = dc;
+                                S.id[j] = ic;
+                                S.d[pj] = da;
+                                S.id[pj] = ia;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                for (int j = t; j < psize; j += T) ids[b + base + j] = S.id[j];
+            }
+            __syncthreads();
+            b1 = b2;
+        }
+    }
+}
+
 // number of elements of the sorted run (rd, ri)[0..n) that order before (d, i)
 __device__ __forceinline__ int rank_in(const double* __restrict__ rd, const uint32_t* __restrict__ ri, int n,
                                        double d, uint32_t i) {
@@ -749,8 +932,9 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
         GPRT_HIP_TRY(hipStreamSynchronize(s));
         return GPRT_OK;
     }
-    // segment lists (4 x nq ints) and the huge segments' merge-scratch offsets
-    GPRT_TRY(big_buf.reserve(16ull * nq + 8ull * nq + 16));
+    // segment lists (4 x nq ints), the long segments' merge-scratch offsets
+    // (nq int64), the retry list (nq ints) and its offsets (nq int64)
+    GPRT_TRY(big_buf.reserve(16ull * nq + 8ull * nq + 4ull * (nq + 1) + 8ull * nq + 16));
     int* lists = big_buf.as<int>();
     int64_t* huge_tmp = (int64_t*)(lists + 4 * (size_t)nq);
     // slots written per query, in the count buffer (unused by the fill call)
@@ -786,20 +970,29 @@ int range_run(Ctx* ctx, const Tree& T, const float* d_q, int nq, float r, int64_
                        dim3(persistent_blocks(segsort_rank_kernel<SEG_LARGE, 1024>, 1024, ctx->num_cus, g_large)),
                        dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_LARGE * nq, cnt + SEG_L_LARGE, used, d_q,
                        d_radii, r, pts, npts, d_ids, capacity);
+    static std::atomic<int> g_parts[64];
+    int* retry_list = reinterpret_cast<int*>(huge_tmp + nq);
+    int64_t* retry_tmp = huge_tmp + nq + (nq + 1) / 2;
+    unsigned* retry_count = counters + CNT_RETRY;
+    hipLaunchKernelGGL((segsort_parts_kernel<SEG_LARGE, 1024, SEG_PARTS>),
+                       dim3(persistent_blocks(segsort_parts_kernel<SEG_LARGE, 1024, SEG_PARTS>, 1024, ctx->num_cus,
+                                              g_parts)),
+                       dim3(1024), 0, s, d_off, lists + (size_t)SEG_L_HUGE * nq, cnt + SEG_L_HUGE, used, d_q, d_radii,
+                       r, pts, npts, d_ids, capacity, huge_tmp, retry_list, retry_tmp, retry_count);
     struct {
-        unsigned nhuge;
+        unsigned nretry;
         unsigned long long total;
     } h{0, 0};
-    GPRT_HIP_TRY(hipMemcpyAsync(&h.nhuge, cnt + SEG_L_HUGE, 4, hipMemcpyDeviceToHost, s));
+    GPRT_HIP_TRY(hipMemcpyAsync(&h.nretry, retry_count, 4, hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipMemcpyAsync(&h.total, huge_total, 8, hipMemcpyDeviceToHost, s));
     GPRT_HIP_TRY(hipStreamSynchronize(s));
-    if (h.nhuge) {
+    if (h.nretry) {
+        // the merge scratch spans every long segment's offset (from the classify pass)
         GPRT_TRY(ctx->sort_tmp[0].reserve(16ull * h.total));
         GPRT_TRY(ctx->sort_tmp[1].reserve(8ull * h.total));
-        hipLaunchKernelGGL(segsort_huge_kernel, dim3(h.nhuge), dim3(SEG_HUGE_T), 0, s, d_off,
-                           lists + (size_t)SEG_L_HUGE * nq, huge_tmp, cnt + SEG_L_HUGE, used, d_q, d_radii, r, pts,
-                           npts, d_ids, capacity, ctx->sort_tmp[0].as<double>(), ctx->sort_tmp[1].as<uint32_t>(),
-                           (int64_t)h.total);
+        hipLaunchKernelGGL(segsort_huge_kernel, dim3(h.nretry), dim3(SEG_HUGE_T), 0, s, d_off, retry_list, retry_tmp,
+                           retry_count, used, d_q, d_radii, r, pts, npts, d_ids, capacity,
+                           ctx->sort_tmp[0].as<double>(), ctx->sort_tmp[1].as<uint32_t>(), (int64_t)h.total);
     }
     GPRT_HIP_TRY(hipGetLastError());
     return GPRT_OK;

@@ -216,16 +216,17 @@ def test_range_long_segments(ctx):
 def test_range_segment_tiers(ctx):
     """Every sort tier at its edges: points at the integers of a line, query i at
     x = 20000.5 with r = m/2 hits exactly m points, in pairs at equal distance
-    (ties broken by id), for m on both sides of 1, 64, 2,048 and 8,192 and
-    beyond (chunks plus merges); plus a segment of 9,000 coincident points
-    (one bucket: the bitonic network, and merges of all-equal distances)."""
+    (ties broken by id), for m on both sides of 1, 64, 2,048, 8,192 and
+    12,288 (sorted in LDS parts up to there) and beyond (chunks plus merges);
+    plus a segment of 9,000 coincident points (one bucket of more than 8,192:
+    left to the chunks and merges of all-equal distances)."""
     x = np.arange(40_001, dtype=np.float32)
     line = np.c_[x, np.zeros((len(x), 2), np.float32)].astype(np.float32)
     dup = np.tile(np.float32([[5e4, 7.0, 7.0]]), (9_000, 1))
     pts = np.concatenate([line, dup])
     t = ctx.build(pts, leaf_size=8)
     try:
-        for m in (2, 64, 66, 2048, 2050, 8192, 8194, 17_000):
+        for m in (2, 64, 66, 2048, 2050, 8192, 8194, 12_288, 12_290, 17_000):
             q = np.float32([[20000.5, 0.0, 0.0]])
             r = float(np.float32(m / 2))
             off, ids = t.range_query(q, r)
