@@ -519,14 +519,23 @@ def main():
                                     "what": "H2D points + LBVH build + H2D queries + batched k-NN + D2H ids, "
                                             "wall clock"}
         steps = max(5, args.steps // 5)
-        ms32, msp32, ib, ip, st = _knn_pair(ctx, tree, dq, 32, cfg, steps, 2)
+        # k = 32 on the head configuration's tree (the autotuner's pick when
+        # adopted: its tree and leaf size) with the default schedule
+        ms32, msp32, ib, ip, st = _knn_pair(ctx, head_tree, dq, 32, cfg, steps, 2)
         ref32, _, _, _ = oracle.kdtree_knn(pts, q, 32)
-        details["knn_k32"] = {"batched_qps": nq / (ms32 * 1e-3), "plain_qps": nq / (msp32 * 1e-3),
+        details["knn_k32"] = {"tree": "sah" if head_sah else "lbvh", "leaf_size": head_leaf,
+                              "batched_qps": nq / (ms32 * 1e-3), "plain_qps": nq / (msp32 * 1e-3),
                               "speedup_vs_plain": msp32 / ms32, "ms_batched": ms32, "ms_plain": msp32,
                               "fallback": st["fallback_queries"], "rounds": st["rounds_run"],
                               "batched_eq_oracle": bool(np.array_equal(ib, ref32)),
                               "plain_eq_oracle": bool(np.array_equal(ip, ref32))}
-        details["range_r0.5"] = _range_line(ctx, tree, pts, q, oracle)
+        # the range query on the SAH tree (the counterpart of the reference's
+        # default build strategy, PreferFastTrace) with leaf 8: the fastest of
+        # leaf 4 / 8 / 16 on either tree for this workload
+        rtree = ctx.build(dp, leaf_size=8, sah=True)
+        details["range_r0.5"] = _range_line(ctx, rtree, pts, q, oracle)
+        details["range_r0.5"].update(tree="sah", leaf_size=8)
+        rtree.close()
         details["lbvh_rebuild"] = _build_line(ctx, dp, len(pts), leaf)
         details["sah_build"] = _sah_build_line(ctx, dp, len(pts), head_leaf)
     if args.all:

@@ -16,10 +16,11 @@
 //   * the walk is wave-uniform: scalar node loads, closest child first, and a
 //     stack of (node, box d64 lower bound) held two entries per lane, written
 //     by a lane select and read with v_readlane;
-//   * a child holding at most 64 points is not descended into: its point range
-//     (children of a Karras node cover adjacent ranges of the sorted points) is
-//     tested one point per lane, and the lanes whose (d64, id) beats the current
-//     k-th are inserted in lane order.
+//   * a child holding at most QW_SCAN (128) points is not descended into: its
+//     point range (children of a Karras node cover adjacent ranges of the
+//     sorted points) is tested 64 points at a time, one per lane, and the lanes
+//     whose (d64, id) beats the current k-th are inserted in lane order (or,
+//     with one slot per lane and 8 or more of them, merged at once).
 // Every comparison is on (d64, id) in binary64, so the ids are exact by
 // construction: there is no guard band and no fallback.  A box whose d64 equals
 // the k-th is still entered (it may hold a smaller id).  Rounds follow the
@@ -34,7 +35,10 @@
 namespace gprt {
 
 constexpr int QW_STACK = 128;  // two (ref, d) entries per lane; the Karras depth bound is <= 96
-constexpr int QW_SCAN = 64;    // children with at most this many points are scanned, not descended into
+#ifndef GPRT_QW_SCAN
+#define GPRT_QW_SCAN 128
+#endif
+constexpr int QW_SCAN = GPRT_QW_SCAN;  // children with at most this many points are scanned, not descended into
 
 __device__ __forceinline__ bool lex_lt(double a, uint32_t ia, double b, uint32_t ib) {
     return a < b || (a == b && ia < ib);
