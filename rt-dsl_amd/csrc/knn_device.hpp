@@ -62,6 +62,14 @@ __device__ __forceinline__ int2 ld2i(const GPRT_AS4 int2* p) {
     return make_int2(f[0], f[1]);
 }
 
+// Adds the lanes that reach this point together (the active lanes of the
+// wavefront) to *counter with one atomic: atomics of every lane on one counter
+// serialise.
+__device__ __forceinline__ void count_entering(unsigned* counter) {
+    const uint64_t m = __ballot(true);
+    if ((int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(counter, (unsigned)__popcll(m));
+}
+
 // Work queue of a persistent round: a wave takes the next 64 queries.  The
 // head is read first, so once the queue is drained the remaining waves leave
 // without contending on the atomic.
@@ -69,6 +77,24 @@ __device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
     unsigned h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (h >= n_in) return h;
     return atomicAdd(head, 64u);
+}
+
+// The same queue with the first task of every wavefront assigned by its index
+// in the grid (tasks [0, waves) in the order the grid's waves are numbered),
+// so the persistent grid's start does not queue all of its waves on one
+// atomic; later tasks come from the counter, offset past those.  `first`
+// tells which call this is for the wave.
+__device__ __forceinline__ unsigned grab_work_first(unsigned* head, unsigned n_in, bool first, unsigned wave,
+                                                    unsigned nwaves) {
+#ifdef GPRT_ATOMIC_FIRST_TASK
+    (void)first; (void)wave; (void)nwaves;
+    return grab_work(head, n_in);
+#else
+    if (first) return wave * 64u;
+    const unsigned skip = nwaves * 64u;
+    if (skip >= n_in) return n_in;
+    return skip + grab_work(head, n_in - skip);
+#endif
 }
 
 // FP64 re-rank of the guard band in (d64, id) order and output of the first k.

@@ -84,10 +84,13 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
                             (size_t)gridDim.x * 256};
     const int max_depth = S + a.spill_depth;
     uint64_t visits = 0, tests = 0, reranks = 0, lane_steps = 0, wave_steps = 0;
+    const unsigned wave = (blockIdx.x * blockDim.x + tid) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    bool first = true;
     while (true) {
         unsigned base = 0;
-        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
+        if (lane == 0) base = grab_work_first(&a.counters[CNT_WORK + a.round], n_in, first, wave, nwaves);
         base = __shfl(base, 0);
+        first = false;
         if (base >= n_in) break;
         const unsigned i = base + lane;
         const bool valid = i < n_in;
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lan
             if (resolved || overflow || !a.inline_rounds) break;
             ++round;
             R *= a.mult;
-            atomicAdd(&a.counters[CNT_ACTIVE + round], 1u);  // statistics: queries entering this round
+            count_entering(&a.counters[CNT_ACTIVE + round]);  // statistics: queries entering this round
             }
             const float kth = Q.kth();
             const bool full = kth < INFINITY;

@@ -154,6 +154,7 @@ __device__ __forceinline__ void merge_batch(double& ld, uint32_t& li, double cd,
 #define GPRT_QW_BATCH 8
 #endif
 constexpr int QW_BATCH = GPRT_QW_BATCH;  // candidates of one scan from which M = 1 lists merge them as a batch
+constexpr unsigned QW_GRAB = 4;          // queries a wavefront takes from the work counter at a time
 
 // one query's walk state
 template <int M>
@@ -277,11 +278,21 @@ __global__ __launch_bounds__(256) void knn_qwave_kernel(RoundArgs a) {
     const unsigned wave0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     uint64_t visits = 0, tests = 0;
     unsigned qi = wave0;
+    // batched form: queries are taken QW_GRAB at a time from the work counter
+    // (one atomic per chunk: atomics of every wavefront on one counter
+    // serialise), the first chunk of each wavefront by its index in the grid
+    unsigned q_next = wave0 * QW_GRAB, q_end = q_next + QW_GRAB;
+    const unsigned skip = nwaves * QW_GRAB;
     while (true) {
         if (!plain) {
-            unsigned h = 0;
-            if (lane == 0) h = atomicAdd(&a.counters[CNT_WORK + a.round], 1u);
-            qi = uniform((int)__shfl((int)h, 0));
+            if (q_next >= q_end) {
+                unsigned h = 0;
+                if (lane == 0 && skip < n_in) h = atomicAdd(&a.counters[CNT_WORK + a.round], (unsigned)QW_GRAB);
+                h = (unsigned)uniform((int)__shfl((int)h, 0));
+                q_next = skip < n_in ? skip + h : n_in;
+                q_end = q_next + QW_GRAB;
+            }
+            qi = q_next++;
         }
         if (qi >= n_in) break;
         float qx, qy, qz;

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256, C == 8 ? GPRT_WIDE_MIN_WAVES : 1) void knn_wid
             if (resolved || overflow || !a.inline_rounds) break;
             ++round;
             R *= a.mult;
-            atomicAdd(&a.counters[CNT_ACTIVE + round], 1u);  // statistics: queries entering this round
+            count_entering(&a.counters[CNT_ACTIVE + round]);  // statistics: queries entering this round
             }
             const float kth = Q.kth();
             const bool full = kth < INFINITY;
