@@ -82,19 +82,15 @@ __device__ __forceinline__ unsigned grab_work(unsigned* head, unsigned n_in) {
 // The same queue with the first task of every wavefront assigned by its index
 // in the grid (tasks [0, waves) in the order the grid's waves are numbered),
 // so the persistent grid's start does not queue all of its waves on one
-// atomic; later tasks come from the counter, offset past those.  `first`
-// tells which call this is for the wave.
+// returning atomic (they serialise on the counter: 5,120 waves took about
+// 60 us to get their first task); later tasks come from the counter, offset
+// past those.  `first` tells which call this is for the wave.
 __device__ __forceinline__ unsigned grab_work_first(unsigned* head, unsigned n_in, bool first, unsigned wave,
                                                     unsigned nwaves) {
-#ifdef GPRT_ATOMIC_FIRST_TASK
-    (void)first; (void)wave; (void)nwaves;
-    return grab_work(head, n_in);
-#else
     if (first) return wave * 64u;
     const unsigned skip = nwaves * 64u;
     if (skip >= n_in) return n_in;
     return skip + grab_work(head, n_in - skip);
-#endif
 }
 
 // FP64 re-rank of the guard band in (d64, id) order and output of the first k.

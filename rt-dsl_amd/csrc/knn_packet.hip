@@ -28,10 +28,13 @@ __global__ __launch_bounds__(256) void knn_packet_kernel(RoundArgs a) {
     const GPRT_AS4 int2* roots = const_view(a.roots);
     const float4* __restrict__ pts = a.pts;                // per-lane loads (re-rank)
     uint64_t visits = 0, tests = 0, reranks = 0;
+    const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    bool first = true;
     while (true) {
         unsigned base = 0;
-        if (lane == 0) base = grab_work(&a.counters[CNT_WORK + a.round], n_in);
+        if (lane == 0) base = grab_work_first(&a.counters[CNT_WORK + a.round], n_in, first, wave, nwaves);
         base = __shfl(base, 0);
+        first = false;
         if (base >= n_in) break;
         const unsigned i = base + lane;
         const bool valid = i < n_in;

@@ -39,6 +39,7 @@ constexpr int SAH_STACK = 24;    // segment stack of a wavefront (log2(1024) + 1
 
 // counters
 constexpr int SC_CUR = 0, SC_NEXT = 1, SC_SMALL = 2, SC_BUCKET = 3, SC_CHUNKS = 4, SC_ERR = 5, SC_WORDS = 8;
+constexpr int SAH_BBUF = 64;  // buckets a wavefront of the small-segment kernel batches before appending them
 
 struct SahSeg {
     int lo, hi;
@@ -395,11 +396,45 @@ __global__ __launch_bounds__(256) void sah_small_kernel(SahArgs a) {
     __shared__ SahSeg stack[4][SAH_STACK];
     __shared__ SahSplit spl[4];
     __shared__ int fnd[4];
+    // buckets of the wavefront's subtrees, appended to the global list in
+    // batches: one returning atomic per batch instead of one per bucket
+    // (returning atomics on one counter serialise)
+    __shared__ int4 bbuf[4][SAH_BBUF];
+    __shared__ int bn[4];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t* w = bins[wv];
     SahSeg* st = stack[wv];
     const unsigned nsmall = min(a.ctr[SC_SMALL], (unsigned)a.cap_small);
     const uint64_t lt = lanemask_lt();
+    if (lane == 0) bn[wv] = 0;
+    // a half of at most leaf_size points (lane 0): a single point's parent
+    // link, or a bucket into the wavefront's batch
+    auto emit_small = [&](int lo, int hi, int slot) {
+        if (hi - lo <= 0) return;
+        if (hi - lo == 1) {
+            a.parent[(a.n - 1) + lo] = parent_link(slot);
+            return;
+        }
+        bbuf[wv][bn[wv]++] = make_int4(lo, hi, slot, 0);
+    };
+    // the batch to the global list (the whole wavefront)
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int cnt = bn[wv];
+        unsigned base = 0;
+        if (lane == 0 && cnt > 0) base = atomicAdd(&a.ctr[SC_BUCKET], (unsigned)cnt);
+        base = __shfl(base, 0);
+        for (int j = lane; j < cnt; j += 64) {
+            if (base + j < (unsigned)a.cap_bucket) a.buckets[base + j] = bbuf[wv][j];
+            else atomicAdd(&a.ctr[SC_ERR], 1u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) bn[wv] = 0;
+    };
     for (unsigned si = blockIdx.x * 4 + wv; si < nsmall; si += gridDim.x * 4) {
         if (lane == 0) st[0] = a.small[si];
         int sp = 1;
@@ -491,8 +526,8 @@ __global__ __launch_bounds__(256) void sah_small_kernel(SahArgs a) {
             if (lane == 0) {
                 sah_node(a, g.lo, mid, g.hi, g.slot);
                 // halves of at most leaf_size points: single points and buckets
-                if (mid - g.lo <= a.leaf) sah_emit(a, g.lo, mid, id << 1, g.depth + 1, llo, lhi);
-                if (g.hi - mid <= a.leaf) sah_emit(a, mid, g.hi, (id << 1) | 1, g.depth + 1, rlo, rhi);
+                if (mid - g.lo <= a.leaf) emit_small(g.lo, mid, id << 1);
+                if (g.hi - mid <= a.leaf) emit_small(mid, g.hi, (id << 1) | 1);
                 // larger halves onto the stack, the larger one first
                 SahSeg h[2];
                 int nh = 0;
@@ -526,8 +561,10 @@ __global__ __launch_bounds__(256) void sah_small_kernel(SahArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (bn[wv] > SAH_BBUF - 2) flush();  // room for the next split's two halves
         }
     }
+    flush();
 }
 
 // subtrees of 2..leaf_size points: index-median splits, one thread each
