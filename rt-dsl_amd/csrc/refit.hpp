@@ -54,7 +54,10 @@ __global__ __launch_bounds__(REFIT_THREADS) void refit_chunk_kernel(Slots slots,
                 // leaving the chunk: the kernel boundary publishes this store
                 P::store(slots, p, side, v);
                 const int old = atomicAdd(&gcnt[p], 1);
-                if (old == 1) ready[atomicAdd(ready_count, 1u)] = p;
+                // queued with one returning atomic per wavefront (returning
+                // atomics on one counter serialise)
+                const int slot = wave_append(ready_count, old == 1);
+                if (slot >= 0) ready[slot] = p;
                 break;
             }
         }
