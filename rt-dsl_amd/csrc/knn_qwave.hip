@@ -102,20 +102,33 @@ struct WaveList {
     }
 };
 
-// (d64, id) of the lane `lane ^ j`
-__device__ __forceinline__ void xor_lane(double d, uint32_t i, int j, double& od, uint32_t& oi) {
-    od = __hiloint2double(__shfl_xor((int)__double2hiint(d), j), __shfl_xor((int)__double2loint(d), j));
-    oi = (uint32_t)__shfl_xor((int)i, j);
+// value of the lane `lane ^ J`: a DPP quad permutation for J = 1, 2 (no LDS
+// traffic), ds_swizzle in its bitmask mode (and 0x1f, xor J) within 32 lanes
+// for J = 4, 8, 16, ds_bpermute for J = 32
+template <int J>
+__device__ __forceinline__ int xor_lane_i(int v) {
+    if constexpr (J == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    else if constexpr (J == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    else if constexpr (J < 32) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (J << 10));
+    else return __shfl_xor(v, J);
+}
+
+// (d64, id) of the lane `lane ^ J`
+template <int J>
+__device__ __forceinline__ void xor_lane(double d, uint32_t i, double& od, uint32_t& oi) {
+    od = __hiloint2double(xor_lane_i<J>((int)__double2hiint(d)), xor_lane_i<J>((int)__double2loint(d)));
+    oi = (uint32_t)xor_lane_i<J>((int)i);
 }
 
 // One compare-exchange stage of a bitonic network over the 64 lanes: the lane
 // keeps the smaller of its pair's two (d64, id) keys when keep_min is set,
 // else the larger.  Keys are distinct pairs except for equal sentinels, whose
 // exchange changes nothing.
-__device__ __forceinline__ void bitonic_stage(double& d, uint32_t& i, int j, bool keep_min) {
+template <int J>
+__device__ __forceinline__ void bitonic_stage(double& d, uint32_t& i, bool keep_min) {
     double od;
     uint32_t oi;
-    xor_lane(d, i, j, od, oi);
+    xor_lane<J>(d, i, od, oi);
     const bool other_lt = lex_lt(od, oi, d, i);
     if (other_lt == keep_min) {
         d = od;
@@ -131,23 +144,51 @@ __device__ __forceinline__ void bitonic_stage(double& d, uint32_t& i, int j, boo
 // bitonic merge puts it in ascending order.  The result equals inserting the
 // candidates one by one.  About 27 stages of 3 lane exchanges each, which is
 // less than sequential inserts from about 8 candidates on.
+// one sort step of the candidates: pairs (l, l ^ J) in blocks of S lanes; pairs
+// in blocks with (lane & S) set ascend, the others descend, so the final block
+// (S = 64) descends
+template <int S, int J>
+__device__ __forceinline__ void sort_step(double& cd, uint32_t& ci, int lane) {
+    const bool asc = (lane & S) != 0, lower = (lane & J) == 0;
+    bitonic_stage<J>(cd, ci, lower == asc);
+}
+template <int J>
+__device__ __forceinline__ void merge_step(double& ld, uint32_t& li, int lane) {
+    bitonic_stage<J>(ld, li, (lane & J) == 0);
+}
+
 __device__ __forceinline__ void merge_batch(double& ld, uint32_t& li, double cd, uint32_t ci, int lane) {
-#pragma unroll
-    for (int s = 2; s <= 64; s <<= 1) {
-#pragma unroll
-        for (int j = s >> 1; j > 0; j >>= 1) {
-            // pairs in blocks with (lane & s) set ascend, the others descend, so
-            // the final block (s = 64) descends
-            const bool asc = (lane & s) != 0, lower = (lane & j) == 0;
-            bitonic_stage(cd, ci, j, lower == asc);
-        }
-    }
+    sort_step<2, 1>(cd, ci, lane);
+    sort_step<4, 2>(cd, ci, lane);
+    sort_step<4, 1>(cd, ci, lane);
+    sort_step<8, 4>(cd, ci, lane);
+    sort_step<8, 2>(cd, ci, lane);
+    sort_step<8, 1>(cd, ci, lane);
+    sort_step<16, 8>(cd, ci, lane);
+    sort_step<16, 4>(cd, ci, lane);
+    sort_step<16, 2>(cd, ci, lane);
+    sort_step<16, 1>(cd, ci, lane);
+    sort_step<32, 16>(cd, ci, lane);
+    sort_step<32, 8>(cd, ci, lane);
+    sort_step<32, 4>(cd, ci, lane);
+    sort_step<32, 2>(cd, ci, lane);
+    sort_step<32, 1>(cd, ci, lane);
+    sort_step<64, 32>(cd, ci, lane);
+    sort_step<64, 16>(cd, ci, lane);
+    sort_step<64, 8>(cd, ci, lane);
+    sort_step<64, 4>(cd, ci, lane);
+    sort_step<64, 2>(cd, ci, lane);
+    sort_step<64, 1>(cd, ci, lane);
     if (lex_lt(cd, ci, ld, li)) {
         ld = cd;
         li = ci;
     }
-#pragma unroll
-    for (int j = 32; j > 0; j >>= 1) bitonic_stage(ld, li, j, (lane & j) == 0);
+    merge_step<32>(ld, li, lane);
+    merge_step<16>(ld, li, lane);
+    merge_step<8>(ld, li, lane);
+    merge_step<4>(ld, li, lane);
+    merge_step<2>(ld, li, lane);
+    merge_step<1>(ld, li, lane);
 }
 
 #ifndef GPRT_QW_BATCH
