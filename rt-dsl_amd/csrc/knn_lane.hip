@@ -7,11 +7,41 @@
 
 namespace gprt {
 
-// minimum wavefronts per SIMD the register allocation of the capacity-8
-// class must allow (1: no constraint)
+// Minimum wavefronts per SIMD the register allocation of each list capacity
+// must allow (1: no constraint), for the k classes whose occupancy pays for
+// the spills it costs (LiDAR, 400 K queries; unconstrained -> constrained):
+//   C = 8:  5 -> 6 waves (85 -> 80 VGPRs, 7 spilled), k = 5 round kernel
+//           0.244 -> 0.222 ms; 7 waves (17 spilled) is slower;
+//   C = 16: 4 -> 5 waves (111 -> 96 VGPRs, 29 spilled), k = 10 0.48 -> 0.41 ms;
+//   C = 40: 1 -> 2 waves (261 -> 256 registers, 6 spilled), k = 32 1.81 -> 1.24 ms.
+// 6 x 1,024 SIMDs x 64 lanes hold 393 K queries, nearly a 400 K batch in one
+// pass of the persistent grid.
 #ifndef GPRT_LANE_MIN_WAVES
-#define GPRT_LANE_MIN_WAVES 1
+#define GPRT_LANE_MIN_WAVES 6
 #endif
+#ifndef GPRT_LANE_MIN_WAVES_16
+#define GPRT_LANE_MIN_WAVES_16 5
+#endif
+#ifndef GPRT_LANE_MIN_WAVES_24
+#define GPRT_LANE_MIN_WAVES_24 1
+#endif
+#ifndef GPRT_LANE_MIN_WAVES_32
+#define GPRT_LANE_MIN_WAVES_32 1
+#endif
+#ifndef GPRT_LANE_MIN_WAVES_40
+#define GPRT_LANE_MIN_WAVES_40 2
+#endif
+#ifndef GPRT_LANE_MIN_WAVES_48
+#define GPRT_LANE_MIN_WAVES_48 1
+#endif
+template <int C>
+constexpr int lane_min_waves() {
+    return C == 8 ? GPRT_LANE_MIN_WAVES
+                  : C == 16 ? GPRT_LANE_MIN_WAVES_16
+                            : C == 24 ? GPRT_LANE_MIN_WAVES_24
+                                      : C == 32 ? GPRT_LANE_MIN_WAVES_32
+                                                : C == 40 ? GPRT_LANE_MIN_WAVES_40 : GPRT_LANE_MIN_WAVES_48;
+}
 
 
 // The lane's memory for lane_walk: nodes and points in HBM, its column of the
@@ -67,7 +97,7 @@ struct LaneMemDev {
 // S + spill_depth is finished by the exact FP64 kernel.
 // ---------------------------------------------------------------------------
 template <int C, int G, int S, bool COUNT, bool ROUND0>
-__global__ __launch_bounds__(256, C == 8 ? GPRT_LANE_MIN_WAVES : 1) void knn_lane_kernel(RoundArgs a) {
+__global__ __launch_bounds__(256, lane_min_waves<C>()) void knn_lane_kernel(RoundArgs a) {
     static_assert((S & (S - 1)) == 0, "LDS ring size must be a power of two");
     __shared__ int sref[S][256];
     __shared__ float sdist[S][256];
