@@ -58,6 +58,8 @@ __device__ __forceinline__ void seg_bounds(const int64_t* __restrict__ off, int 
 // walk order and checks that the hits fill the range exactly (a range that
 // does not match the queries and radius is counted in `bad`).  COUNT adds
 // node visits, point tests and slots to stats64.
+constexpr int RANGE_RING = 16;  // stack entries per lane kept in LDS by the range walk
+
 template <bool WRITE, bool COUNT>
 __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nodes, const float4* __restrict__ pts,
                                                     int2 root, const float4* __restrict__ qs, int nq, float r_all,
@@ -67,6 +69,7 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
                                                     uint32_t* __restrict__ used, int max_steps, unsigned* err,
                                                     unsigned* bad,
                                                     unsigned long long* stats64) {
+    __shared__ int ring[RANGE_RING][256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t visits = 0, tests = 0, slots = 0;
     if (i < nq) {
@@ -122,8 +125,11 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
             if (root.x < 0) {
                 visit(~root.x, root.y);
             } else {
+                // the stack: its RANGE_RING most recent entries in the lane's LDS
+                // column, older ones in the private array (scratch memory)
                 int stk[EXACT_STACK];
-                int sp = 0, node = root.x, steps = 0;
+                int sp = 0, lo = 0, node = root.x, steps = 0;  // entries [lo, sp) are in the ring
+                const int tid = threadIdx.x;
                 while (true) {
                     if (++steps > max_steps || sp >= EXACT_STACK) { atomicAdd(err, 1u); break; }
                     const Node nd = nodes[node];
@@ -150,14 +156,25 @@ __global__ __launch_bounds__(256) void range_kernel(const Node* __restrict__ nod
                     }
                     const bool gA = rA >= 0 && inA && !wA, gB = rB >= 0 && inB && !wB;
                     if (gA && gB) {
-                        stk[sp++] = rB;
+                        if (sp - lo == RANGE_RING) {  // ring full: its oldest entry to the array
+                            stk[lo] = ring[lo & (RANGE_RING - 1)][tid];
+                            ++lo;
+                        }
+                        ring[sp & (RANGE_RING - 1)][tid] = rB;
+                        ++sp;
                         node = rA;
                         continue;
                     }
                     if (gA) { node = rA; continue; }
                     if (gB) { node = rB; continue; }
                     if (sp == 0) break;
-                    node = stk[--sp];
+                    --sp;
+                    if (sp >= lo) {
+                        node = ring[sp & (RANGE_RING - 1)][tid];
+                    } else {
+                        node = stk[sp];
+                        lo = sp;
+                    }
                 }
             }
         }
