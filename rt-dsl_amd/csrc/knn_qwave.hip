@@ -310,8 +310,11 @@ __device__ __forceinline__ bool qwave_walk(WaveQuery<M>& Q, const GPRT_AS4 Node*
 // Persistent launch over the queries of one k class (batched form: sorted
 // positions a.base + i, i < the class size in a.counters[CNT_ACTIVE]) or over
 // the caller's xyz array in input order (plain form: a.q_xyz set, a.last set).
+#ifndef GPRT_QW_MIN_WAVES
+#define GPRT_QW_MIN_WAVES 1
+#endif
 template <int M, bool COUNT>
-__global__ __launch_bounds__(256) void knn_qwave_kernel(RoundArgs a) {
+__global__ __launch_bounds__(256, M == 1 ? GPRT_QW_MIN_WAVES : 1) void knn_qwave_kernel(RoundArgs a) {
     const int lane = threadIdx.x & 63;
     const bool plain = a.q_xyz != nullptr;
     const unsigned n_in = plain ? (unsigned)a.nq_plain : a.counters[CNT_ACTIVE + a.round];
