@@ -279,6 +279,7 @@ struct LaneMemHost {
     float* ring_dd;
     std::vector<int2>* spill;
     int spill_depth;
+    bool turn(bool) const { return true; }
     void fetch4(bool scanning, int first, int node, int last, float4* v) const {
         for (int u = 0; u < 4; ++u) {
             const int o = std::min(u, last);

@@ -57,6 +57,20 @@ struct LaneMemDev {
     int tid;
     int2* __restrict__ spill;
     size_t gstride;
+    // Trip vote: with GPRT_TRIP_VOTE a wavefront's trip is taken only by the
+    // lanes of the kind (scanning a leaf range, or visiting a node) that at
+    // least half of its walking lanes want, so each trip runs one of the two
+    // paths instead of both; the others wait for a later trip.
+    __device__ __forceinline__ bool turn(bool scanning) const {
+#ifdef GPRT_TRIP_VOTE
+        const uint64_t all = __ballot(true), sc = __ballot(scanning);
+        const bool scan_turn = 2 * __popcll(sc) >= __popcll(all);
+        return scanning == scan_turn;
+#else
+        (void)scanning;
+        return true;
+#endif
+    }
     __device__ __forceinline__ void fetch4(bool scanning, int first, int node, int last, float4* v) const {
         const float4* src = scanning ? pts + first : nodes4 + 4 * (size_t)node;
 #pragma unroll

@@ -5,6 +5,7 @@
 // HBM spill column of its lane; the host build of the same code
 // (host_walk.cpp) runs it one query at a time with bounds-checked memory, so
 // the walk's indexing is checked on the CPU.  The memory policy `Mem` supplies
+//   turn(scanning)                                   -- whether the lane takes this trip
 //   fetch4(scanning, first_point, node, last, v[4])  -- the trip's 64-byte load
 //   ring_ref(j) / ring_d(j) / ring_put(j, ref, d)     -- LDS ring slot j
 //   spill_get(level) / spill_put(level, int2)         -- HBM spill of the lane
@@ -116,6 +117,10 @@ GPRT_HD int lane_walk(F32Query<C, G>& Q, int2 root, const Mem& m, int max_steps,
     float cdA = INFINITY, cdB = INFINITY;
     int crA = -1, crB = -1;
     while (true) {
+        // m.turn: whether this lane takes the trip (the device build may let
+        // lanes of one kind, node or scan, wait for the others; each lane's
+        // own sequence of trips is the same either way)
+        if (!m.turn(scanning)) continue;
         ++trips;
         const int avail = scanning ? sm - st : 4;
         float4 v[4];
